@@ -1,0 +1,706 @@
+// st_api.hip - C ABI of the tracer (include/soltrace_hip.h): contexts, scenes,
+// single and batched traces. The serving loop lives in st_server.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "st_api_internal.h"
+#include "st_bvh.h"
+
+using namespace st;
+
+// ---------------------------------------------------------------------------
+// errors and fault injection
+namespace st {
+static thread_local std::string g_last_error;
+static std::atomic<int> g_fail_alloc{0}, g_fail_launch{0};
+
+int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(ST_ERR_DEVICE, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+}
+
+bool take_injected(std::atomic<int>& c) {
+    int v = c.load();
+    while (v > 0)
+        if (c.compare_exchange_weak(v, v - 1)) return true;
+    return false;
+}
+
+int dev_alloc(void** p, size_t bytes, const char* what) {
+    *p = nullptr;
+    if (take_injected(g_fail_alloc)) return fail(ST_ERR_INJECTED, "injected allocation failure (%s)", what);
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(e == hipErrorOutOfMemory ? ST_ERR_ALLOC : ST_ERR_DEVICE, "hipMalloc(%zu) for %s: %s", bytes, what,
+                    hipGetErrorString(e));
+    }
+    return ST_OK;
+}
+
+bool launch_injected() { return take_injected(g_fail_launch); }
+
+void DevBuf::reset() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+int DevBuf::ensure(size_t n, const char* what) {
+    if (ptr && bytes >= n) return ST_OK;
+    reset();
+    int rc = dev_alloc(&ptr, n, what);
+    if (rc == ST_OK) bytes = n;
+    return rc;
+}
+
+LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel) {
+    LaunchShape sh;
+    sh.block = block > 0 ? block : 256;
+    const int cus = device_cu_count();
+    sh.grid = grid > 0 ? grid : cus * 8;
+    const uint64_t waves = (uint64_t)sh.grid * (uint64_t)(sh.block / 64);
+    uint64_t chunk = rays / (4 * waves);
+    chunk = (chunk + 63) / 64 * 64;
+    sh.chunk = (int)std::min<uint64_t>(std::max<uint64_t>(chunk, 64), 2048);
+    sh.kernel = kernel == ST_KERNEL_AUTO ? ST_KERNEL_MEGA : kernel;
+    return sh;
+}
+
+}  // namespace st
+
+#define HIP_TRY(expr, what)                            \
+    do {                                               \
+        hipError_t e__ = (expr);                       \
+        if (e__ != hipSuccess) return hip_fail(e__, what); \
+    } while (0)
+
+#define ST_TRY(expr)                 \
+    do {                             \
+        int rc__ = (expr);           \
+        if (rc__ != ST_OK) return rc__; \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+extern "C" int st_abi_version(void) { return ST_ABI_VERSION; }
+extern "C" const char* st_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int st_device_count(int* count) {
+    if (!count) return fail(ST_ERR_INVALID, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        (void)hipGetLastError();
+        return hip_fail(e, "hipGetDeviceCount");
+    }
+    *count = n;
+    return ST_OK;
+}
+
+extern "C" int st_context_create(int device, st_context* out) {
+    if (!out) return fail(ST_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    int n = 0;
+    ST_TRY(st_device_count(&n));
+    if (device < 0 || device >= n) return fail(ST_ERR_INVALID, "device %d out of range (%d devices)", device, n);
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    auto ctx = std::make_unique<st_context_s>();
+    ctx->device = device;
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+    *out = ctx.release();
+    return ST_OK;
+}
+
+extern "C" int st_context_destroy(st_context ctx) {
+    if (!ctx) return fail(ST_ERR_INVALID, "null context");
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+static int validate_elements(const st_element_f64* el, int32_t n) {
+    if (!el) return fail(ST_ERR_INVALID, "null element array");
+    if (n <= 0) return fail(ST_ERR_INVALID, "scene needs at least one element (got %d)", n);
+    if ((uint32_t)n >= kMaxElements) return fail(ST_ERR_INVALID, "too many elements (%d)", n);
+    for (int32_t i = 0; i < n; ++i) {
+        const st_element_f64& e = el[i];
+        if (e.aperture != ST_APERTURE_RECTANGLE)
+            return fail(ST_ERR_INVALID, "element %d: only rectangular apertures are supported", i);
+        if (e.surface != ST_SURFACE_FLAT && e.surface != ST_SURFACE_PARABOLIC && e.surface != ST_SURFACE_CYLINDER)
+            return fail(ST_ERR_INVALID, "element %d: unknown surface type %d", i, e.surface);
+        if (e.role != ST_ROLE_MIRROR && e.role != ST_ROLE_RECEIVER)
+            return fail(ST_ERR_INVALID, "element %d: unknown role %d", i, e.role);
+        if (!(e.dim_x > 0) || !(e.dim_y > 0)) return fail(ST_ERR_INVALID, "element %d: non-positive aperture", i);
+        if (!(e.reflectivity >= 0 && e.reflectivity <= 1))
+            return fail(ST_ERR_INVALID, "element %d: reflectivity outside [0,1]", i);
+        if (!(e.slope_error >= 0) || !(e.spec_error >= 0))
+            return fail(ST_ERR_INVALID, "element %d: negative optical error", i);
+        double d[3] = {e.aim_point[0] - e.origin[0], e.aim_point[1] - e.origin[1], e.aim_point[2] - e.origin[2]};
+        if (!(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > 0))
+            return fail(ST_ERR_INVALID, "element %d: aim point equals origin", i);
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(e.origin[k]) || !std::isfinite(e.aim_point[k]))
+                return fail(ST_ERR_INVALID, "element %d: non-finite coordinates", i);
+    }
+    return ST_OK;
+}
+
+namespace st {
+// frames + refit for the scene's current elements / aim overrides (async on ctx stream)
+int scene_refresh(st_scene s, bool use_overrides) {
+    hipStream_t st = s->ctx->stream;
+    if (launch_injected()) return fail(ST_ERR_INJECTED, "injected launch failure (frames)");
+    HIP_TRY(launch_frames(st, (const st_element_f64*)s->d_elems.ptr, s->n, (const int32_t*)s->d_elem_slot.ptr,
+                          use_overrides ? (const double*)s->d_aim.ptr : nullptr,
+                          use_overrides ? (const double*)s->d_zrot.ptr : nullptr, (double*)s->d_frames.ptr,
+                          (double*)s->d_ref_aabb.ptr, (float*)s->d_bounds.ptr, (float4*)s->d_prims.ptr,
+                          (uint32_t*)s->d_meta.ptr, (float4*)s->d_optics.ptr),
+            "frames kernel");
+    HIP_TRY(launch_refit(st, (const int32_t*)s->d_child.ptr, (const int32_t*)s->d_level_nodes.ptr,
+                         s->level_offsets.data(), (int32_t)s->level_offsets.size() - 1, s->n_nodes, s->n, 1,
+                         (const float*)s->d_bounds.ptr, (float4*)s->d_nodes.ptr),
+            "refit kernel");
+    return ST_OK;
+}
+
+int scene_sun_plane(st_scene s) {
+    if (!s->sun_set) return ST_OK;
+    hipStream_t st = s->ctx->stream;
+    HIP_TRY(hipMemcpyAsync(s->d_sun_unit.ptr, s->sun_unit, sizeof(double) * 3, hipMemcpyHostToDevice, st),
+            "sun vector upload");
+    HIP_TRY(launch_sun_plane(st, (const double*)s->d_ref_aabb.ptr, s->n, 1, (const double*)s->d_sun_unit.ptr,
+                             &s->sun, (SunFrame*)s->d_sun.ptr),
+            "sun plane kernel");
+    SunFrame F;
+    HIP_TRY(hipMemcpyAsync(&F, s->d_sun.ptr, sizeof(F), hipMemcpyDeviceToHost, st), "sun plane download");
+    HIP_TRY(hipStreamSynchronize(st), "sun plane sync");
+    for (int k = 0; k < 3; ++k) {
+        s->plane.v0[k] = F.v0[k];
+        s->plane.v1[k] = F.v1[k];
+        s->plane.v2[k] = F.v2[k];
+        s->plane.v3[k] = F.v3[k];
+    }
+    s->plane.distance = F.distance;
+    s->plane.area = F.area;
+    s->plane.box_half_angle = F.box_half_angle;
+    return ST_OK;
+}
+}  // namespace st
+
+extern "C" int st_scene_create(st_context ctx, const st_element_f64* elements, int32_t n, st_scene* out) {
+    if (!ctx || !out) return fail(ST_ERR_INVALID, "null context or output handle");
+    *out = nullptr;
+    ST_TRY(validate_elements(elements, n));
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    auto s = std::make_unique<st_scene_s>();
+    s->ctx = ctx;
+    s->n = n;
+    s->host_elems.assign(elements, elements + n);
+    // host FP64 bounds -> BVH topology
+    std::vector<float> lo_hi(6 * (size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+        double l2g[9];
+        element_frame(elements[i], elements[i].aim_point, elements[i].zrot_deg, l2g);
+        element_bounds(elements[i], l2g, &lo_hi[6 * i], &lo_hi[6 * i + 3]);
+    }
+    try {
+        s->topo = build_bvh(lo_hi.data(), n, kMaxLeaf);
+    } catch (const std::exception& ex) {
+        return fail(ST_ERR_INVALID, "BVH build failed: %s", ex.what());
+    }
+    s->n_nodes = s->topo.n_nodes();
+    // nodes grouped by level
+    const int32_t levels = s->topo.depth;
+    s->level_offsets.assign(levels + 1, 0);
+    for (int32_t lv : s->topo.node_level) s->level_offsets[lv + 1]++;
+    for (int32_t l = 0; l < levels; ++l) s->level_offsets[l + 1] += s->level_offsets[l];
+    std::vector<int32_t> level_nodes(s->n_nodes), fillp(s->level_offsets.begin(), s->level_offsets.end() - 1);
+    for (int32_t i = 0; i < s->n_nodes; ++i) level_nodes[fillp[s->topo.node_level[i]]++] = i;
+    // device buffers
+    const size_t N = (size_t)n, NN = (size_t)s->n_nodes;
+    ST_TRY(s->d_elems.ensure(sizeof(st_element_f64) * N, "elements"));
+    ST_TRY(s->d_elem_slot.ensure(sizeof(int32_t) * N, "element slots"));
+    ST_TRY(s->d_frames.ensure(sizeof(double) * 9 * N, "frames"));
+    ST_TRY(s->d_ref_aabb.ensure(sizeof(double) * 6 * N, "reference AABBs"));
+    ST_TRY(s->d_bounds.ensure(sizeof(float) * 6 * N, "bounds"));
+    ST_TRY(s->d_prims.ensure(sizeof(float4) * 4 * N, "primitives"));
+    ST_TRY(s->d_meta.ensure(sizeof(uint32_t) * N, "primitive meta"));
+    ST_TRY(s->d_optics.ensure(sizeof(float4) * N, "optics"));
+    ST_TRY(s->d_nodes.ensure(sizeof(float4) * 4 * NN, "BVH nodes"));
+    ST_TRY(s->d_child.ensure(sizeof(int32_t) * 2 * NN, "BVH children"));
+    ST_TRY(s->d_level_nodes.ensure(sizeof(int32_t) * NN, "BVH levels"));
+    ST_TRY(s->d_sun.ensure(sizeof(SunFrame), "sun frame"));
+    ST_TRY(s->d_sun_unit.ensure(sizeof(double) * 3, "sun vector"));
+    ST_TRY(s->d_aim.ensure(sizeof(double) * 3 * N, "aim points"));
+    ST_TRY(s->d_zrot.ensure(sizeof(double) * N, "zrot"));
+    ST_TRY(s->d_counter.ensure(sizeof(uint32_t) * 2, "work counter"));
+    hipStream_t st = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(s->d_elems.ptr, elements, sizeof(st_element_f64) * N, hipMemcpyHostToDevice, st), "upload");
+    HIP_TRY(hipMemcpyAsync(s->d_elem_slot.ptr, s->topo.elem_slot.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st),
+            "upload");
+    HIP_TRY(hipMemcpyAsync(s->d_child.ptr, s->topo.child.data(), sizeof(int32_t) * 2 * NN, hipMemcpyHostToDevice, st),
+            "upload");
+    HIP_TRY(hipMemcpyAsync(s->d_level_nodes.ptr, level_nodes.data(), sizeof(int32_t) * NN, hipMemcpyHostToDevice, st),
+            "upload");
+    HIP_TRY(hipMemsetAsync(s->d_counter.ptr, 0, sizeof(uint32_t) * 2, st), "counter clear");
+    ST_TRY(scene_refresh(s.get(), false));
+    HIP_TRY(hipStreamSynchronize(st), "scene build sync");
+    *out = s.release();
+    return ST_OK;
+}
+
+st_scene_s::~st_scene_s() {
+    if (ctx) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    if (batch_graph_exec) (void)hipGraphExecDestroy(batch_graph_exec);
+}
+
+extern "C" int st_scene_destroy(st_scene scene) {
+    if (!scene) return fail(ST_ERR_INVALID, "null scene");
+    delete scene;
+    return ST_OK;
+}
+
+extern "C" int st_scene_num_elements(st_scene scene, int32_t* n) {
+    if (!scene || !n) return fail(ST_ERR_INVALID, "null argument");
+    *n = scene->n;
+    return ST_OK;
+}
+
+extern "C" int st_scene_update_aim(st_scene s, const double* aim_points, const double* zrot_deg) {
+    if (!s || !aim_points) return fail(ST_ERR_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    for (int32_t i = 0; i < s->n; ++i) {
+        double d[3] = {aim_points[3 * i] - s->host_elems[i].origin[0], aim_points[3 * i + 1] - s->host_elems[i].origin[1],
+                       aim_points[3 * i + 2] - s->host_elems[i].origin[2]};
+        if (!(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > 0)) return fail(ST_ERR_INVALID, "element %d: aim point equals origin", i);
+        for (int k = 0; k < 3; ++k) s->host_elems[i].aim_point[k] = aim_points[3 * i + k];
+        if (zrot_deg) s->host_elems[i].zrot_deg = zrot_deg[i];
+    }
+    hipStream_t st = s->ctx->stream;
+    HIP_TRY(hipMemcpyAsync(s->d_elems.ptr, s->host_elems.data(), sizeof(st_element_f64) * s->n, hipMemcpyHostToDevice, st),
+            "upload");
+    ST_TRY(scene_refresh(s, false));
+    ST_TRY(scene_sun_plane(s));
+    HIP_TRY(hipStreamSynchronize(st), "update sync");
+    return ST_OK;
+}
+
+extern "C" int st_scene_update_elements(st_scene s, const st_element_f64* elements, int32_t n) {
+    if (!s) return fail(ST_ERR_INVALID, "null scene");
+    if (n != s->n) return fail(ST_ERR_INVALID, "element count changed (%d -> %d); create a new scene", s->n, n);
+    ST_TRY(validate_elements(elements, n));
+    for (int32_t i = 0; i < n; ++i)
+        if (elements[i].surface != s->host_elems[i].surface || elements[i].role != s->host_elems[i].role)
+            return fail(ST_ERR_INVALID, "element %d changed type or role; create a new scene", i);
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    s->host_elems.assign(elements, elements + n);
+    hipStream_t st = s->ctx->stream;
+    HIP_TRY(hipMemcpyAsync(s->d_elems.ptr, elements, sizeof(st_element_f64) * n, hipMemcpyHostToDevice, st), "upload");
+    ST_TRY(scene_refresh(s, false));
+    ST_TRY(scene_sun_plane(s));
+    HIP_TRY(hipStreamSynchronize(st), "update sync");
+    return ST_OK;
+}
+
+extern "C" int st_scene_aim_heliostats(st_scene s, const double sun_vector[3], const double target[3]) {
+    if (!s || !sun_vector || !target) return fail(ST_ERR_INVALID, "null argument");
+    double l = std::sqrt(sun_vector[0] * sun_vector[0] + sun_vector[1] * sun_vector[1] + sun_vector[2] * sun_vector[2]);
+    if (!(l > 0)) return fail(ST_ERR_INVALID, "zero sun vector");
+    double su[3] = {sun_vector[0] / l, sun_vector[1] / l, sun_vector[2] / l};
+    for (int32_t i = 0; i < s->n; ++i) {
+        st_element_f64& e = s->host_elems[i];
+        if (e.role != ST_ROLE_MIRROR) continue;
+        bisector_aim_point(e.origin, su, target, e.aim_point);
+    }
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    hipStream_t st = s->ctx->stream;
+    HIP_TRY(hipMemcpyAsync(s->d_elems.ptr, s->host_elems.data(), sizeof(st_element_f64) * s->n, hipMemcpyHostToDevice, st),
+            "upload");
+    ST_TRY(scene_refresh(s, false));
+    ST_TRY(scene_sun_plane(s));
+    HIP_TRY(hipStreamSynchronize(st), "aim sync");
+    return ST_OK;
+}
+
+extern "C" int st_scene_set_sun(st_scene s, const st_sun_desc* sun) {
+    if (!s || !sun) return fail(ST_ERR_INVALID, "null argument");
+    double l = std::sqrt(sun->vector[0] * sun->vector[0] + sun->vector[1] * sun->vector[1] + sun->vector[2] * sun->vector[2]);
+    if (!(l > 0) || !std::isfinite(l)) return fail(ST_ERR_INVALID, "sun vector must be finite and non-zero");
+    if (sun->shape != ST_SUN_PILLBOX && sun->shape != ST_SUN_GAUSSIAN) return fail(ST_ERR_INVALID, "unknown sun shape %d", sun->shape);
+    if (!(sun->param >= 0) || sun->param > 0.2) return fail(ST_ERR_INVALID, "sun angle %g rad outside [0, 0.2]", sun->param);
+    if (sun->position_sampling != ST_POS_PHILOX && sun->position_sampling != ST_POS_HALTON)
+        return fail(ST_ERR_INVALID, "unknown position sampling %d", sun->position_sampling);
+    s->sun = *sun;
+    if (!(s->sun.dni > 0)) s->sun.dni = 1000.0;
+    for (int k = 0; k < 3; ++k) s->sun_unit[k] = sun->vector[k] / l;
+    s->sun_set = true;
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    return scene_sun_plane(s);
+}
+
+extern "C" int st_scene_get_sun_plane(st_scene s, st_sun_plane* out) {
+    if (!s || !out) return fail(ST_ERR_INVALID, "null argument");
+    if (!s->sun_set) return fail(ST_ERR_STATE, "sun not set");
+    *out = s->plane;
+    return ST_OK;
+}
+
+extern "C" int st_scene_get_frames(st_scene s, double* l2g) {
+    if (!s || !l2g) return fail(ST_ERR_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    HIP_TRY(hipMemcpyAsync(l2g, s->d_frames.ptr, sizeof(double) * 9 * s->n, hipMemcpyDeviceToHost, s->ctx->stream), "download");
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream), "sync");
+    return ST_OK;
+}
+
+extern "C" int st_scene_get_aabbs(st_scene s, float* lo_hi) {
+    if (!s || !lo_hi) return fail(ST_ERR_INVALID, "null argument");
+    std::vector<double> tmp(6 * (size_t)s->n);
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    HIP_TRY(hipMemcpyAsync(tmp.data(), s->d_ref_aabb.ptr, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, s->ctx->stream),
+            "download");
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream), "sync");
+    for (size_t i = 0; i < tmp.size(); ++i) lo_hi[i] = (float)tmp[i];
+    return ST_OK;
+}
+
+extern "C" int st_scene_bvh_info(st_scene s, int32_t* n_nodes, int32_t* depth, int32_t* max_leaf) {
+    if (!s) return fail(ST_ERR_INVALID, "null scene");
+    if (n_nodes) *n_nodes = s->n_nodes;
+    if (depth) *depth = s->topo.depth;
+    if (max_leaf) *max_leaf = s->topo.max_leaf;
+    return ST_OK;
+}
+
+extern "C" int st_scene_synchronize(st_scene s) {
+    if (!s) return fail(ST_ERR_INVALID, "null scene");
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream), "sync");
+    return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+namespace st {
+int resolve_flux_element(st_scene s, int32_t requested, int32_t nx, int32_t ny, int32_t* slot) {
+    *slot = -1;
+    if (nx < 0 || ny < 0 || (int64_t)nx * ny > (1 << 26)) return fail(ST_ERR_INVALID, "bad flux grid %dx%d", nx, ny);
+    if (nx == 0 || ny == 0) return ST_OK;
+    int32_t fe = requested;
+    if (fe < 0)
+        for (int32_t i = s->n - 1; i >= 0; --i)
+            if (s->host_elems[i].role == ST_ROLE_RECEIVER) { fe = i; break; }
+    if (fe < 0) return ST_OK;  // no receiver: nothing to bin
+    if (fe >= s->n) return fail(ST_ERR_INVALID, "flux element %d out of range", fe);
+    if (s->host_elems[fe].role != ST_ROLE_RECEIVER) return fail(ST_ERR_INVALID, "flux element %d is not a receiver", fe);
+    *slot = s->topo.elem_slot[fe];
+    return ST_OK;
+}
+
+int fill_job(st_scene s, const st_trace_desc* d, const st_outputs* out, const void* prims, const void* nodes,
+             const void* sun, JobDev* J) {
+    memset(J, 0, sizeof(*J));
+    J->prims = (const float4*)prims;
+    J->meta = (const uint32_t*)s->d_meta.ptr;
+    J->optics = (const float4*)s->d_optics.ptr;
+    J->nodes = (const float4*)nodes;
+    J->sun = (const SunFrame*)sun;
+    J->ray_begin = d->ray_begin;
+    J->ray_count = d->ray_count ? d->ray_count : d->num_rays - d->ray_begin;
+    J->record_base = d->ray_begin;
+    J->seed_lo = (uint32_t)d->seed;
+    J->seed_hi = (uint32_t)(d->seed >> 32);
+    J->job_id = d->job_id;
+    J->max_depth = d->max_depth;
+    J->flags = d->flags;
+    J->flux_nx = d->flux_nx;
+    J->flux_ny = d->flux_ny;
+    ST_TRY(resolve_flux_element(s, d->flux_element, d->flux_nx, d->flux_ny, &J->flux_slot));
+    if (out) {
+        J->flux = J->flux_slot >= 0 ? out->flux : nullptr;
+        J->hits = out->element_hits;
+        J->stats = out->stats;
+        if (d->flags & ST_TRACE_RECORDS) {
+            J->records = (float4*)out->records;
+            J->rec_elem = out->record_elements;
+        }
+    }
+    return ST_OK;
+}
+
+int validate_trace(st_scene s, const st_trace_desc* d, const st_outputs* out) {
+    if (!s || !d) return fail(ST_ERR_INVALID, "null scene or trace descriptor");
+    if (!s->sun_set) return fail(ST_ERR_STATE, "st_scene_set_sun must be called before tracing");
+    if (d->num_rays == 0) return fail(ST_ERR_INVALID, "num_rays must be positive");
+    if (d->ray_begin >= d->num_rays) return fail(ST_ERR_INVALID, "ray_begin beyond num_rays");
+    uint64_t count = d->ray_count ? d->ray_count : d->num_rays - d->ray_begin;
+    if (d->ray_begin + count > d->num_rays) return fail(ST_ERR_INVALID, "ray range exceeds num_rays");
+    if (d->max_depth < 1 || d->max_depth > kMaxDepthLimit)
+        return fail(ST_ERR_INVALID, "max_depth %d outside [1, %d]", d->max_depth, kMaxDepthLimit);
+    if (d->block_size != 0 && d->block_size != 64 && d->block_size != 128 && d->block_size != 256)
+        return fail(ST_ERR_INVALID, "block_size must be 0, 64, 128 or 256");
+    if (d->grid_blocks < 0) return fail(ST_ERR_INVALID, "negative grid_blocks");
+    if (d->kernel < 0 || d->kernel > ST_KERNEL_PERSISTENT) return fail(ST_ERR_INVALID, "unknown kernel %d", d->kernel);
+    if ((d->flags & ST_TRACE_RECORDS) && (!out || !out->records))
+        return fail(ST_ERR_INVALID, "ST_TRACE_RECORDS needs outputs->records");
+    if (d->flux_nx > 0 && d->flux_ny > 0 && (!out || !out->flux))
+        return fail(ST_ERR_INVALID, "flux grid requested without outputs->flux");
+    return ST_OK;
+}
+}  // namespace st
+
+extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* out) {
+    ST_TRY(validate_trace(s, d, out));
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    JobDev J;
+    ST_TRY(fill_job(s, d, out, s->d_prims.ptr, s->d_nodes.ptr, s->d_sun.ptr, &J));
+    hipStream_t st = d->stream ? (hipStream_t)d->stream : s->ctx->stream;
+    if ((d->flags & ST_TRACE_ZERO_OUTPUTS) && out) {
+        if (J.flux) HIP_TRY(launch_fill_u64(st, J.flux, (uint64_t)d->flux_nx * d->flux_ny, 0), "zero flux");
+        if (out->element_hits) HIP_TRY(launch_fill_u64(st, out->element_hits, (uint64_t)s->n, 0), "zero hits");
+        if (out->stats) HIP_TRY(launch_fill_u64(st, out->stats, ST_STAT_COUNT, 0), "zero stats");
+    }
+    const LaunchShape sh = pick_shape(J.ray_count, d->block_size, d->grid_blocks, d->kernel);
+    const uint64_t chunks = (J.ray_count + sh.chunk - 1) / sh.chunk;
+    if (chunks >= (1ull << 32)) return fail(ST_ERR_INVALID, "too many rays for one call");
+    if (launch_injected()) return fail(ST_ERR_INJECTED, "injected launch failure (trace)");
+    HIP_TRY(launch_trace(st, nullptr, nullptr, 1, chunks, (uint32_t*)s->d_counter.ptr, sh, J), "trace kernel");
+    if (!d->stream) HIP_TRY(hipStreamSynchronize(st), "trace sync");
+    return ST_OK;
+}
+
+extern "C" int st_flux_to_watts(st_scene s, const st_trace_desc* d, const int64_t* flux_dev, double* flux_wm2_dev,
+                                double* power_per_ray) {
+    if (!s || !d) return fail(ST_ERR_INVALID, "null argument");
+    if (!s->sun_set) return fail(ST_ERR_STATE, "sun not set");
+    if (d->num_rays == 0) return fail(ST_ERR_INVALID, "num_rays must be positive");
+    int32_t slot = -1;
+    ST_TRY(resolve_flux_element(s, d->flux_element, d->flux_nx, d->flux_ny, &slot));
+    int32_t fe = slot >= 0 ? s->topo.slot_elem[slot] : -1;
+    double bin_area = 1.0;
+    int32_t nbins = 0;
+    if (fe >= 0 && flux_dev && flux_wm2_dev) {
+        const st_element_f64& e = s->host_elems[fe];
+        const double W = e.surface == ST_SURFACE_CYLINDER ? kPi * e.dim_x : e.dim_x;  // 2 pi r, r = w/2
+        bin_area = (W / d->flux_nx) * (e.dim_y / d->flux_ny);
+        nbins = d->flux_nx * d->flux_ny;
+    }
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    hipStream_t st = d->stream ? (hipStream_t)d->stream : s->ctx->stream;
+    double* ppr_dev = nullptr;
+    DevBuf tmp;
+    if (power_per_ray) {
+        ST_TRY(tmp.ensure(sizeof(double), "power per ray"));
+        ppr_dev = (double*)tmp.ptr;
+    }
+    HIP_TRY(launch_flux_watts(st, flux_dev, nbins, (const SunFrame*)s->d_sun.ptr, (double)d->num_rays, bin_area,
+                              nbins ? flux_wm2_dev : nullptr, ppr_dev),
+            "flux normalisation");
+    if (power_per_ray) HIP_TRY(hipMemcpyAsync(power_per_ray, ppr_dev, sizeof(double), hipMemcpyDeviceToHost, st), "download");
+    HIP_TRY(hipStreamSynchronize(st), "sync");
+    return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched sweep
+namespace st {
+static bool same_batch(const BatchCache& c, const st_batch_desc* d) {
+    if (c.steps != d->steps || c.max_depth != d->max_depth || c.seed != d->seed || c.flags != d->flags ||
+        c.flux_element != d->flux_element || c.flux_nx != d->flux_nx || c.flux_ny != d->flux_ny ||
+        c.block != d->block_size || c.grid != d->grid_blocks || c.stream != d->stream)
+        return false;
+    if (memcmp(&c.sun, &d->sun, sizeof(st_sun_desc)) != 0) return false;
+    if (memcmp(c.target, d->target, sizeof(c.target)) != 0) return false;
+    if (c.suns.size() != (size_t)d->steps * 3 || memcmp(c.suns.data(), d->sun_vectors, sizeof(double) * 3 * d->steps) != 0)
+        return false;
+    if (c.budgets.size() != (size_t)d->steps || memcmp(c.budgets.data(), d->budgets, sizeof(uint64_t) * d->steps) != 0)
+        return false;
+    return true;
+}
+}  // namespace st
+
+static int batch_enqueue(st_scene s, hipStream_t st) {
+    BatchCache& b = s->batch;
+    const int32_t K = b.steps, n = s->n;
+    if (b.flux_out) HIP_TRY(launch_fill_u64(st, b.flux_out, (uint64_t)K * b.nbins, 0), "zero flux");
+    if (b.hits_out) HIP_TRY(launch_fill_u64(st, b.hits_out, (uint64_t)K * n, 0), "zero hits");
+    if (b.stats_out) HIP_TRY(launch_fill_u64(st, b.stats_out, (uint64_t)K * ST_STAT_COUNT, 0), "zero stats");
+    HIP_TRY(launch_frames_batch(st, (const st_element_f64*)s->d_elems.ptr, n, (const int32_t*)s->d_elem_slot.ptr, K,
+                                (const double*)b.d_sun_units.ptr, (const double*)b.d_target.ptr,
+                                (double*)b.d_ref_aabb.ptr, (float*)b.d_bounds.ptr, (float4*)b.d_prims.ptr),
+            "batched frames");
+    HIP_TRY(launch_refit(st, (const int32_t*)s->d_child.ptr, (const int32_t*)s->d_level_nodes.ptr,
+                         s->level_offsets.data(), (int32_t)s->level_offsets.size() - 1, s->n_nodes, n, K,
+                         (const float*)b.d_bounds.ptr, (float4*)b.d_nodes.ptr),
+            "batched refit");
+    HIP_TRY(launch_sun_plane(st, (const double*)b.d_ref_aabb.ptr, n, K, (const double*)b.d_sun_units.ptr, &b.sun,
+                             (SunFrame*)b.d_sun.ptr),
+            "batched sun plane");
+    JobDev dummy;
+    memset(&dummy, 0, sizeof(dummy));
+    HIP_TRY(launch_trace(st, (const JobDev*)b.d_jobs.ptr, (const uint64_t*)b.d_prefix.ptr, K, b.total_chunks,
+                         (uint32_t*)b.d_counter.ptr, b.shape, dummy),
+            "batched trace");
+    return ST_OK;
+}
+
+extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux, uint64_t* element_hits,
+                              uint64_t* stats) {
+    if (!s || !d) return fail(ST_ERR_INVALID, "null argument");
+    if (d->steps <= 0 || d->steps > 65535) return fail(ST_ERR_INVALID, "steps %d outside [1, 65535]", d->steps);
+    if (!d->sun_vectors || !d->budgets) return fail(ST_ERR_INVALID, "null sun vectors or budgets");
+    if (d->max_depth < 1 || d->max_depth > kMaxDepthLimit) return fail(ST_ERR_INVALID, "bad max_depth %d", d->max_depth);
+    if (d->block_size != 0 && d->block_size != 64 && d->block_size != 128 && d->block_size != 256)
+        return fail(ST_ERR_INVALID, "block_size must be 0, 64, 128 or 256");
+    if (d->flags & ST_TRACE_RECORDS) return fail(ST_ERR_INVALID, "records are not available in batched sweeps");
+    for (int k = 0; k < d->steps; ++k) {
+        const double* v = d->sun_vectors + 3 * k;
+        if (!(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] > 0)) return fail(ST_ERR_INVALID, "step %d: zero sun vector", k);
+        if (d->budgets[k] == 0) return fail(ST_ERR_INVALID, "step %d: zero ray budget", k);
+    }
+    int32_t fslot = -1;
+    ST_TRY(resolve_flux_element(s, d->flux_element, d->flux_nx, d->flux_ny, &fslot));
+    if (fslot >= 0 && !flux) return fail(ST_ERR_INVALID, "flux grid requested without a flux buffer");
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    hipStream_t st = d->stream ? (hipStream_t)d->stream : s->ctx->stream;
+    BatchCache& b = s->batch;
+    const bool reuse = b.valid && same_batch(b, d) && b.flux_out == flux && b.hits_out == element_hits &&
+                       b.stats_out == stats;
+    if (!reuse) {
+        if (s->batch_graph_exec) {
+            (void)hipGraphExecDestroy(s->batch_graph_exec);
+            s->batch_graph_exec = nullptr;
+        }
+        b.valid = false;
+        const int32_t K = d->steps, n = s->n;
+        b.steps = K;
+        b.max_depth = d->max_depth;
+        b.seed = d->seed;
+        b.flags = d->flags;
+        b.flux_element = d->flux_element;
+        b.flux_nx = d->flux_nx;
+        b.flux_ny = d->flux_ny;
+        b.block = d->block_size;
+        b.grid = d->grid_blocks;
+        b.stream = d->stream;
+        b.sun = d->sun;
+        if (!(b.sun.dni > 0)) b.sun.dni = 1000.0;
+        memcpy(b.target, d->target, sizeof(b.target));
+        b.suns.assign(d->sun_vectors, d->sun_vectors + 3 * (size_t)K);
+        b.budgets.assign(d->budgets, d->budgets + K);
+        b.flux_out = flux;
+        b.hits_out = element_hits;
+        b.stats_out = stats;
+        b.nbins = fslot >= 0 ? (uint64_t)d->flux_nx * d->flux_ny : 0;
+        std::vector<double> units(3 * (size_t)K);
+        for (int k = 0; k < K; ++k) {
+            const double* v = d->sun_vectors + 3 * k;
+            const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            for (int j = 0; j < 3; ++j) units[3 * k + j] = v[j] / l;
+        }
+        ST_TRY(b.d_sun_units.ensure(sizeof(double) * 3 * K, "batch suns"));
+        ST_TRY(b.d_target.ensure(sizeof(double) * 3, "batch target"));
+        ST_TRY(b.d_ref_aabb.ensure(sizeof(double) * 6 * (size_t)n * K, "batch AABBs"));
+        ST_TRY(b.d_bounds.ensure(sizeof(float) * 6 * (size_t)n * K, "batch bounds"));
+        ST_TRY(b.d_prims.ensure(sizeof(float4) * 4 * (size_t)n * K, "batch primitives"));
+        ST_TRY(b.d_nodes.ensure(sizeof(float4) * 4 * (size_t)s->n_nodes * K, "batch nodes"));
+        ST_TRY(b.d_sun.ensure(sizeof(SunFrame) * K, "batch sun frames"));
+        ST_TRY(b.d_jobs.ensure(sizeof(JobDev) * K, "batch jobs"));
+        ST_TRY(b.d_prefix.ensure(sizeof(uint64_t) * (K + 1), "batch chunk prefix"));
+        ST_TRY(b.d_counter.ensure(sizeof(uint32_t) * 2, "batch counter"));
+        uint64_t total_rays = 0;
+        for (int k = 0; k < K; ++k) total_rays += d->budgets[k];
+        b.shape = pick_shape(total_rays / K, d->block_size, d->grid_blocks, ST_KERNEL_MEGA);
+        std::vector<JobDev> jobs(K);
+        std::vector<uint64_t> prefix(K + 1, 0);
+        for (int k = 0; k < K; ++k) {
+            st_trace_desc td;
+            memset(&td, 0, sizeof(td));
+            td.num_rays = d->budgets[k];
+            td.seed = d->seed;
+            td.job_id = (uint32_t)k;
+            td.max_depth = d->max_depth;
+            td.flags = d->flags;
+            td.flux_element = d->flux_element;
+            td.flux_nx = d->flux_nx;
+            td.flux_ny = d->flux_ny;
+            st_outputs o;
+            memset(&o, 0, sizeof(o));
+            o.flux = flux ? flux + (size_t)k * b.nbins : nullptr;
+            o.element_hits = element_hits ? element_hits + (size_t)k * n : nullptr;
+            o.stats = stats ? stats + (size_t)k * ST_STAT_COUNT : nullptr;
+            ST_TRY(fill_job(s, &td, &o, (const float4*)b.d_prims.ptr + (size_t)k * n * 4,
+                            (const float4*)b.d_nodes.ptr + (size_t)k * s->n_nodes * 4,
+                            (const SunFrame*)b.d_sun.ptr + k, &jobs[k]));
+            prefix[k + 1] = prefix[k] + (d->budgets[k] + b.shape.chunk - 1) / b.shape.chunk;
+        }
+        b.total_chunks = prefix[K];
+        if (b.total_chunks >= (1ull << 32)) return fail(ST_ERR_INVALID, "too many rays for one sweep");
+        HIP_TRY(hipMemcpyAsync(b.d_sun_units.ptr, units.data(), sizeof(double) * 3 * K, hipMemcpyHostToDevice, st), "upload");
+        HIP_TRY(hipMemcpyAsync(b.d_target.ptr, d->target, sizeof(double) * 3, hipMemcpyHostToDevice, st), "upload");
+        HIP_TRY(hipMemcpyAsync(b.d_jobs.ptr, jobs.data(), sizeof(JobDev) * K, hipMemcpyHostToDevice, st), "upload");
+        HIP_TRY(hipMemcpyAsync(b.d_prefix.ptr, prefix.data(), sizeof(uint64_t) * (K + 1), hipMemcpyHostToDevice, st),
+                "upload");
+        HIP_TRY(hipMemsetAsync(b.d_counter.ptr, 0, sizeof(uint32_t) * 2, st), "counter clear");
+        HIP_TRY(hipStreamSynchronize(st), "batch setup sync");
+        b.valid = true;
+        if (d->use_graph) {
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+            int rc = batch_enqueue(s, st);
+            hipError_t ce = hipStreamEndCapture(st, &g);
+            if (rc != ST_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (ce != hipSuccess) return hip_fail(ce, "end capture");
+            hipError_t ie = hipGraphInstantiate(&s->batch_graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                s->batch_graph_exec = nullptr;
+                return hip_fail(ie, "graph instantiate");
+            }
+        }
+    }
+    if (d->use_graph && s->batch_graph_exec) {
+        HIP_TRY(hipGraphLaunch(s->batch_graph_exec, st), "graph launch");
+    } else {
+        ST_TRY(batch_enqueue(s, st));
+    }
+    if (!d->stream) HIP_TRY(hipStreamSynchronize(st), "batch sync");
+    return ST_OK;
+}
+
+extern "C" int st_debug_inject_fault(int32_t what, int32_t count) {
+    if (count < 0) return fail(ST_ERR_INVALID, "negative count");
+    if (what == 0) g_fail_alloc.store(count);
+    else if (what == 1) g_fail_launch.store(count);
+    else return fail(ST_ERR_INVALID, "unknown fault kind %d", what);
+    return ST_OK;
+}

@@ -1,0 +1,82 @@
+// st_api_internal.h - handle structs behind the opaque C-ABI handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <vector>
+
+#include "st_bvh.h"
+#include "st_internal.h"
+
+namespace st {
+
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+int dev_alloc(void** p, size_t bytes, const char* what);
+bool launch_injected();
+
+// owning device allocation
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { reset(); }
+    void reset();
+    int ensure(size_t n, const char* what);  // (re)allocate to at least n bytes
+};
+
+LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel);
+
+struct BatchCache {
+    bool valid = false;
+    int32_t steps = 0, max_depth = 0, flux_element = 0, flux_nx = 0, flux_ny = 0, block = 0, grid = 0;
+    uint64_t seed = 0, nbins = 0, total_chunks = 0;
+    uint32_t flags = 0;
+    void* stream = nullptr;
+    st_sun_desc sun{};
+    double target[3] = {0, 0, 0};
+    std::vector<double> suns;
+    std::vector<uint64_t> budgets;
+    int64_t* flux_out = nullptr;
+    uint64_t* hits_out = nullptr;
+    uint64_t* stats_out = nullptr;
+    LaunchShape shape;
+    DevBuf d_sun_units, d_target, d_ref_aabb, d_bounds, d_prims, d_nodes, d_sun, d_jobs, d_prefix, d_counter;
+};
+
+}  // namespace st
+
+struct st_context_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct st_scene_s {
+    st_context ctx = nullptr;
+    int32_t n = 0, n_nodes = 0;
+    std::vector<st_element_f64> host_elems;
+    st::BvhTopology topo;
+    std::vector<int32_t> level_offsets;  // host: nodes of level l are level_nodes[off[l], off[l+1])
+    st::DevBuf d_elems, d_elem_slot, d_frames, d_ref_aabb, d_bounds, d_prims, d_meta, d_optics, d_nodes, d_child,
+        d_level_nodes, d_sun, d_sun_unit, d_aim, d_zrot, d_counter;
+    bool sun_set = false;
+    st_sun_desc sun{};
+    double sun_unit[3] = {0, 0, 1};
+    st_sun_plane plane{};
+    st::BatchCache batch;
+    hipGraphExec_t batch_graph_exec = nullptr;
+    ~st_scene_s();
+};
+
+namespace st {
+int scene_refresh(st_scene s, bool use_overrides);
+int scene_sun_plane(st_scene s);
+int resolve_flux_element(st_scene s, int32_t requested, int32_t nx, int32_t ny, int32_t* slot);
+int fill_job(st_scene s, const st_trace_desc* d, const st_outputs* out, const void* prims, const void* nodes,
+             const void* sun, JobDev* J);
+int validate_trace(st_scene s, const st_trace_desc* d, const st_outputs* out);
+}  // namespace st

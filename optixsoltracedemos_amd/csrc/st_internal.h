@@ -1,0 +1,73 @@
+// st_internal.h - device-side data structures and kernel launchers shared by
+// the C-ABI implementation (st_api.hip) and the kernels (st_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "st_common.h"
+
+namespace st {
+
+// Rays of one trace job (a single st_trace call, one batched sun step or one
+// served request). The trace kernels consume a list of jobs split into
+// fixed-size chunks; every chunk belongs to exactly one job.
+struct JobDev {
+    const float4* prims;     // [n_prims][4] FP32 primitive records, leaf order
+    const uint32_t* meta;    // [n_prims] element index | receiver | kind
+    const float4* optics;    // [n_elements] (reflectivity, slope, spec, 0)
+    const float4* nodes;     // [n_nodes][4]
+    const SunFrame* sun;
+    uint64_t ray_begin;      // first Philox ray index of this job's range
+    uint64_t ray_count;
+    uint64_t record_base;    // records are indexed by ray - record_base
+    uint32_t seed_lo, seed_hi, job_id;
+    int32_t max_depth;
+    uint32_t flags;
+    int32_t flux_slot;       // primitive slot of the flux element (-1 none)
+    int32_t flux_nx, flux_ny;
+    int64_t* flux;
+    uint64_t* hits;          // [n_elements]
+    uint64_t* stats;         // [ST_STAT_COUNT]
+    float4* records;         // [rays][max_depth]
+    uint32_t* rec_elem;      // [rays][max_depth]
+};
+
+struct LaunchShape {
+    int block = 256;     // threads per block (multiple of 64)
+    int grid = 0;        // persistent blocks (0 -> 8 per CU)
+    int chunk = 2048;    // rays per chunk (multiple of 64)
+    int kernel = ST_KERNEL_MEGA;
+};
+
+// ---- launchers (all asynchronous on `stream`) -----------------------------
+// frames + packing for one scene: elements in element order -> prims in slot order
+hipError_t launch_frames(hipStream_t s, const st_element_f64* elems, int32_t n, const int32_t* elem_slot,
+                         const double* aim_override /* n*3 or null */, const double* zrot_override /* n or null */,
+                         double* frames, double* ref_aabb, float* bounds_slot, float4* prims, uint32_t* meta,
+                         float4* optics);
+// batched: per step k re-aim mirrors along the bisector of sun[k] and target
+hipError_t launch_frames_batch(hipStream_t s, const st_element_f64* elems, int32_t n, const int32_t* elem_slot,
+                               int32_t steps, const double* sun_units /* steps*3 */, const double* target,
+                               double* ref_aabb /* steps*n*6 */, float* bounds_slot /* steps*n*6 */,
+                               float4* prims /* steps*n*4 */);
+// refit inner nodes level by level (deepest first); nodes per step are
+// contiguous blocks of n_nodes*4 float4, bounds per step n*6 floats
+hipError_t launch_refit(hipStream_t s, const int32_t* child, const int32_t* level_nodes,
+                        const int32_t* level_offsets_host, int32_t levels, int32_t n_nodes, int32_t n_prims,
+                        int32_t steps, const float* bounds_slot, float4* nodes);
+hipError_t launch_sun_plane(hipStream_t s, const double* ref_aabb, int32_t n, int32_t steps,
+                            const double* sun_units /* steps*3 */, const st_sun_desc* sun, SunFrame* out);
+// jobs == nullptr traces `single` (passed by value); work_counter: 2 x u32,
+// zero before the first launch, left zero by every launch
+hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk_prefix, int32_t n_jobs,
+                        uint64_t total_chunks, uint32_t* work_counter, const LaunchShape& shape,
+                        const JobDev& single);
+hipError_t launch_fill_u64(hipStream_t s, void* p, uint64_t n_u64, uint64_t value);
+hipError_t launch_flux_watts(hipStream_t s, const int64_t* flux, int32_t nbins, const SunFrame* sun,
+                             double num_rays, double bin_area, double* out, double* power_per_ray_dev);
+
+int device_cu_count();
+
+}  // namespace st

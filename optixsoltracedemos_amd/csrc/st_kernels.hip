@@ -1,0 +1,777 @@
+// st_kernels.hip - gfx950 kernels of the SolTrace-style tracer.
+//
+//   k_frames / k_frames_batch  FP64 element frames, reference AABBs, FP32 packing
+//                              (lib/element.cpp:77-275, lib/math_util.h:21-59)
+//   k_refit                    bottom-up BVH refit, one level per launch
+//                              (replaces optixAccelBuild UPDATE, lib/geometry_manager.cpp:249-262)
+//   k_sun_plane                FP64 sun plane (lib/sun_utils.cu:53-179,
+//                              lib/geometry_manager.cpp:97-148)
+//   k_trace                    sun sampling + BVH traversal + intersection +
+//                              shading + flux binning (cuda/sun.cu, cuda/intersection.cu,
+//                              cuda/materials.cu, flux_map_calculator.py:95-179)
+//   k_flux_watts               FP64 power normalisation
+//
+// Trace design: persistent waves pull fixed-size chunks of rays from a global
+// counter; each lane carries one ray through all its bounces with the BVH stack
+// in LDS ([depth][lane], conflict-free); statistics live in registers and are
+// flushed once per chunk with one atomic per wave.
+#include <hip/hip_runtime.h>
+
+#include "st_internal.h"
+
+namespace st {
+
+// ---------------------------------------------------------------------------
+// FP32 vector helpers
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 v3(const float4& q) { return V3{q.x, q.y, q.z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ V3 normalize(V3 a) { return rsqrtf(dot(a, a)) * a; }
+__device__ __forceinline__ V3 fma3(float s, V3 a, V3 b) { return v3(fmaf(s, a.x, b.x), fmaf(s, a.y, b.y), fmaf(s, a.z, b.z)); }
+
+// ---------------------------------------------------------------------------
+// Sampling
+
+// basis of cuda/sun.cu:77-80: u = normalize(cross(|w.x| > 0.99 ? y : x, w)), v = w x u
+__device__ __forceinline__ void cone_basis(V3 w, V3& u, V3& v) {
+    V3 a = fabsf(w.x) > 0.99f ? v3(0.f, 1.f, 0.f) : v3(1.f, 0.f, 0.f);
+    u = normalize(cross(a, w));
+    v = cross(w, u);
+}
+
+// isotropic 2-D Gaussian deviation (polar Box-Muller): theta = sigma sqrt(-2 ln xr)
+__device__ __forceinline__ V3 perturb(V3 a, float sigma, float xr, float xp) {
+    V3 u, v;
+    cone_basis(a, u, v);
+    float th = sigma * sqrtf(-2.f * logf(xr));
+    float st, ct, sp, cp;
+    sincosf(th, &st, &ct);
+    sincospif(2.f * xp, &sp, &cp);
+    return normalize(fma3(ct, a, st * fma3(cp, u, sp * v)));
+}
+
+__device__ __forceinline__ float halton2(uint32_t i) {
+    return (float)((double)__brev(i) * 2.3283064365386962890625e-10);
+}
+__device__ __forceinline__ float halton3(uint32_t i) {
+    uint64_t num = 0, den = 1;
+    while (i > 0) {
+        uint32_t q = i / 3u;
+        num = num * 3u + (i - 3u * q);
+        den *= 3u;
+        i = q;
+    }
+    return den > 1 ? (float)((double)num / (double)den) : 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// Intersections (FP32). Same root selection and bounds rules as the
+// reference programs; the roots are evaluated in cancellation-free form.
+
+// cuda/intersection.cu:47-85
+__device__ __forceinline__ bool isect_flat(const float4& q0, const float4& q1, const float4& q2, const float4& q3,
+                                          V3 o, V3 d, float tmin, float tmax, float& tout) {
+    V3 c = v3(q0), n = v3(q3);
+    V3 oc = c - o;
+    float t = dot(n, oc) * __builtin_amdgcn_rcpf(dot(d, n));
+    if (!(t > tmin && t < tmax)) return false;
+    V3 w = fma3(t, d, -oc);
+    float x = dot(v3(q1), w), y = dot(v3(q2), w);
+    if (!(fabsf(x) <= q0.w && fabsf(y) <= q1.w)) return false;
+    tout = t;
+    return true;
+}
+
+// cuda/intersection.cu:318-448
+__device__ __forceinline__ bool isect_parabolic(const float4& q0, const float4& q1, const float4& q2,
+                                               const float4& q3, V3 o, V3 d, float tmin, float tmax, float& tout) {
+    V3 r = o - v3(q0);
+    V3 e1 = v3(q1), e2 = v3(q2), n = v3(q3);
+    float cx = q2.w, cy = q3.w;
+    float ox = dot(r, e1), oy = dot(r, e2), oz = dot(r, n);
+    float dx = dot(d, e1), dy = dot(d, e2), dz = dot(d, n);
+    float A = 0.5f * cx * dx * dx + 0.5f * cy * dy * dy;
+    float B = cx * ox * dx + cy * oy * dy - dz;
+    float C = 0.5f * cx * ox * ox + 0.5f * cy * oy * oy - oz;
+    float t = 0.f;
+    bool valid = false;
+    if (fabsf(A) < 1e-12f) {
+        t = -C / B;
+        valid = t > 0.f;
+    } else {
+        float disc = B * B - 4.f * A * C;
+        if (disc >= 0.f) {
+            float sq = sqrtf(disc);
+            float qq = B >= 0.f ? -0.5f * (B + sq) : -0.5f * (B - sq);
+            float ra = qq / A, rb = qq != 0.f ? C / qq : ra;
+            float t1 = B >= 0.f ? ra : rb;
+            float t2 = B >= 0.f ? rb : ra;
+            if (t1 > 0.f && t1 < t2) {
+                t = t1;
+                valid = true;
+            } else if (t2 > 0.f) {
+                t = t2;
+                valid = true;
+            }
+        }
+    }
+    if (!valid || t < tmin || t > tmax) return false;
+    float x = fmaf(t, dx, ox), y = fmaf(t, dy, oy);
+    if (x < -q0.w || x > q0.w || y < -q1.w || y > q1.w) return false;
+    tout = t;
+    return true;
+}
+
+// cuda/intersection.cu:177-296 (capped cylinder, axis = local y)
+__device__ __forceinline__ bool isect_cylinder(const float4& q0, const float4& q1, const float4& q2,
+                                              const float4& q3, V3 o, V3 d, float tmin, float tmax, float& tout,
+                                              int& cap) {
+    V3 r = o - v3(q0);
+    V3 ax = v3(q1), ay = v3(q2), az = v3(q3);
+    float rad = q0.w, hh = q1.w;
+    float ox = dot(r, ax), oy = dot(r, ay), oz = dot(r, az);
+    float dx = dot(d, ax), dy = dot(d, ay), dz = dot(d, az);
+    float tbig = tmax + 1.f;
+    float t_wall = tbig;
+    float A = dx * dx + dz * dz;
+    if (A > 0.f) {
+        float tc = -(ox * dx + oz * dz) / A;
+        float px = fmaf(tc, dx, ox), pz = fmaf(tc, dz, oz);
+        float h2 = rad * rad - (px * px + pz * pz);
+        if (h2 >= 0.f) {
+            float dt = sqrtf(h2 / A);
+            float t1 = tc - dt, t2 = tc + dt;
+            if (t1 > tmin && t1 < tmax && fabsf(fmaf(t1, dy, oy)) <= hh) t_wall = t1;
+            else if (t2 > tmin && t2 < tmax && fabsf(fmaf(t2, dy, oy)) <= hh) t_wall = t2;
+        }
+    }
+    float t_cap = tbig;
+    if (fabsf(dy) > 1e-6f) {
+        float inv = 1.f / dy;
+        float tb = (-hh - oy) * inv, tt = (hh - oy) * inv;
+        float bx = fmaf(tb, dx, ox), bz = fmaf(tb, dz, oz);
+        if (tb > tmin && tb < tmax && bx * bx + bz * bz <= rad * rad) t_cap = tb;
+        float cx = fmaf(tt, dx, ox), cz = fmaf(tt, dz, oz);
+        if (tt > tmin && tt < tmax && cx * cx + cz * cz <= rad * rad) t_cap = fminf(t_cap, tt);
+    }
+    float t = fminf(t_wall, t_cap);
+    if (t >= tmax || t <= tmin) return false;
+    cap = !(t == t_wall);
+    tout = t;
+    return true;
+}
+
+__device__ __forceinline__ bool isect_prim(const float4* __restrict__ prims, uint32_t kind, int slot, V3 o, V3 d,
+                                          float tmin, float tmax, float& t, int& cap) {
+    const float4 q0 = prims[4 * slot + 0], q1 = prims[4 * slot + 1], q2 = prims[4 * slot + 2],
+                 q3 = prims[4 * slot + 3];
+    cap = 0;
+    if (kind == kKindFlat) return isect_flat(q0, q1, q2, q3, o, d, tmin, tmax, t);
+    if (kind == kKindParabolic) return isect_parabolic(q0, q1, q2, q3, o, d, tmin, tmax, t);
+    return isect_cylinder(q0, q1, q2, q3, o, d, tmin, tmax, t, cap);
+}
+
+// ---------------------------------------------------------------------------
+// Closest hit over the BVH: smallest t, equal t -> lower element index.
+__device__ __forceinline__ float next_up(float t) { return __int_as_float(__float_as_int(t) + 1); }
+
+template <int STRIDE>
+__device__ __forceinline__ void traverse(const float4* __restrict__ nodes, const float4* __restrict__ prims,
+                                         const uint32_t* __restrict__ meta, V3 o, V3 d, float tmin, float& tbest,
+                                         int& bslot, int& bcap, int32_t* __restrict__ stk) {
+    const float ix = 1.f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x));
+    const float iy = 1.f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y));
+    const float iz = 1.f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
+    const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
+    int sp = 0;
+    int32_t node = 0;
+    for (;;) {
+        while (node >= 0) {
+            const float4 n0 = nodes[4 * node + 0], n1 = nodes[4 * node + 1], n2 = nodes[4 * node + 2],
+                         n3 = nodes[4 * node + 3];
+            float a0 = fmaf(n0.x, ix, -ox), a1 = fmaf(n0.y, ix, -ox);
+            float b0 = fmaf(n0.z, iy, -oy), b1 = fmaf(n0.w, iy, -oy);
+            float c0 = fmaf(n2.x, iz, -oz), c1 = fmaf(n2.y, iz, -oz);
+            float lo0 = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fmaxf(fminf(c0, c1), tmin));
+            float hi0 = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fminf(fmaxf(c0, c1), tbest));
+            float e0 = fmaf(n1.x, ix, -ox), e1 = fmaf(n1.y, ix, -ox);
+            float f0 = fmaf(n1.z, iy, -oy), f1 = fmaf(n1.w, iy, -oy);
+            float g0 = fmaf(n2.z, iz, -oz), g1 = fmaf(n2.w, iz, -oz);
+            float lo1 = fmaxf(fmaxf(fminf(e0, e1), fminf(f0, f1)), fmaxf(fminf(g0, g1), tmin));
+            float hi1 = fminf(fminf(fmaxf(e0, e1), fmaxf(f0, f1)), fminf(fmaxf(g0, g1), tbest));
+            const bool h0 = lo0 <= hi0, h1 = lo1 <= hi1;
+            const int32_t ch0 = __float_as_int(n3.x), ch1 = __float_as_int(n3.y);
+            if (h0 && h1) {
+                const bool swap = lo1 < lo0;
+                stk[sp * STRIDE] = swap ? ch0 : ch1;
+                ++sp;
+                node = swap ? ch1 : ch0;
+            } else if (h0) {
+                node = ch0;
+            } else if (h1) {
+                node = ch1;
+            } else {
+                if (sp == 0) return;
+                node = stk[--sp * STRIDE];
+            }
+        }
+        const int32_t start = leaf_start(node), cnt = leaf_count(node);
+        for (int k = 0; k < cnt; ++k) {
+            const int slot = start + k;
+            const uint32_t m = meta[slot];
+            float t;
+            int cap;
+            const float lim = bslot >= 0 ? next_up(tbest) : tbest;
+            if (isect_prim(prims, meta_kind(m), slot, o, d, tmin, lim, t, cap)) {
+                if (bslot < 0 || t < tbest || (t == tbest && meta_elem(m) < meta_elem(meta[bslot]))) {
+                    tbest = t;
+                    bslot = slot;
+                    bcap = cap;
+                }
+            }
+        }
+        if (sp == 0) return;
+        node = stk[--sp * STRIDE];
+    }
+}
+
+// ---------------------------------------------------------------------------
+struct Counters {
+    uint32_t c[12];  // ST_STAT_* 0..10, [11] = receiver hits on the flux element
+};
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void record(const JobDev& J, uint64_t ray, int slot, float tag, V3 p, uint32_t elem) {
+    if (J.records) {
+        const uint64_t i = (ray - J.record_base) * (uint64_t)J.max_depth + (uint64_t)slot;
+        J.records[i] = make_float4(tag, p.x, p.y, p.z);
+        if (J.rec_elem) J.rec_elem[i] = elem;
+    }
+}
+
+template <int STRIDE>
+__device__ __forceinline__ void trace_ray(const JobDev& J, const SunFrame& S, uint64_t local, int32_t* stk,
+                                          Counters& cnt) {
+    const uint64_t ray = J.ray_begin + local;
+    const uint32_t rlo = (uint32_t)ray, rhi = (uint32_t)(ray >> 32);
+    // ---- sun sample: cuda/sun.cu:94-134
+    const U4 u = philox4x32_10(U4{rlo, rhi, 0u, J.job_id}, J.seed_lo, J.seed_hi);
+    float a, b;
+    if (S.position_sampling == ST_POS_HALTON) {
+        a = halton2(rlo);
+        b = halton3(rlo);
+    } else {
+        a = u01(u.x);
+        b = u01(u.y);
+    }
+    V3 o = v3(fmaf(a, S.e1[0], fmaf(b, S.e2[0], S.o[0])), fmaf(a, S.e1[1], fmaf(b, S.e2[1], S.o[1])),
+              fmaf(a, S.e1[2], fmaf(b, S.e2[2], S.o[2])));
+    V3 d;
+    {
+        const V3 w = v3(S.w[0], S.w[1], S.w[2]), su = v3(S.u[0], S.u[1], S.u[2]), sv = v3(S.v[0], S.v[1], S.v[2]);
+        float sp, cp, st, ct;
+        sincospif(2.f * u01(u.z), &sp, &cp);
+        if (S.shape == ST_SUN_GAUSSIAN) {
+            sincosf(S.sigma * sqrtf(-2.f * logf(u01(u.w))), &st, &ct);
+        } else {
+            const float s = S.vers * (1.f - u01(u.w));
+            ct = 1.f - s;
+            st = sqrtf(s * (2.f - s));
+        }
+        d = normalize(fma3(ct, w, st * fma3(cp, su, sp * sv)));
+    }
+    cnt.c[ST_STAT_RAYS]++;
+    record(J, ray, 0, 0.f, o, 0xFFFFFFFEu);
+    int depth = 0, last = 0;
+    float tmin = kTminPrimary;
+    for (;;) {
+        cnt.c[ST_STAT_SEGMENTS]++;
+        float t = kTmax;
+        int slot = -1, cap = 0;
+        traverse<STRIDE>(J.nodes, J.prims, J.meta, o, d, tmin, t, slot, cap, stk);
+        if (slot < 0) {  // __miss__ms, cuda/materials.cu:236-253
+            cnt.c[ST_STAT_ESCAPED]++;
+            break;
+        }
+        const int nd = depth + 1;
+        const uint32_t m = J.meta[slot];
+        const uint32_t kind = meta_kind(m), elem = meta_elem(m);
+        const float4 q0 = J.prims[4 * slot + 0], q1 = J.prims[4 * slot + 1], q2 = J.prims[4 * slot + 2],
+                     q3 = J.prims[4 * slot + 3];
+        const V3 p = fma3(t, d, o);
+        const V3 r = o - v3(q0);
+        if (meta_receiver(m)) {
+            // cuda/materials.cu:86-121 (flat: front face only) and :123-163 (cylinder)
+            if (kind == kKindFlat && !(dot(d, v3(q3)) < 0.f)) {
+                cnt.c[ST_STAT_BACK_FACE]++;
+                break;
+            }
+            if (nd < J.max_depth) {
+                record(J, ray, nd, 2.f, p, elem);
+                last = nd;
+                const bool fe = slot == J.flux_slot;
+                if (fe) cnt.c[11]++;
+                else if (J.hits) atomicAdd((unsigned long long*)&J.hits[elem], 1ull);
+                cnt.c[ST_STAT_RECEIVER_HITS]++;
+                const bool direct = depth == 0;
+                cnt.c[ST_STAT_DIRECT_SUN] += direct;
+                cnt.c[ST_STAT_CAP_HITS] += cap;
+                if (fe && J.flux && !cap && (!direct || (J.flags & ST_TRACE_FLUX_DIRECT_SUN))) {
+                    // scripts/post_processing/flux_map_calculator.py:95-179
+                    const float lx = fmaf(t, dot(d, v3(q1)), dot(r, v3(q1)));
+                    const float ly = fmaf(t, dot(d, v3(q2)), dot(r, v3(q2)));
+                    float fx, fy;
+                    if (kind == kKindCylinder) {
+                        const float lz = fmaf(t, dot(d, v3(q3)), dot(r, v3(q3)));
+                        float th = atan2f(-lz, lx);
+                        if (th < 0.f) th += 6.283185307179586f;
+                        fx = th * (float)J.flux_nx * 0.15915494309189535f;
+                        fy = (ly + q1.w) * (float)J.flux_ny / (2.f * q1.w);
+                    } else {
+                        fx = (lx + q0.w) * (float)J.flux_nx / (2.f * q0.w);
+                        fy = (ly + q1.w) * (float)J.flux_ny / (2.f * q1.w);
+                    }
+                    int ix = (int)floorf(fx), iy = (int)floorf(fy);
+                    ix = min(max(ix, 0), J.flux_nx - 1);
+                    iy = min(max(iy, 0), J.flux_ny - 1);
+                    atomicAdd((unsigned long long*)&J.flux[iy * J.flux_nx + ix], 1ull);
+                    cnt.c[ST_STAT_FLUX_BINNED]++;
+                }
+            } else {
+                cnt.c[ST_STAT_TRUNCATED]++;
+            }
+            break;
+        }
+        // mirror: cuda/materials.cu:26-84, 170-231
+        if (nd >= J.max_depth) {
+            cnt.c[ST_STAT_TRUNCATED]++;
+            break;
+        }
+        record(J, ray, nd, 1.f, p, elem);
+        last = nd;
+        if (J.hits) atomicAdd((unsigned long long*)&J.hits[elem], 1ull);
+        cnt.c[ST_STAT_MIRROR_HITS]++;
+        const float4 op = J.optics[elem];
+        if (op.x < 1.f) {
+            const U4 ur = philox4x32_10(U4{rlo, rhi, 2u * (uint32_t)nd, J.job_id}, J.seed_lo, J.seed_hi);
+            if (u01(ur.x) >= op.x) {
+                cnt.c[ST_STAT_ABSORBED]++;
+                break;
+            }
+        }
+        V3 n;
+        if (kind == kKindFlat) {
+            n = v3(q3);
+        } else {
+            const V3 ax = v3(q1), ay = v3(q2), az = v3(q3);
+            const float lx = fmaf(t, dot(d, ax), dot(r, ax)), ly = fmaf(t, dot(d, ay), dot(r, ay));
+            V3 nl;
+            if (kind == kKindParabolic) {
+                nl = normalize(v3(-q2.w * lx, -q3.w * ly, 1.f));
+            } else {
+                const float lz = fmaf(t, dot(d, az), dot(r, az));
+                nl = cap ? v3(0.f, ly < 0.f ? -1.f : 1.f, 0.f) : normalize(v3(lx, 0.f, lz));
+            }
+            n = normalize(fma3(nl.x, ax, fma3(nl.y, ay, nl.z * az)));
+        }
+        if (dot(d, n) > 0.f) n = -n;  // faceforward(n, -d, n)
+        V3 rd;
+        if (op.y > 0.f || op.z > 0.f) {
+            const U4 ue = philox4x32_10(U4{rlo, rhi, 2u * (uint32_t)nd - 1u, J.job_id}, J.seed_lo, J.seed_hi);
+            if (op.y > 0.f) n = perturb(n, op.y, u01(ue.x), u01(ue.y));
+            rd = normalize(fma3(-2.f * dot(n, d), n, d));
+            if (op.z > 0.f) rd = perturb(rd, op.z, u01(ue.z), u01(ue.w));
+        } else {
+            rd = normalize(fma3(-2.f * dot(n, d), n, d));
+        }
+        o = p;
+        d = rd;
+        tmin = kTminBounce;
+        depth = nd;
+    }
+    // empty slots are all-zero (the reference memsets the buffer, lib/soltrace_system.cpp:125)
+    if (J.records)
+        for (int k = last + 1; k < J.max_depth; ++k) record(J, ray, k, 0.f, v3(0.f, 0.f, 0.f), 0xFFFFFFFFu);
+}
+
+// work_counter[0] hands out chunks, work_counter[1] counts waves that ran out
+// of work; the last wave to leave resets both to zero, so the next launch on
+// the stream starts from a clean counter without a memset (graph-safe).
+// jobs == nullptr: a single job passed by value in `single`.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace(const JobDev* __restrict__ jobs,
+                                                 const uint64_t* __restrict__ chunk_prefix, int32_t n_jobs,
+                                                 uint64_t total_chunks, uint32_t* __restrict__ work_counter,
+                                                 int32_t chunk_rays, JobDev single) {
+    __shared__ int32_t lds_stack[kStackDepth * BLOCK];
+    int32_t* stk = lds_stack + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(&work_counter[0], 1u);
+        c = __builtin_amdgcn_readfirstlane(c);
+        if ((uint64_t)c >= total_chunks) break;
+        int lo = 0;
+        uint64_t base = 0;
+        if (jobs) {
+            int hi = n_jobs;  // chunk_prefix[lo] <= c < chunk_prefix[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (chunk_prefix[mid] <= (uint64_t)c) lo = mid;
+                else hi = mid;
+            }
+            base = chunk_prefix[lo];
+        }
+        const JobDev& J = jobs ? jobs[lo] : single;
+        const SunFrame& S = *J.sun;
+        const uint64_t first = ((uint64_t)c - base) * (uint64_t)chunk_rays;
+        const uint64_t end = min(first + (uint64_t)chunk_rays, J.ray_count);
+        Counters cnt;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) cnt.c[k] = 0;
+        for (uint64_t i = first + lane; i < end; i += 64) trace_ray<BLOCK>(J, S, i, stk, cnt);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const uint32_t v = wave_sum(cnt.c[k]);
+            if (lane == 0 && v) {
+                if (k < 11) {
+                    if (J.stats) atomicAdd((unsigned long long*)&J.stats[k], (unsigned long long)v);
+                } else if (J.hits && J.flux_slot >= 0) {
+                    atomicAdd((unsigned long long*)&J.hits[meta_elem(J.meta[J.flux_slot])], (unsigned long long)v);
+                }
+            }
+        }
+    }
+    if (lane == 0) {
+        __threadfence();
+        const uint32_t waves = gridDim.x * (BLOCK / 64);
+        if (atomicAdd(&work_counter[1], 1u) == waves - 1u) {
+            atomicExch(&work_counter[0], 0u);
+            atomicExch(&work_counter[1], 0u);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_frames(const st_element_f64* __restrict__ elems, int32_t n, const int32_t* __restrict__ elem_slot,
+                         const double* __restrict__ aim_override, const double* __restrict__ zrot_override,
+                         double* __restrict__ frames, double* __restrict__ ref_aabb, float* __restrict__ bounds_slot,
+                         float4* __restrict__ prims, uint32_t* __restrict__ meta, float4* __restrict__ optics) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const st_element_f64 E = elems[e];
+    double aim[3] = {E.aim_point[0], E.aim_point[1], E.aim_point[2]};
+    if (aim_override) for (int k = 0; k < 3; ++k) aim[k] = aim_override[3 * e + k];
+    const double zrot = zrot_override ? zrot_override[e] : E.zrot_deg;
+    double l2g[9];
+    element_frame(E, aim, zrot, l2g);
+    if (frames) for (int k = 0; k < 9; ++k) frames[9 * (size_t)e + k] = l2g[k];
+    double lo[3], hi[3];
+    element_ref_aabb(E, l2g, lo, hi);
+    for (int k = 0; k < 3; ++k) {
+        ref_aabb[6 * (size_t)e + k] = lo[k];
+        ref_aabb[6 * (size_t)e + 3 + k] = hi[k];
+    }
+    const int slot = elem_slot[e];
+    float blo[3], bhi[3];
+    element_bounds(E, l2g, blo, bhi);
+    for (int k = 0; k < 3; ++k) {
+        bounds_slot[6 * (size_t)slot + k] = blo[k];
+        bounds_slot[6 * (size_t)slot + 3 + k] = bhi[k];
+    }
+    float4 q[4];
+    element_pack(E, l2g, q);
+    for (int k = 0; k < 4; ++k) prims[4 * (size_t)slot + k] = q[k];
+    meta[slot] = meta_pack((uint32_t)e, element_kind(E), E.role == ST_ROLE_RECEIVER ? 1u : 0u);
+    optics[e] = make_float4((float)E.reflectivity, (float)E.slope_error, (float)E.spec_error, 0.f);
+}
+
+__global__ void k_frames_batch(const st_element_f64* __restrict__ elems, int32_t n,
+                               const int32_t* __restrict__ elem_slot, int32_t steps,
+                               const double* __restrict__ sun_units, const double* __restrict__ target,
+                               double* __restrict__ ref_aabb, float* __restrict__ bounds_slot,
+                               float4* __restrict__ prims) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)n * steps) return;
+    const int k = (int)(g / n), e = (int)(g % n);
+    const st_element_f64 E = elems[e];
+    double aim[3] = {E.aim_point[0], E.aim_point[1], E.aim_point[2]};
+    if (E.role == ST_ROLE_MIRROR) {
+        const double s[3] = {sun_units[3 * k], sun_units[3 * k + 1], sun_units[3 * k + 2]};
+        bisector_aim_point(E.origin, s, target, aim);
+    }
+    double l2g[9];
+    element_frame(E, aim, E.zrot_deg, l2g);
+    double lo[3], hi[3];
+    element_ref_aabb(E, l2g, lo, hi);
+    double* ra = ref_aabb + ((size_t)k * n + e) * 6;
+    for (int j = 0; j < 3; ++j) {
+        ra[j] = lo[j];
+        ra[3 + j] = hi[j];
+    }
+    const int slot = elem_slot[e];
+    float blo[3], bhi[3];
+    element_bounds(E, l2g, blo, bhi);
+    float* bs = bounds_slot + ((size_t)k * n + slot) * 6;
+    for (int j = 0; j < 3; ++j) {
+        bs[j] = blo[j];
+        bs[3 + j] = bhi[j];
+    }
+    float4 q[4];
+    element_pack(E, l2g, q);
+    float4* pq = prims + ((size_t)k * n + slot) * 4;
+    for (int j = 0; j < 4; ++j) pq[j] = q[j];
+}
+
+__device__ __forceinline__ void child_box(int32_t code, const float4* __restrict__ nodes,
+                                          const float* __restrict__ bounds, float lo[3], float hi[3]) {
+    if (code >= 0) {
+        const float4 n0 = nodes[4 * code + 0], n1 = nodes[4 * code + 1], n2 = nodes[4 * code + 2];
+        lo[0] = fminf(n0.x, n1.x);
+        hi[0] = fmaxf(n0.y, n1.y);
+        lo[1] = fminf(n0.z, n1.z);
+        hi[1] = fmaxf(n0.w, n1.w);
+        lo[2] = fminf(n2.x, n2.z);
+        hi[2] = fmaxf(n2.y, n2.w);
+    } else {
+        const int32_t s = leaf_start(code), c = leaf_count(code);
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = 3.0e38f;
+            hi[k] = -3.0e38f;
+        }
+        for (int i = 0; i < c; ++i)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = fminf(lo[k], bounds[6 * (s + i) + k]);
+                hi[k] = fmaxf(hi[k], bounds[6 * (s + i) + 3 + k]);
+            }
+    }
+}
+
+__global__ void k_refit(const int32_t* __restrict__ child, const int32_t* __restrict__ level_nodes, int32_t begin,
+                        int32_t count, int32_t n_nodes, int32_t n_prims, int32_t steps,
+                        const float* __restrict__ bounds_slot, float4* __restrict__ nodes) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)count * steps) return;
+    const int k = (int)(g / count), i = (int)(g % count);
+    const int32_t node = level_nodes[begin + i];
+    float4* N = nodes + (size_t)k * n_nodes * 4;
+    const float* B = bounds_slot + (size_t)k * n_prims * 6;
+    const int32_t c0 = child[2 * node], c1 = child[2 * node + 1];
+    float l0[3], h0[3], l1[3], h1[3];
+    child_box(c0, N, B, l0, h0);
+    child_box(c1, N, B, l1, h1);
+    N[4 * node + 0] = make_float4(l0[0], h0[0], l0[1], h0[1]);
+    N[4 * node + 1] = make_float4(l1[0], h1[0], l1[1], h1[1]);
+    N[4 * node + 2] = make_float4(l0[2], h0[2], l1[2], h1[2]);
+    N[4 * node + 3] = make_float4(__int_as_float(c0), __int_as_float(c1), 0.f, 0.f);
+}
+
+// FP64 sun plane, one block per step (lib/sun_utils.cu:53-179 + geometry_manager.cpp:97-148)
+__global__ __launch_bounds__(256) void k_sun_plane(const double* __restrict__ ref_aabb, int32_t n,
+                                                   const double* __restrict__ sun_units, st_sun_desc sun,
+                                                   SunFrame* __restrict__ out) {
+    __shared__ double red[5][256];
+    const int k = blockIdx.x, tid = threadIdx.x;
+    const double s[3] = {sun_units[3 * k], sun_units[3 * k + 1], sun_units[3 * k + 2]};
+    const double angle = sun.shape == ST_SUN_GAUSSIAN ? 4.0 * sun.param : sun.param;
+    const double tana = tan(angle);
+    double ax[3] = {fabs(s[0]) < 0.9 ? 1.0 : 0.0, fabs(s[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+    double u[3] = {ax[1] * s[2] - ax[2] * s[1], ax[2] * s[0] - ax[0] * s[2], ax[0] * s[1] - ax[1] * s[0]};
+    double lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    for (int j = 0; j < 3; ++j) u[j] /= lu;
+    double v[3] = {s[1] * u[2] - s[2] * u[1], s[2] * u[0] - s[0] * u[2], s[0] * u[1] - s[1] * u[0]};
+    double lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (int j = 0; j < 3; ++j) v[j] /= lv;
+    double dmax = 0.0, umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300;
+    const double* A = ref_aabb + (size_t)k * n * 6;
+    for (int e = tid; e < n; e += blockDim.x) {
+        for (int c = 0; c < 8; ++c) {
+            const double p[3] = {A[6 * e + ((c & 1) ? 3 : 0)], A[6 * e + 1 + ((c & 2) ? 3 : 0)],
+                                 A[6 * e + 2 + ((c & 4) ? 3 : 0)]};
+            const double ps = fabs(p[0] * s[0] + p[1] * s[1] + p[2] * s[2]);
+            dmax = fmax(dmax, ps);
+            const double buf = ps * tana;
+            const double pu = p[0] * u[0] + p[1] * u[1] + p[2] * u[2];
+            const double pv = p[0] * v[0] + p[1] * v[1] + p[2] * v[2];
+            umin = fmin(umin, pu - buf);
+            umax = fmax(umax, pu + buf);
+            vmin = fmin(vmin, pv - buf);
+            vmax = fmax(vmax, pv + buf);
+        }
+    }
+    red[0][tid] = dmax;
+    red[1][tid] = umin;
+    red[2][tid] = -umax;
+    red[3][tid] = vmin;
+    red[4][tid] = -vmax;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (tid < st) {
+            red[0][tid] = fmax(red[0][tid], red[0][tid + st]);
+            for (int j = 1; j < 5; ++j) red[j][tid] = fmin(red[j][tid], red[j][tid + st]);
+        }
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    const double d = red[0][0];
+    umin = red[1][0];
+    umax = -red[2][0];
+    vmin = red[3][0];
+    vmax = -red[4][0];
+    SunFrame F;
+    for (int j = 0; j < 3; ++j) {
+        F.v0[j] = umin * u[j] + vmin * v[j] + d * s[j];
+        F.v1[j] = umax * u[j] + vmin * v[j] + d * s[j];
+        F.v2[j] = umax * u[j] + vmax * v[j] + d * s[j];
+        F.v3[j] = umin * u[j] + vmax * v[j] + d * s[j];
+    }
+    double e1[3], e2[3];
+    for (int j = 0; j < 3; ++j) {
+        e1[j] = F.v1[j] - F.v0[j];
+        e2[j] = F.v3[j] - F.v0[j];
+    }
+    F.distance = d;
+    F.area = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]) * sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+    F.box_half_angle = angle;
+    F.dni = sun.dni > 0 ? sun.dni : 1000.0;
+    // sampling basis around w = -s (cuda/sun.cu:76-80), FP64 then rounded
+    const double w[3] = {-s[0], -s[1], -s[2]};
+    const double a2[3] = {fabs(w[0]) > 0.99 ? 0.0 : 1.0, fabs(w[0]) > 0.99 ? 1.0 : 0.0, 0.0};
+    double su[3] = {a2[1] * w[2] - a2[2] * w[1], a2[2] * w[0] - a2[0] * w[2], a2[0] * w[1] - a2[1] * w[0]};
+    const double lsu = sqrt(su[0] * su[0] + su[1] * su[1] + su[2] * su[2]);
+    for (int j = 0; j < 3; ++j) su[j] /= lsu;
+    const double sv[3] = {w[1] * su[2] - w[2] * su[1], w[2] * su[0] - w[0] * su[2], w[0] * su[1] - w[1] * su[0]};
+    for (int j = 0; j < 3; ++j) {
+        F.o[j] = (float)F.v0[j];
+        F.e1[j] = (float)e1[j];
+        F.e2[j] = (float)e2[j];
+        F.w[j] = (float)w[j];
+        F.u[j] = (float)su[j];
+        F.v[j] = (float)sv[j];
+    }
+    const double hs = sin(0.5 * sun.param);
+    F.vers = (float)(2.0 * hs * hs);
+    F.sigma = (float)sun.param;
+    F.shape = sun.shape;
+    F.position_sampling = sun.position_sampling;
+    out[k] = F;
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t value) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+__global__ void k_flux_watts(const int64_t* __restrict__ flux, int32_t nbins, const SunFrame* __restrict__ sun,
+                             double num_rays, double bin_area, double* __restrict__ out, double* ppr) {
+    const double power_per_ray = sun->dni * sun->area / num_rays;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && ppr) *ppr = power_per_ray;
+    if (i < nbins && out) out[i] = (double)flux[i] * power_per_ray / bin_area;
+}
+
+// ---------------------------------------------------------------------------
+int device_cu_count() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) return 256;
+    return cus;
+}
+
+hipError_t launch_frames(hipStream_t s, const st_element_f64* elems, int32_t n, const int32_t* elem_slot,
+                         const double* aim_override, const double* zrot_override, double* frames, double* ref_aabb,
+                         float* bounds_slot, float4* prims, uint32_t* meta, float4* optics) {
+    hipLaunchKernelGGL(k_frames, dim3((n + 127) / 128), dim3(128), 0, s, elems, n, elem_slot, aim_override,
+                       zrot_override, frames, ref_aabb, bounds_slot, prims, meta, optics);
+    return hipGetLastError();
+}
+
+hipError_t launch_frames_batch(hipStream_t s, const st_element_f64* elems, int32_t n, const int32_t* elem_slot,
+                               int32_t steps, const double* sun_units, const double* target, double* ref_aabb,
+                               float* bounds_slot, float4* prims) {
+    const int64_t total = (int64_t)n * steps;
+    hipLaunchKernelGGL(k_frames_batch, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, elems, n, elem_slot,
+                       steps, sun_units, target, ref_aabb, bounds_slot, prims);
+    return hipGetLastError();
+}
+
+hipError_t launch_refit(hipStream_t s, const int32_t* child, const int32_t* level_nodes,
+                        const int32_t* level_offsets_host, int32_t levels, int32_t n_nodes, int32_t n_prims,
+                        int32_t steps, const float* bounds_slot, float4* nodes) {
+    for (int lv = levels - 1; lv >= 0; --lv) {
+        const int32_t b = level_offsets_host[lv], c = level_offsets_host[lv + 1] - b;
+        if (c <= 0) continue;
+        const int64_t total = (int64_t)c * steps;
+        hipLaunchKernelGGL(k_refit, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, child, level_nodes, b, c,
+                           n_nodes, n_prims, steps, bounds_slot, nodes);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_sun_plane(hipStream_t s, const double* ref_aabb, int32_t n, int32_t steps, const double* sun_units,
+                            const st_sun_desc* sun, SunFrame* out) {
+    hipLaunchKernelGGL(k_sun_plane, dim3(steps), dim3(256), 0, s, ref_aabb, n, sun_units, *sun, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk_prefix, int32_t n_jobs,
+                        uint64_t total_chunks, uint32_t* work_counter, const LaunchShape& shape,
+                        const JobDev& single) {
+    int grid = shape.grid > 0 ? shape.grid : device_cu_count() * 8;
+    const uint64_t waves_needed = total_chunks;
+    const int waves_per_block = shape.block / 64;
+    const uint64_t max_blocks = (waves_needed + waves_per_block - 1) / waves_per_block;
+    if ((uint64_t)grid > max_blocks) grid = (int)(max_blocks > 0 ? max_blocks : 1);
+    switch (shape.block) {
+        case 64:
+            hipLaunchKernelGGL(k_trace<64>, dim3(grid), dim3(64), 0, s, jobs, chunk_prefix, n_jobs, total_chunks,
+                               work_counter, shape.chunk, single);
+            break;
+        case 128:
+            hipLaunchKernelGGL(k_trace<128>, dim3(grid), dim3(128), 0, s, jobs, chunk_prefix, n_jobs, total_chunks,
+                               work_counter, shape.chunk, single);
+            break;
+        case 256:
+            hipLaunchKernelGGL(k_trace<256>, dim3(grid), dim3(256), 0, s, jobs, chunk_prefix, n_jobs, total_chunks,
+                               work_counter, shape.chunk, single);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(hipStream_t s, void* p, uint64_t n_u64, uint64_t value) {
+    if (n_u64 == 0) return hipSuccess;
+    uint64_t blocks = (n_u64 + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)blocks), dim3(256), 0, s, (uint64_t*)p, n_u64, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_flux_watts(hipStream_t s, const int64_t* flux, int32_t nbins, const SunFrame* sun, double num_rays,
+                             double bin_area, double* out, double* power_per_ray_dev) {
+    const int n = nbins > 0 ? nbins : 1;
+    hipLaunchKernelGGL(k_flux_watts, dim3((n + 255) / 256), dim3(256), 0, s, flux, nbins, sun, num_rays, bin_area, out,
+                       power_per_ray_dev);
+    return hipGetLastError();
+}
+
+}  // namespace st
