@@ -180,6 +180,14 @@ int st_device_count(int* count);
 int st_context_create(int device, st_context* out);
 int st_context_destroy(st_context ctx);
 
+/* Device buffers for hosts without HIP headers (the C++ drop-in layer owns
+ * hit_point_buffer this way, lib/soltrace_system.cpp:118-125). */
+int st_buffer_alloc(st_context ctx, size_t bytes, void** dev_ptr);
+int st_buffer_free(st_context ctx, void* dev_ptr);
+int st_buffer_upload(st_context ctx, void* dev_dst, const void* host_src, size_t bytes);
+int st_buffer_download(st_context ctx, void* host_dst, const void* dev_src, size_t bytes);
+int st_buffer_zero(st_context ctx, void* dev_ptr, size_t bytes);
+
 /* ---- scenes ------------------------------------------------------------ */
 /* Upload FP64 elements, compute FP64 frames on the GPU, pack FP32 primitives,
  * build the BVH (host binned SAH over the FP64 bounds) and refit it on the GPU. */

@@ -193,6 +193,11 @@ SIGNATURES = {
     "st_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "st_context_create": (C.c_int, [C.c_int, C.POINTER(P)]),
     "st_context_destroy": (C.c_int, [P]),
+    "st_buffer_alloc": (C.c_int, [P, C.c_size_t, C.POINTER(P)]),
+    "st_buffer_free": (C.c_int, [P, P]),
+    "st_buffer_upload": (C.c_int, [P, P, P, C.c_size_t]),
+    "st_buffer_download": (C.c_int, [P, P, P, C.c_size_t]),
+    "st_buffer_zero": (C.c_int, [P, P, C.c_size_t]),
     "st_scene_create": (C.c_int, [P, C.POINTER(Element), C.c_int32, C.POINTER(P)]),
     "st_scene_destroy": (C.c_int, [P]),
     "st_scene_num_elements": (C.c_int, [P, C.POINTER(C.c_int32)]),

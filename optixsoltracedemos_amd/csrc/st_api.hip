@@ -141,6 +141,49 @@ extern "C" int st_context_destroy(st_context ctx) {
     return ST_OK;
 }
 
+extern "C" int st_buffer_alloc(st_context ctx, size_t bytes, void** dev_ptr) {
+    if (!ctx || !dev_ptr) return fail(ST_ERR_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    return dev_alloc(dev_ptr, bytes, "user buffer");
+}
+
+extern "C" int st_buffer_free(st_context ctx, void* dev_ptr) {
+    if (!ctx) return fail(ST_ERR_INVALID, "null context");
+    if (!dev_ptr) return ST_OK;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    HIP_TRY(hipFree(dev_ptr), "hipFree");
+    return ST_OK;
+}
+
+extern "C" int st_buffer_upload(st_context ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(ST_ERR_INVALID, "null argument");
+    if (!bytes) return ST_OK;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "upload");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    return ST_OK;
+}
+
+extern "C" int st_buffer_download(st_context ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(ST_ERR_INVALID, "null argument");
+    if (!bytes) return ST_OK;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    HIP_TRY(hipDeviceSynchronize(), "sync");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "download");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    return ST_OK;
+}
+
+extern "C" int st_buffer_zero(st_context ctx, void* dev_ptr, size_t bytes) {
+    if (!ctx || (!dev_ptr && bytes)) return fail(ST_ERR_INVALID, "null argument");
+    if (!bytes) return ST_OK;
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    HIP_TRY(hipMemsetAsync(dev_ptr, 0, bytes, ctx->stream), "memset");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    return ST_OK;
+}
+
 // ---------------------------------------------------------------------------
 static int validate_elements(const st_element_f64* el, int32_t n) {
     if (!el) return fail(ST_ERR_INVALID, "null element array");
@@ -255,7 +298,7 @@ extern "C" int st_scene_create(st_context ctx, const st_element_f64* elements, i
     ST_TRY(s->d_level_nodes.ensure(sizeof(int32_t) * NN, "BVH levels"));
     ST_TRY(s->d_sun.ensure(sizeof(SunFrame), "sun frame"));
     ST_TRY(s->d_sun_unit.ensure(sizeof(double) * 3, "sun vector"));
-    ST_TRY(s->d_aim.ensure(sizeof(double) * 3 * N, "aim points"));
+    ST_TRY(s->d_aim.ensure(sizeof(double) * std::max<size_t>(3 * N, 6), "aim points"));
     ST_TRY(s->d_zrot.ensure(sizeof(double) * N, "zrot"));
     ST_TRY(s->d_counter.ensure(sizeof(uint32_t) * 2, "work counter"));
     hipStream_t st = ctx->stream;
@@ -333,18 +376,18 @@ extern "C" int st_scene_aim_heliostats(st_scene s, const double sun_vector[3], c
     if (!s || !sun_vector || !target) return fail(ST_ERR_INVALID, "null argument");
     double l = std::sqrt(sun_vector[0] * sun_vector[0] + sun_vector[1] * sun_vector[1] + sun_vector[2] * sun_vector[2]);
     if (!(l > 0)) return fail(ST_ERR_INVALID, "zero sun vector");
-    double su[3] = {sun_vector[0] / l, sun_vector[1] / l, sun_vector[2] / l};
-    for (int32_t i = 0; i < s->n; ++i) {
-        st_element_f64& e = s->host_elems[i];
-        if (e.role != ST_ROLE_MIRROR) continue;
-        bisector_aim_point(e.origin, su, target, e.aim_point);
-    }
+    double st_in[6] = {sun_vector[0] / l, sun_vector[1] / l, sun_vector[2] / l, target[0], target[1], target[2]};
     HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
     hipStream_t st = s->ctx->stream;
-    HIP_TRY(hipMemcpyAsync(s->d_elems.ptr, s->host_elems.data(), sizeof(st_element_f64) * s->n, hipMemcpyHostToDevice, st),
-            "upload");
+    // sun unit vector and target share the scene's aim scratch (6 doubles)
+    HIP_TRY(hipMemcpyAsync(s->d_aim.ptr, st_in, sizeof(st_in), hipMemcpyHostToDevice, st), "upload");
+    HIP_TRY(launch_aim(st, (st_element_f64*)s->d_elems.ptr, s->n, (const double*)s->d_aim.ptr,
+                       (const double*)s->d_aim.ptr + 3),
+            "aim kernel");
     ST_TRY(scene_refresh(s, false));
-    ST_TRY(scene_sun_plane(s));
+    ST_TRY(scene_sun_plane(s));  // synchronizes the stream
+    HIP_TRY(hipMemcpyAsync(s->host_elems.data(), s->d_elems.ptr, sizeof(st_element_f64) * s->n, hipMemcpyDeviceToHost, st),
+            "download");
     HIP_TRY(hipStreamSynchronize(st), "aim sync");
     return ST_OK;
 }

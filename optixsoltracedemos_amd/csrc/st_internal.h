@@ -47,6 +47,8 @@ hipError_t launch_frames(hipStream_t s, const st_element_f64* elems, int32_t n, 
                          const double* aim_override /* n*3 or null */, const double* zrot_override /* n or null */,
                          double* frames, double* ref_aabb, float* bounds_slot, float4* prims, uint32_t* meta,
                          float4* optics);
+// FP64 bisector aim of all mirrors (in place on the device element array)
+hipError_t launch_aim(hipStream_t s, st_element_f64* elems, int32_t n, const double* sun_unit, const double* target);
 // batched: per step k re-aim mirrors along the bisector of sun[k] and target
 hipError_t launch_frames_batch(hipStream_t s, const st_element_f64* elems, int32_t n, const int32_t* elem_slot,
                                int32_t steps, const double* sun_units /* steps*3 */, const double* target,

@@ -538,6 +538,17 @@ __global__ void k_frames_batch(const st_element_f64* __restrict__ elems, int32_t
     for (int j = 0; j < 4; ++j) pq[j] = q[j];
 }
 
+// FP64 bisector aim of every mirror at target (receivers keep their aim point)
+__global__ void k_aim(st_element_f64* __restrict__ elems, int32_t n, const double* __restrict__ sun_unit,
+                      const double* __restrict__ target) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n || elems[e].role != ST_ROLE_MIRROR) return;
+    const double s[3] = {sun_unit[0], sun_unit[1], sun_unit[2]};
+    double aim[3];
+    bisector_aim_point(elems[e].origin, s, target, aim);
+    for (int k = 0; k < 3; ++k) elems[e].aim_point[k] = aim[k];
+}
+
 __device__ __forceinline__ void child_box(int32_t code, const float4* __restrict__ nodes,
                                           const float* __restrict__ bounds, float lo[3], float hi[3]) {
     if (code >= 0) {
@@ -698,6 +709,11 @@ hipError_t launch_frames(hipStream_t s, const st_element_f64* elems, int32_t n, 
                          float* bounds_slot, float4* prims, uint32_t* meta, float4* optics) {
     hipLaunchKernelGGL(k_frames, dim3((n + 127) / 128), dim3(128), 0, s, elems, n, elem_slot, aim_override,
                        zrot_override, frames, ref_aabb, bounds_slot, prims, meta, optics);
+    return hipGetLastError();
+}
+
+hipError_t launch_aim(hipStream_t s, st_element_f64* elems, int32_t n, const double* sun_unit, const double* target) {
+    hipLaunchKernelGGL(k_aim, dim3((n + 127) / 128), dim3(128), 0, s, elems, n, sun_unit, target);
     return hipGetLastError();
 }
 

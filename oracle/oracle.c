@@ -793,10 +793,13 @@ static void trace_ray(or_worker* w, uint64_t local) {
         if (rid) rid[nd] = (uint32_t)h.elem;
         if (w->hits) w->hits[h.elem]++;
         w->stats[2]++;
-        if (p->rho < 1.0) {
+        /* reflectivity draw (new): absorbed when u >= rho; rho is rounded to
+         * FP32 as the device stores it, so both sides compare the same numbers */
+        const double rho = (double)(float)p->rho;
+        if (rho < 1.0) {
             double ur[4];
             draw4(ray, (uint32_t)(2 * nd), dsc->job_id, dsc->seed, ur);
-            if (ur[0] >= p->rho) { w->stats[5]++; return; }
+            if (ur[0] >= rho) { w->stats[5]++; return; }
         }
         /* faceforward(n, -d, n) (:38): n facing the incoming ray */
         double nf[3] = {h.n[0], h.n[1], h.n[2]};

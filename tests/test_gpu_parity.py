@@ -1,0 +1,245 @@
+"""HIP path (FP32, gfx950) against the FP64 oracle on shared Philox draws.
+
+Bar (stated here, checked below):
+  * element-ID sequences per ray identical except edge-grazing rays: at most
+    ID_MISMATCH_MAX of the rays;
+  * hit points of agreeing rays within POS_TOL_REL * scene scale + 1e-5 m
+    (FP32 at the scene's coordinates and path lengths);
+  * per-element counts and statistics differ only by rays whose IDs differ;
+  * flux maps: L1(gpu - oracle) / hits <= FLUX_L1_MAX (bin flips at edges).
+Full-size runs are checked through size-independent properties.
+"""
+import numpy as np
+import pytest
+
+from optixsoltracedemos_amd import _abi, scenes, tracer
+from tests.parity import compare_records, flux_l1, ray_outcomes_balance
+
+pytestmark = pytest.mark.gpu
+
+ID_MISMATCH_MAX = 1e-4
+POS_TOL_REL = 2e-6
+FLUX_L1_MAX = 2e-3
+
+
+def scene_scale(w):
+    o = w.elements["origin"]
+    return float(np.abs(o).max() + np.linalg.norm(o.max(0) - o.min(0)) + 10.0)
+
+
+def run_both(ctx, oracle_mod, w, n, pos=_abi.POS_PHILOX, seed=0, job_id=0, **kw):
+    sc = tracer.Scene(ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=pos)
+    g = sc.trace(n, seed=seed, job_id=job_id, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny,
+                 records=True, **kw).numpy()
+    os_ = oracle_mod.Scene(w.elements)
+    os_.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=pos)
+    o = os_.trace(n, seed=seed, job_id=job_id, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny,
+                  records=True)
+    return sc, os_, g, o
+
+
+def check_parity(w, g, o, n):
+    frac, err, same = compare_records(g["records"], g["record_elements"], o["records"], o["record_elements"])
+    bad = int(round(frac * n))
+    assert frac <= ID_MISMATCH_MAX, f"{frac:.2e} of rays changed element sequence"
+    assert err <= POS_TOL_REL * scene_scale(w) + 1e-5, f"max hit point error {err:.3e} m"
+    ds = np.abs(g["stats"].astype(np.int64)[:11] - o["stats"].astype(np.int64)[:11])
+    assert ds.max() <= 2 * bad, ds
+    dh = np.abs(g["element_hits"].astype(np.int64) - o["element_hits"].astype(np.int64)).sum()
+    assert dh <= 4 * bad
+    assert flux_l1(g["flux"], o["flux"]) <= FLUX_L1_MAX
+    assert ray_outcomes_balance(g["stats"])
+
+
+@pytest.mark.parametrize("pos", [_abi.POS_PHILOX, _abi.POS_HALTON])
+def test_three_heliostats_parity(gpu_ctx, oracle_mod, pos):
+    w = scenes.three_heliostats()
+    n = 200_000
+    sc, os_, g, o = run_both(gpu_ctx, oracle_mod, w, n, pos=pos)
+    gp, op = sc.sun_plane(), os_.sun_plane()
+    for k in ("v0", "v1", "v2", "v3"):
+        assert np.allclose(gp[k], op[k], atol=1e-12)
+    assert gp["area"] == pytest.approx(op["area"], rel=1e-14)
+    check_parity(w, g, o, n)
+    assert g["stats"][3] > 0
+
+
+def test_parabolic_dish_parity(gpu_ctx, oracle_mod):
+    w = scenes.parabolic_dish()
+    n = 200_000
+    _, _, g, o = run_both(gpu_ctx, oracle_mod, w, n, seed=7, job_id=3)
+    check_parity(w, g, o, n)
+    assert g["stats"][9] > 0 and g["stats"][4] > 0          # caps and direct-sun hits occur
+
+
+def test_field_10k_parity(gpu_ctx, oracle_mod):
+    w = scenes.heliostat_field(10_000)
+    n = 200_000
+    sc, _, g, o = run_both(gpu_ctx, oracle_mod, w, n)
+    info = sc.bvh_info()
+    assert info["depth"] < 32 and info["max_leaf"] <= 4
+    check_parity(w, g, o, n)
+
+
+def test_field_parity_at_other_sun_and_seed(gpu_ctx, oracle_mod):
+    w = scenes.heliostat_field(1000, elevation=25.0, azimuth=100.0)
+    n = 100_000
+    _, _, g, o = run_both(gpu_ctx, oracle_mod, w, n, seed=12345, job_id=9)
+    check_parity(w, g, o, n)
+
+
+def test_include_direct_sun_flag(gpu_ctx, oracle_mod):
+    w = scenes.parabolic_dish()
+    n = 100_000
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape)
+    a = sc.trace(n, max_depth=3, flux_nx=32, flux_ny=16).numpy()
+    b = sc.trace(n, max_depth=3, flux_nx=32, flux_ny=16, include_direct_sun=True).numpy()
+    os_ = oracle_mod.Scene(w.elements)
+    os_.set_sun(w.sun_vector, w.sun_param, w.sun_shape)
+    ob = os_.trace(n, max_depth=3, flux_nx=32, flux_ny=16, include_direct_sun=True)
+    assert b["flux"].sum() > a["flux"].sum()
+    assert flux_l1(b["flux"], ob["flux"]) <= FLUX_L1_MAX
+
+
+def test_chunked_and_block_sizes_bitwise(gpu_ctx):
+    w = scenes.heliostat_field(10_000)
+    n = 2_000_000
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param)
+    kw = dict(max_depth=5, flux_nx=w.flux_nx, flux_ny=w.flux_ny)
+    full = sc.trace(n, **kw).numpy()
+    h1 = sc.trace(n, ray_begin=0, ray_count=n // 3, **kw).numpy()
+    h2 = sc.trace(n, ray_begin=n // 3, **kw).numpy()
+    for k in ("flux", "element_hits", "stats"):
+        assert np.array_equal(h1[k] + h2[k], full[k]), k
+    for bs, grid in ((64, 0), (128, 7), (256, 1)):
+        r = sc.trace(n, block_size=bs, grid_blocks=grid, **kw).numpy()
+        for k in ("flux", "element_hits", "stats"):
+            assert np.array_equal(r[k], full[k]), (bs, grid, k)
+
+
+def test_full_size_field_properties(gpu_ctx):
+    """1e8 rays on the 10k field: outcomes balance, flux = binned hits, element
+    counts add up, reflectivity 0.93 within binomial noise, run-to-run identical."""
+    w = scenes.heliostat_field(10_000)
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param)
+    a = sc.trace(w.num_rays, max_depth=5, flux_nx=w.flux_nx, flux_ny=w.flux_ny).numpy()
+    b = sc.trace(w.num_rays, max_depth=5, flux_nx=w.flux_nx, flux_ny=w.flux_ny).numpy()
+    for k in ("flux", "element_hits", "stats"):
+        assert np.array_equal(a[k], b[k])
+    st = a["stats"].astype(np.int64)
+    assert st[0] == w.num_rays and ray_outcomes_balance(st)
+    assert a["flux"].sum() == st[10]
+    assert a["element_hits"].astype(np.int64).sum() == st[2] + st[3]
+    m, ab = st[2], st[5]
+    assert abs(ab - 0.07 * m) < 5 * np.sqrt(m * 0.07 * 0.93)
+
+
+def test_flux_to_watts_normalisation(gpu_ctx):
+    import torch
+    w = scenes.heliostat_field(1000)
+    n = 1_000_000
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param)
+    desc = sc.make_desc(n, max_depth=5, flux_nx=64, flux_ny=32)
+    res = sc.alloc_outputs(desc)
+    sc.trace_into(desc, res)
+    wm2, ppr = sc.flux_to_watts(desc, res.flux)
+    area = sc.sun_plane()["area"]
+    assert ppr == pytest.approx(1000.0 * area / n, rel=1e-15)
+    r = w.elements[-1]
+    bin_area = (np.pi * r["dim_x"] / 64) * (r["dim_y"] / 32)
+    assert torch.allclose(wm2.cpu(), res.flux.cpu().double() * ppr / bin_area, rtol=1e-15)
+    # total binned power = binned hits x power per ray
+    assert float(wm2.sum()) * bin_area == pytest.approx(float(res.flux.sum()) * ppr, rel=1e-12)
+
+
+def test_update_aim_and_sun(gpu_ctx, oracle_mod):
+    """GPU re-aim + refit + new sun plane equals a scene built from re-aimed elements."""
+    w0 = scenes.heliostat_field(1000)
+    w1 = scenes.heliostat_field(1000, elevation=35.0, azimuth=140.0)
+    sc = tracer.Scene(gpu_ctx, w0.elements)
+    sc.set_sun(w1.sun_vector, w1.sun_param)
+    sc.aim_heliostats(w1.sun_vector, w1.extra["target"])
+    fresh = tracer.Scene(gpu_ctx, w1.elements)
+    fresh.set_sun(w1.sun_vector, w1.sun_param)
+    assert np.allclose(sc.frames(), fresh.frames(), atol=1e-14)
+    n = 300_000
+    a = sc.trace(n, max_depth=5, flux_nx=64, flux_ny=32).numpy()
+    b = fresh.trace(n, max_depth=5, flux_nx=64, flux_ny=32).numpy()
+    assert np.abs(a["stats"].astype(np.int64) - b["stats"].astype(np.int64)).max() <= 4
+    os_ = oracle_mod.Scene(w1.elements)
+    os_.set_sun(w1.sun_vector, w1.sun_param)
+    o = os_.trace(n, max_depth=5, flux_nx=64, flux_ny=32)
+    assert flux_l1(a["flux"], o["flux"]) <= FLUX_L1_MAX
+
+
+def test_frames_match_oracle(gpu_ctx, oracle_mod):
+    w = scenes.heliostat_field(1000)
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    os_ = oracle_mod.Scene(w.elements)
+    assert np.allclose(sc.frames(), os_.frames(), atol=1e-14)
+
+
+def test_batch_equals_individual_steps(gpu_ctx):
+    w = scenes.heliostat_field(10_000)
+    suns, _ = scenes.annual_sweep(5, seed=3)
+    budgets = np.array([300_000, 1_000_000, 64, 700_001, 2_000_000], dtype=np.uint64)
+    tgt = w.extra["target"]
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    for graph in (False, True):
+        out = sc.trace_batch(suns, budgets, tgt, w.sun_param, seed=11, max_depth=5, flux_nx=64, flux_ny=32,
+                             use_graph=graph)
+        flux, hits, st = (out[k].cpu().numpy() for k in ("flux", "element_hits", "stats"))
+        ind = tracer.Scene(gpu_ctx, w.elements)
+        for k in range(len(budgets)):
+            ind.set_sun(suns[k], w.sun_param)
+            ind.aim_heliostats(suns[k], tgt)
+            r = ind.trace(int(budgets[k]), seed=11, job_id=k, max_depth=5, flux_nx=64, flux_ny=32).numpy()
+            assert np.array_equal(r["stats"], st[k]), (graph, k)
+            assert np.array_equal(r["flux"], flux[k]), (graph, k)
+            assert np.array_equal(r["element_hits"], hits[k]), (graph, k)
+    # graph replays reproduce the eager result
+    for _ in range(3):
+        again = sc.trace_batch(suns, budgets, tgt, w.sun_param, seed=11, max_depth=5, flux_nx=64, flux_ny=32,
+                               use_graph=True)
+        assert np.array_equal(again["stats"].cpu().numpy(), st)
+        assert np.array_equal(again["flux"].cpu().numpy(), flux)
+
+
+def test_invalid_trace_requests_are_rejected(gpu_ctx):
+    w = scenes.three_heliostats()
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    with pytest.raises(_abi.STError):
+        sc.trace(1000)                                  # sun not set
+    sc.set_sun(w.sun_vector, w.sun_param)
+    with pytest.raises(_abi.STError):
+        sc.trace(1000, max_depth=0)
+    with pytest.raises(_abi.STError):
+        sc.trace(1000, ray_begin=2000)
+    with pytest.raises(_abi.STError):
+        sc.trace(1000, flux_nx=8, flux_ny=8, flux_element=0)   # element 0 is a mirror
+    bad = w.elements.copy()
+    bad[0]["aim_point"] = bad[0]["origin"]
+    with pytest.raises(_abi.STError):
+        tracer.Scene(gpu_ctx, bad)
+
+
+def test_injected_faults_fail_cleanly(gpu_ctx):
+    lib = _abi.load()
+    w = scenes.three_heliostats()
+    lib.st_debug_inject_fault(0, 1)
+    with pytest.raises(_abi.STError) as ei:
+        tracer.Scene(gpu_ctx, w.elements)
+    assert ei.value.code == _abi.ST_ERR_INJECTED
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param)
+    lib.st_debug_inject_fault(1, 1)
+    with pytest.raises(_abi.STError) as ei:
+        sc.trace(1000)
+    assert ei.value.code == _abi.ST_ERR_INJECTED
+    r = sc.trace(1000).numpy()                         # next call works
+    assert r["stats"][0] == 1000
