@@ -1,0 +1,169 @@
+"""Benchmark: traced rays/s on the 10,000-heliostat field (one MI355X).
+
+One step = one st_trace of 1e8 sun rays over the 10k-heliostat radial-stagger
+field with a 150 m tower and a cylindrical receiver (flux map 256 x 128, int64
+bins, at most 4 interactions per ray). Scene, BVH and outputs are resident in
+HBM before timing starts. Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--rays 1e8]
+                  [--no-cpu-baseline] [--cpu-sample 1e7] [--no-serve]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "traced rays/sec at 10k-heliostat field + p50/p99 job latency, 1 MI355X"
+# FP32 VALU issue rate of MI355X: 256 CU x 64 lanes x 2.4 GHz (FMA counted
+# once, no packed FP32) = published 157.3 TFLOP/s / (2 FMA x 2 packed)
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# lane operations per unit of work (counting rules: add/mul/fma/min/max/cmp = 1,
+# sqrt/rcp = 4, sincos = 20, atan2 = 30, log = 10)
+OPS_PER_RAY_FIXED = 160      # Philox draw (~100 integer ops) + sun sample (~60)
+OPS_PER_SEGMENT = 520        # BVH traversal of the 10k field (~28 box tests x 12 + stack) + leaf tests
+OPS_PER_MIRROR_HIT = 178     # normal (15), reflectivity draw (~103: Philox + compare), slope error (~60)
+OPS_PER_RECEIVER_HIT = 140   # cylinder re-intersection (~90) + cylindrical bin (~50)
+HBM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_field10k_hbm_traffic.txt")
+
+
+def ops_per_ray(stats) -> float:
+    rays = float(stats[0])
+    return (OPS_PER_RAY_FIXED + OPS_PER_SEGMENT * stats[1] / rays + OPS_PER_MIRROR_HIT * stats[2] / rays
+            + OPS_PER_RECEIVER_HIT * stats[3] / rays)
+
+
+def read_traffic():
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    try:
+        for line in open(HBM_TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(workload, sample: int):
+    from oracle import oracle as O
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    s = O.Scene(workload.elements)
+    s.set_sun(workload.sun_vector, workload.sun_param, workload.sun_shape)
+    t0 = time.perf_counter()
+    s.trace(workload.num_rays, ray_begin=0, ray_count=sample, max_depth=workload.max_depth,
+            flux_nx=workload.flux_nx, flux_ny=workload.flux_ny, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": sample / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"rays [0, {sample}) of the same 1e8-ray field workload, FP64 scalar oracle, "
+                      f"{threads} threads, {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rays", type=float, default=1e8)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=float, default=1e7)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy 3 times (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")   # 1 GiB
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": a.numel() * 4}))
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (rays shard by index range; "
+              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from optixsoltracedemos_amd import scenes, tracer
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    rays = int(args.rays)
+    w = scenes.heliostat_field(10_000, num_rays=rays, seed=args.seed)
+    ctx = tracer.Context(0)
+    sc = tracer.Scene(ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape)
+    stream = torch.cuda.Stream(device=0)
+    desc = sc.make_desc(rays, seed=args.seed, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny,
+                        zero=False, block_size=args.block, grid_blocks=args.grid, stream=stream.cuda_stream)
+    res = sc.alloc_outputs(desc)
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        sc.trace_into(desc, res)
+    torch.cuda.synchronize()
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        sc.trace_into(desc, res)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(starts, ends)]))
+    stats = res.stats.cpu().numpy().astype(np.int64)
+    calls = args.warmup + args.steps
+    assert stats[0] == calls * rays, "every launched ray must be counted once"
+    per_call = stats / calls
+    value = args.steps * rays / wall
+    w_ops = ops_per_ray(per_call)
+    achieved = w_ops * rays / (kernel_ms * 1e-3) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "field_10k_heliostats_cylinder_receiver", "heliostats": 10_000,
+                   "rays_per_step": rays, "flux_bins": f"{w.flux_nx}x{w.flux_ny}", "max_depth": w.max_depth,
+                   "sun": "pillbox 4.65 mrad, el 60 az 180", "slope_error_mrad": 1.5, "reflectivity": 0.93,
+                   "seed": args.seed, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s",
+                     "frac": achieved / VALU_PEAK_TOPS, "traffic": read_traffic(),
+                     "ops_per_ray": w_ops, "kernel_ms": kernel_ms},
+        "segments_per_s": float(per_call[1]) * args.steps / wall,
+        "receiver_hits_per_ray": float(per_call[3] / rays),
+    }
+    if not args.no_serve:
+        try:
+            from optixsoltracedemos_amd import serving
+            line["latency"] = serving.quick_latency(ctx)
+        except Exception as ex:  # reported, never silently replaced
+            line["latency"] = {"error": f"{type(ex).__name__}: {ex}"}
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(w, int(args.cpu_sample))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

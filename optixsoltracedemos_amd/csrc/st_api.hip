@@ -132,12 +132,22 @@ extern "C" int st_context_create(int device, st_context* out) {
     return ST_OK;
 }
 
-extern "C" int st_context_destroy(st_context ctx) {
-    if (!ctx) return fail(ST_ERR_INVALID, "null context");
+namespace st {
+void context_retain(st_context ctx) { ctx->refs.fetch_add(1); }
+void context_release(st_context ctx) {
+    if (ctx->refs.fetch_sub(1) != 1) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+}  // namespace st
+
+extern "C" int st_context_destroy(st_context ctx) {
+    if (!ctx) return fail(ST_ERR_INVALID, "null context");
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    context_release(ctx);
     return ST_OK;
 }
 
@@ -261,6 +271,7 @@ extern "C" int st_scene_create(st_context ctx, const st_element_f64* elements, i
     HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
     auto s = std::make_unique<st_scene_s>();
     s->ctx = ctx;
+    s->ctx_ref.hold(ctx);
     s->n = n;
     s->host_elems.assign(elements, elements + n);
     // host FP64 bounds -> BVH topology
@@ -326,7 +337,7 @@ st_scene_s::~st_scene_s() {
 
 extern "C" int st_scene_destroy(st_scene scene) {
     if (!scene) return fail(ST_ERR_INVALID, "null scene");
-    delete scene;
+    delete scene;  // device buffers are freed before the context reference goes
     return ST_OK;
 }
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <mutex>
 #include <vector>
@@ -50,12 +51,39 @@ struct BatchCache {
 
 }  // namespace st
 
+// A context stays alive while scenes or servers created from it exist:
+// st_context_destroy drops the caller's reference, the last owner frees it.
 struct st_context_s {
     int device = 0;
     hipStream_t stream = nullptr;
+    std::atomic<int> refs{1};
 };
 
+namespace st {
+void context_retain(st_context ctx);
+void context_release(st_context ctx);
+}  // namespace st
+
+namespace st {
+// context reference held by a scene or server; declared first so it is
+// released after every device buffer of its owner has been freed
+struct ContextRef {
+    st_context ctx = nullptr;
+    ContextRef() = default;
+    ContextRef(const ContextRef&) = delete;
+    ContextRef& operator=(const ContextRef&) = delete;
+    void hold(st_context c) {
+        context_retain(c);
+        ctx = c;
+    }
+    ~ContextRef() {
+        if (ctx) context_release(ctx);
+    }
+};
+}  // namespace st
+
 struct st_scene_s {
+    st::ContextRef ctx_ref;
     st_context ctx = nullptr;
     int32_t n = 0, n_nodes = 0;
     std::vector<st_element_f64> host_elems;

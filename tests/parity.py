@@ -4,12 +4,13 @@ from __future__ import annotations
 import numpy as np
 
 
-def compare_records(g_rec, g_id, o_rec, o_id):
+def compare_records(g_rec, g_id, o_rec, o_id, per_ray=False):
     """Per-ray parity of hit records.
 
     Returns (id_mismatch_fraction, max position error over rays whose element
-    sequences agree, the position scale used). Slot ids: 0xFFFFFFFE sun origin,
-    0xFFFFFFFF empty.
+    sequences agree, mask of agreeing rays); with per_ray=True the second item
+    is the array of per-ray maximum errors instead. Slot ids: 0xFFFFFFFE sun
+    origin, 0xFFFFFFFF empty.
     """
     g_id = np.asarray(g_id).astype(np.uint32)
     o_id = np.asarray(o_id).astype(np.uint32)
@@ -18,10 +19,12 @@ def compare_records(g_rec, g_id, o_rec, o_id):
     g = np.asarray(g_rec, dtype=np.float64)[same]
     o = np.asarray(o_rec, dtype=np.float64)[same]
     if len(g) == 0:
-        return frac, 0.0, 1.0
+        return frac, (np.zeros(0) if per_ray else 0.0), same
     tag_ok = np.all(g[..., 0] == o[..., 0])
     assert tag_ok, "record tags differ on rays whose element sequences agree"
     err = np.linalg.norm(g[..., 1:] - o[..., 1:], axis=-1)
+    if per_ray:
+        return frac, err.max(axis=1), same
     return frac, float(err.max()), same
 
 

@@ -4,7 +4,10 @@ Bar (stated here, checked below):
   * element-ID sequences per ray identical except edge-grazing rays: at most
     ID_MISMATCH_MAX of the rays;
   * hit points of agreeing rays within POS_TOL_REL * scene scale + 1e-5 m
-    (FP32 at the scene's coordinates and path lengths);
+    (FP32 at the scene's coordinates and path lengths) for all but
+    POS_OUTLIER_MAX of them, and every agreeing ray within POS_TOL_GRAZE *
+    scene scale: near-tangent cylinder / paraboloid hits are ill-conditioned,
+    their FP32 intersection parameter errs like the square root of the rounding;
   * per-element counts and statistics differ only by rays whose IDs differ;
   * flux maps: L1(gpu - oracle) / hits <= FLUX_L1_MAX (bin flips at edges).
 Full-size runs are checked through size-independent properties.
@@ -19,6 +22,8 @@ pytestmark = pytest.mark.gpu
 
 ID_MISMATCH_MAX = 1e-4
 POS_TOL_REL = 2e-6
+POS_OUTLIER_MAX = 1e-4
+POS_TOL_GRAZE = 1e-4
 FLUX_L1_MAX = 2e-3
 
 
@@ -40,10 +45,15 @@ def run_both(ctx, oracle_mod, w, n, pos=_abi.POS_PHILOX, seed=0, job_id=0, **kw)
 
 
 def check_parity(w, g, o, n):
-    frac, err, same = compare_records(g["records"], g["record_elements"], o["records"], o["record_elements"])
+    frac, err, same = compare_records(g["records"], g["record_elements"], o["records"], o["record_elements"],
+                                      per_ray=True)
     bad = int(round(frac * n))
     assert frac <= ID_MISMATCH_MAX, f"{frac:.2e} of rays changed element sequence"
-    assert err <= POS_TOL_REL * scene_scale(w) + 1e-5, f"max hit point error {err:.3e} m"
+    scale = scene_scale(w)
+    tight = POS_TOL_REL * scale + 1e-5
+    outliers = float(np.mean(err > tight)) if len(err) else 0.0
+    assert outliers <= POS_OUTLIER_MAX, f"{outliers:.2e} of rays beyond {tight:.2e} m (max {err.max():.3e} m)"
+    assert err.max() <= POS_TOL_GRAZE * scale, f"max hit point error {err.max():.3e} m"
     ds = np.abs(g["stats"].astype(np.int64)[:11] - o["stats"].astype(np.int64)[:11])
     assert ds.max() <= 2 * bad, ds
     dh = np.abs(g["element_hits"].astype(np.int64) - o["element_hits"].astype(np.int64)).sum()
