@@ -50,11 +50,11 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(workload, sample: int):
+def cpu_baseline(workload, sample: int, pos: int):
     from oracle import oracle as O
     threads = max(1, min(os.cpu_count() or 1, 16))
     s = O.Scene(workload.elements)
-    s.set_sun(workload.sun_vector, workload.sun_param, workload.sun_shape)
+    s.set_sun(workload.sun_vector, workload.sun_param, workload.sun_shape, position_sampling=pos)
     t0 = time.perf_counter()
     s.trace(workload.num_rays, ray_begin=0, ray_count=sample, max_depth=workload.max_depth,
             flux_nx=workload.flux_nx, flux_ny=workload.flux_ny, threads=threads)
@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=float, default=1e7)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--sampling", choices=["stratified", "philox", "halton"], default="stratified",
+                    help="sun-plane position sampling (stratified: 64-ray wave-coherent cells)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -103,7 +105,9 @@ def main():
     w = scenes.heliostat_field(10_000, num_rays=rays, seed=args.seed)
     ctx = tracer.Context(0)
     sc = tracer.Scene(ctx, w.elements)
-    sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape)
+    from optixsoltracedemos_amd import _abi
+    pos = {"stratified": _abi.POS_STRATIFIED, "philox": _abi.POS_PHILOX, "halton": _abi.POS_HALTON}[args.sampling]
+    sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=pos)
     stream = torch.cuda.Stream(device=0)
     desc = sc.make_desc(rays, seed=args.seed, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny,
                         zero=False, block_size=args.block, grid_blocks=args.grid, stream=stream.cuda_stream)
@@ -147,7 +151,7 @@ def main():
         "config": {"workload": "field_10k_heliostats_cylinder_receiver", "heliostats": 10_000,
                    "rays_per_step": rays, "flux_bins": f"{w.flux_nx}x{w.flux_ny}", "max_depth": w.max_depth,
                    "sun": "pillbox 4.65 mrad, el 60 az 180", "slope_error_mrad": 1.5, "reflectivity": 0.93,
-                   "seed": args.seed, "parallelism": "single GPU"},
+                   "seed": args.seed, "sampling": args.sampling, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s",
                      "frac": achieved / VALU_PEAK_TOPS, "traffic": read_traffic(),
                      "ops_per_ray": w_ops, "kernel_ms": kernel_ms},
@@ -161,7 +165,7 @@ def main():
         except Exception as ex:  # reported, never silently replaced
             line["latency"] = {"error": f"{type(ex).__name__}: {ex}"}
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(w, int(args.cpu_sample))
+        line["cpu_baseline"] = cpu_baseline(w, int(args.cpu_sample), pos)
     print(json.dumps(line), flush=True)
 
 

@@ -71,7 +71,11 @@ enum st_sun_shape {
 };
 enum st_position_sampling {
     ST_POS_PHILOX = 0,    /* sun-plane point from Philox uniforms (default)   */
-    ST_POS_HALTON = 1     /* Halton(2,3) of the ray index, as cuda/sun.cu:16-38 */
+    ST_POS_HALTON = 1,    /* Halton(2,3) of the ray index, as cuda/sun.cu:16-38 */
+    ST_POS_STRATIFIED = 2 /* 256 x 256 cells of the sun plane; every 64 consecutive rays
+                             (one wave) share a cell, each sweep of 65,536 groups visits
+                             every cell once in a seeded Feistel order; uniform within
+                             the cell from the ray's own Philox draw */
 };
 
 /* trace flags */
@@ -80,6 +84,9 @@ enum st_position_sampling {
                                           (reference CSV keeps them; SolTrace drops them,
                                           scripts/post_processing/clean_direct_sun.py:6-16) */
 #define ST_TRACE_ZERO_OUTPUTS 0x4u     /* zero flux / element_hits / stats before tracing */
+#define ST_JOB_KEEP_AIM 0x100u         /* server job: trace the scene's stored aims instead of
+                                          re-aiming every mirror at the job's sun (bisector,
+                                          target = centre of the scene's last receiver) */
 
 /* trace kernels */
 #define ST_KERNEL_AUTO 0

@@ -28,7 +28,7 @@ SURFACE_FLAT, SURFACE_PARABOLIC, SURFACE_CYLINDER = 0, 1, 3
 APERTURE_RECTANGLE, APERTURE_CIRCLE = 0, 1
 ROLE_MIRROR, ROLE_RECEIVER = 0, 1
 SUN_PILLBOX, SUN_GAUSSIAN = 0, 1
-POS_PHILOX, POS_HALTON = 0, 1
+POS_PHILOX, POS_HALTON, POS_STRATIFIED = 0, 1, 2
 
 TRACE_RECORDS = 0x1
 TRACE_FLUX_DIRECT_SUN = 0x2

@@ -409,7 +409,8 @@ extern "C" int st_scene_set_sun(st_scene s, const st_sun_desc* sun) {
     if (!(l > 0) || !std::isfinite(l)) return fail(ST_ERR_INVALID, "sun vector must be finite and non-zero");
     if (sun->shape != ST_SUN_PILLBOX && sun->shape != ST_SUN_GAUSSIAN) return fail(ST_ERR_INVALID, "unknown sun shape %d", sun->shape);
     if (!(sun->param >= 0) || sun->param > 0.2) return fail(ST_ERR_INVALID, "sun angle %g rad outside [0, 0.2]", sun->param);
-    if (sun->position_sampling != ST_POS_PHILOX && sun->position_sampling != ST_POS_HALTON)
+    if (sun->position_sampling != ST_POS_PHILOX && sun->position_sampling != ST_POS_HALTON &&
+        sun->position_sampling != ST_POS_STRATIFIED)
         return fail(ST_ERR_INVALID, "unknown position sampling %d", sun->position_sampling);
     s->sun = *sun;
     if (!(s->sun.dni > 0)) s->sun.dni = 1000.0;
@@ -496,6 +497,15 @@ int fill_job(st_scene s, const st_trace_desc* d, const st_outputs* out, const vo
     J->flux_nx = d->flux_nx;
     J->flux_ny = d->flux_ny;
     ST_TRY(resolve_flux_element(s, d->flux_element, d->flux_nx, d->flux_ny, &J->flux_slot));
+    // the hottest counter is the main receiver's: aggregate it in registers
+    // whether or not a flux map is requested (one atomic per wave and chunk)
+    J->agg_slot = J->flux_slot;
+    if (J->agg_slot < 0)
+        for (int32_t i = s->n - 1; i >= 0; --i)
+            if (s->host_elems[i].role == ST_ROLE_RECEIVER) {
+                J->agg_slot = s->topo.elem_slot[i];
+                break;
+            }
     if (out) {
         J->flux = J->flux_slot >= 0 ? out->flux : nullptr;
         J->hits = out->element_hits;

@@ -26,6 +26,7 @@ struct JobDev {
     int32_t max_depth;
     uint32_t flags;
     int32_t flux_slot;       // primitive slot of the flux element (-1 none)
+    int32_t agg_slot;        // receiver whose hit count is kept in registers (-1 none)
     int32_t flux_nx, flux_ny;
     int64_t* flux;
     uint64_t* hits;          // [n_elements]

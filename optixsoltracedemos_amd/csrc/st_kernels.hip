@@ -74,6 +74,23 @@ __device__ __forceinline__ float halton3(uint32_t i) {
     return den > 1 ? (float)((double)num / (double)den) : 0.f;
 }
 
+// cell of the stratified sun-plane sampling: 64 consecutive rays per cell of a
+// 256 x 256 grid, cells in a keyed 16-bit Feistel order within each sweep
+__device__ __forceinline__ uint32_t stratified_cell(uint64_t ray, uint32_t seed_lo, uint32_t seed_hi, uint32_t job) {
+    const uint64_t group = ray >> 6, sweep = group >> 16;
+    const U4 k = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), 0x53545254u, job}, seed_lo, seed_hi);
+    const uint32_t keys[4] = {k.x, k.y, k.z, k.w};
+    uint32_t L = ((uint32_t)group >> 8) & 0xFFu, R = (uint32_t)group & 0xFFu;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t F = ((R ^ keys[r]) * 0x9E3779B1u) >> 24;
+        const uint32_t nl = R;
+        R = L ^ F;
+        L = nl;
+    }
+    return (L << 8) | R;
+}
+
 // ---------------------------------------------------------------------------
 // Intersections (FP32). Same root selection and bounds rules as the
 // reference programs; the roots are evaluated in cancellation-free form.
@@ -272,7 +289,11 @@ __device__ __forceinline__ void trace_ray(const JobDev& J, const SunFrame& S, ui
     // ---- sun sample: cuda/sun.cu:94-134
     const U4 u = philox4x32_10(U4{rlo, rhi, 0u, J.job_id}, J.seed_lo, J.seed_hi);
     float a, b;
-    if (S.position_sampling == ST_POS_HALTON) {
+    if (S.position_sampling == ST_POS_STRATIFIED) {
+        const uint32_t cell = stratified_cell(ray, J.seed_lo, J.seed_hi, J.job_id);
+        a = ((float)(cell & 255u) + u01(u.x)) * (1.f / 256.f);
+        b = ((float)(cell >> 8) + u01(u.y)) * (1.f / 256.f);
+    } else if (S.position_sampling == ST_POS_HALTON) {
         a = halton2(rlo);
         b = halton3(rlo);
     } else {
@@ -325,7 +346,7 @@ __device__ __forceinline__ void trace_ray(const JobDev& J, const SunFrame& S, ui
                 record(J, ray, nd, 2.f, p, elem);
                 last = nd;
                 const bool fe = slot == J.flux_slot;
-                if (fe) cnt.c[11]++;
+                if (slot == J.agg_slot) cnt.c[11]++;
                 else if (J.hits) atomicAdd((unsigned long long*)&J.hits[elem], 1ull);
                 cnt.c[ST_STAT_RECEIVER_HITS]++;
                 const bool direct = depth == 0;
@@ -451,8 +472,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const JobDev* __restrict__ jobs
             if (lane == 0 && v) {
                 if (k < 11) {
                     if (J.stats) atomicAdd((unsigned long long*)&J.stats[k], (unsigned long long)v);
-                } else if (J.hits && J.flux_slot >= 0) {
-                    atomicAdd((unsigned long long*)&J.hits[meta_elem(J.meta[J.flux_slot])], (unsigned long long)v);
+                } else if (J.hits && J.agg_slot >= 0) {
+                    atomicAdd((unsigned long long*)&J.hits[meta_elem(J.meta[J.agg_slot])], (unsigned long long)v);
                 }
             }
         }
@@ -497,8 +518,8 @@ __global__ void k_frames(const st_element_f64* __restrict__ elems, int32_t n, co
     float4 q[4];
     element_pack(E, l2g, q);
     for (int k = 0; k < 4; ++k) prims[4 * (size_t)slot + k] = q[k];
-    meta[slot] = meta_pack((uint32_t)e, element_kind(E), E.role == ST_ROLE_RECEIVER ? 1u : 0u);
-    optics[e] = make_float4((float)E.reflectivity, (float)E.slope_error, (float)E.spec_error, 0.f);
+    if (meta) meta[slot] = meta_pack((uint32_t)e, element_kind(E), E.role == ST_ROLE_RECEIVER ? 1u : 0u);
+    if (optics) optics[e] = make_float4((float)E.reflectivity, (float)E.slope_error, (float)E.spec_error, 0.f);
 }
 
 __global__ void k_frames_batch(const st_element_f64* __restrict__ elems, int32_t n,

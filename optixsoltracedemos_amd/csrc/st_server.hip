@@ -1,20 +1,526 @@
 // st_server.hip - continuous-batching trace service (st_server_*).
+//
+// No counterpart in the reference, which runs one blocking launch per scene
+// (lib/soltrace_system.cpp:145-170). Jobs reference device-resident scenes;
+// a worker thread owns one stream and runs rounds:
+//   1. admit queued jobs (FIFO) while a slot is free and the summed HBM
+//      footprint of resident jobs stays within the budget; a job larger than
+//      the whole budget is rejected;
+//   2. set up every newly admitted job: copy the scene's elements, re-aim its
+//      mirrors for the job's sun (FP64 bisector at the scene's last receiver),
+//      pack frames, refit the BVH and build the sun plane - all on the device;
+//   3. launch ONE trace kernel over a slice of every active job (jobs are
+//      chunk lists of a single launch, so work of different jobs overlaps);
+//   4. retire jobs whose rays are all traced (statistics copied to the host,
+//      per-job scene copy freed, outputs kept until released).
+// Ray draws are keyed by (seed, job_id, ray index), so a job's result does not
+// depend on the slices or on which other jobs shared its launches.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
 
 #include "st_api_internal.h"
 
 using namespace st;
 
-extern "C" int st_server_create(st_context, const st_server_config*, st_server* out) {
-    if (out) *out = nullptr;
-    return fail(ST_ERR_STATE, "serving loop not available in this build");
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
-extern "C" int st_server_destroy(st_server) { return fail(ST_ERR_STATE, "serving loop not available"); }
-extern "C" int st_server_add_scene(st_server, st_scene, int32_t*) { return fail(ST_ERR_STATE, "serving loop not available"); }
-extern "C" int st_server_submit(st_server, const st_job_desc*, int64_t*) { return fail(ST_ERR_STATE, "serving loop not available"); }
-extern "C" int st_server_poll(st_server, int64_t, st_job_result*) { return fail(ST_ERR_STATE, "serving loop not available"); }
-extern "C" int st_server_wait(st_server, int64_t, st_job_result*) { return fail(ST_ERR_STATE, "serving loop not available"); }
-extern "C" int st_server_fetch(st_server, int64_t, int64_t*, uint64_t*) { return fail(ST_ERR_STATE, "serving loop not available"); }
-extern "C" int st_server_cancel(st_server, int64_t) { return fail(ST_ERR_STATE, "serving loop not available"); }
-extern "C" int st_server_release(st_server, int64_t) { return fail(ST_ERR_STATE, "serving loop not available"); }
-extern "C" int st_server_get_stats(st_server, st_server_stats*) { return fail(ST_ERR_STATE, "serving loop not available"); }
+
+struct ServerJob {
+    int64_t handle = 0;
+    st_job_desc desc{};
+    st_scene scene = nullptr;
+    int32_t status = ST_OK;
+    bool done = false, cancel = false, admitted = false, ready = false;
+    double submit_s = 0, start_s = 0, finish_s = 0, power_per_ray = 0;
+    uint64_t dispatched = 0;
+    uint64_t stats[ST_STAT_COUNT] = {0};
+    double aim_host[6] = {0, 0, 0, 0, 0, 0};  // sun unit vector + aim target (source of an async upload)
+    size_t footprint = 0, scene_bytes = 0;
+    DevBuf elems, aim_in, ref_aabb, bounds, prims, nodes, sun, flux, hits, stats_dev;
+    JobDev dev{};
+};
+
+}  // namespace
+
+struct st_server_s {
+    st::ContextRef ctx_ref;
+    st_context ctx = nullptr;
+    st_server_config cfg{};
+    std::vector<st_scene> scenes;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ServerJob*> pending;
+    std::vector<ServerJob*> active;
+    std::unordered_map<int64_t, std::unique_ptr<ServerJob>> jobs;
+    int64_t next_handle = 1;
+    bool stop = false;
+    std::thread worker;
+    hipStream_t stream = nullptr, fetch_stream = nullptr;
+    DevBuf d_jobs, d_prefix, d_counter;
+    JobDev* h_jobs = nullptr;     // pinned
+    uint64_t* h_prefix = nullptr; // pinned
+    st_server_stats stats{};
+    std::string worker_error;
+};
+
+namespace {
+
+size_t job_footprint(st_scene sc, const st_job_desc& d) {
+    const size_t n = (size_t)sc->n, nn = (size_t)sc->n_nodes;
+    size_t scene_copy = n * (sizeof(st_element_f64) + 6 * sizeof(double) + 6 * sizeof(float) + 4 * sizeof(float4)) +
+                        nn * 4 * sizeof(float4) + sizeof(SunFrame) + 6 * sizeof(double);
+    size_t outputs = (size_t)std::max(d.flux_nx, 0) * (size_t)std::max(d.flux_ny, 0) * sizeof(int64_t) +
+                     n * sizeof(uint64_t) + ST_STAT_COUNT * sizeof(uint64_t);
+    return scene_copy + outputs;
+}
+
+size_t job_scene_bytes(st_scene sc) {
+    const size_t n = (size_t)sc->n, nn = (size_t)sc->n_nodes;
+    return n * (sizeof(st_element_f64) + 6 * sizeof(double) + 6 * sizeof(float) + 4 * sizeof(float4)) +
+           nn * 4 * sizeof(float4) + sizeof(SunFrame) + 6 * sizeof(double);
+}
+
+// device set-up of one admitted job on the server stream
+int setup_job(st_server srv, ServerJob* j) {
+    st_scene sc = j->scene;
+    const size_t n = (size_t)sc->n, nn = (size_t)sc->n_nodes;
+    hipStream_t st = srv->stream;
+    const st_job_desc& d = j->desc;
+    int rc;
+    if ((rc = j->elems.ensure(sizeof(st_element_f64) * n, "job elements")) != ST_OK) return rc;
+    if ((rc = j->aim_in.ensure(sizeof(double) * 6, "job sun")) != ST_OK) return rc;
+    if ((rc = j->ref_aabb.ensure(sizeof(double) * 6 * n, "job AABBs")) != ST_OK) return rc;
+    if ((rc = j->bounds.ensure(sizeof(float) * 6 * n, "job bounds")) != ST_OK) return rc;
+    if ((rc = j->prims.ensure(sizeof(float4) * 4 * n, "job primitives")) != ST_OK) return rc;
+    if ((rc = j->nodes.ensure(sizeof(float4) * 4 * nn, "job nodes")) != ST_OK) return rc;
+    if ((rc = j->sun.ensure(sizeof(SunFrame), "job sun frame")) != ST_OK) return rc;
+    const size_t nbins = (size_t)std::max(d.flux_nx, 0) * (size_t)std::max(d.flux_ny, 0);
+    if (nbins && (rc = j->flux.ensure(sizeof(int64_t) * nbins, "job flux")) != ST_OK) return rc;
+    if ((rc = j->hits.ensure(sizeof(uint64_t) * n, "job hits")) != ST_OK) return rc;
+    if ((rc = j->stats_dev.ensure(sizeof(uint64_t) * ST_STAT_COUNT, "job stats")) != ST_OK) return rc;
+    const double l = std::sqrt(d.sun.vector[0] * d.sun.vector[0] + d.sun.vector[1] * d.sun.vector[1] +
+                               d.sun.vector[2] * d.sun.vector[2]);
+    double* in = j->aim_host;
+    for (int k = 0; k < 3; ++k) in[k] = d.sun.vector[k] / l;
+    for (int32_t i = sc->n - 1; i >= 0; --i)
+        if (sc->host_elems[i].role == ST_ROLE_RECEIVER) {
+            for (int k = 0; k < 3; ++k) in[3 + k] = sc->host_elems[i].origin[k];
+            break;
+        }
+    hipError_t e;
+    if ((e = hipMemcpyAsync(j->aim_in.ptr, in, sizeof(j->aim_host), hipMemcpyHostToDevice, st)) != hipSuccess)
+        return hip_fail(e, "job sun upload");
+    if ((e = hipMemcpyAsync(j->elems.ptr, sc->d_elems.ptr, sizeof(st_element_f64) * n, hipMemcpyDeviceToDevice, st)) !=
+        hipSuccess)
+        return hip_fail(e, "job elements copy");
+    if (!(d.flags & ST_JOB_KEEP_AIM) &&
+        (e = launch_aim(st, (st_element_f64*)j->elems.ptr, sc->n, (const double*)j->aim_in.ptr,
+                        (const double*)j->aim_in.ptr + 3)) != hipSuccess)
+        return hip_fail(e, "job aim");
+    if ((e = launch_frames(st, (const st_element_f64*)j->elems.ptr, sc->n, (const int32_t*)sc->d_elem_slot.ptr, nullptr,
+                           nullptr, nullptr, (double*)j->ref_aabb.ptr, (float*)j->bounds.ptr, (float4*)j->prims.ptr,
+                           nullptr, nullptr)) != hipSuccess)
+        return hip_fail(e, "job frames");
+    if ((e = launch_refit(st, (const int32_t*)sc->d_child.ptr, (const int32_t*)sc->d_level_nodes.ptr,
+                          sc->level_offsets.data(), (int32_t)sc->level_offsets.size() - 1, sc->n_nodes, sc->n, 1,
+                          (const float*)j->bounds.ptr, (float4*)j->nodes.ptr)) != hipSuccess)
+        return hip_fail(e, "job refit");
+    st_sun_desc sun = d.sun;
+    if (!(sun.dni > 0)) sun.dni = 1000.0;
+    if ((e = launch_sun_plane(st, (const double*)j->ref_aabb.ptr, sc->n, 1, (const double*)j->aim_in.ptr, &sun,
+                              (SunFrame*)j->sun.ptr)) != hipSuccess)
+        return hip_fail(e, "job sun plane");
+    if (nbins && (e = launch_fill_u64(st, j->flux.ptr, nbins, 0)) != hipSuccess) return hip_fail(e, "job zero");
+    if ((e = launch_fill_u64(st, j->hits.ptr, n, 0)) != hipSuccess) return hip_fail(e, "job zero");
+    if ((e = launch_fill_u64(st, j->stats_dev.ptr, ST_STAT_COUNT, 0)) != hipSuccess) return hip_fail(e, "job zero");
+    st_trace_desc td;
+    memset(&td, 0, sizeof(td));
+    td.num_rays = d.num_rays;
+    td.seed = d.seed;
+    td.job_id = d.job_id;
+    td.max_depth = d.max_depth;
+    td.flags = d.flags & ST_TRACE_FLUX_DIRECT_SUN;
+    td.flux_element = -1;
+    td.flux_nx = d.flux_nx;
+    td.flux_ny = d.flux_ny;
+    st_outputs o;
+    memset(&o, 0, sizeof(o));
+    o.flux = nbins ? (int64_t*)j->flux.ptr : nullptr;
+    o.element_hits = (uint64_t*)j->hits.ptr;
+    o.stats = (uint64_t*)j->stats_dev.ptr;
+    return fill_job(sc, &td, &o, j->prims.ptr, j->nodes.ptr, j->sun.ptr, &j->dev);
+}
+
+void release_scene_copy(ServerJob* j) {
+    j->elems.reset();
+    j->aim_in.reset();
+    j->ref_aabb.reset();
+    j->bounds.reset();
+    j->prims.reset();
+    j->nodes.reset();
+    j->sun.reset();
+}
+
+void finish_job(st_server srv, ServerJob* j, int32_t status) {
+    // caller holds srv->mu
+    j->status = status;
+    j->done = true;
+    j->finish_s = now_s();
+    srv->stats.resident_bytes -= j->scene_bytes;
+    if (status != ST_OK) srv->stats.resident_bytes -= (j->footprint - j->scene_bytes);
+    if (status == ST_OK) srv->stats.completed++;
+    else if (status == ST_ERR_CANCELLED) srv->stats.cancelled++;
+}
+
+void worker_main(st_server srv) {
+    (void)hipSetDevice(srv->ctx->device);
+    for (;;) {
+        std::vector<ServerJob*> fresh;
+        {
+            std::unique_lock<std::mutex> lk(srv->mu);
+            // sleep unless there is active work or the head of the queue can be decided
+            // (admitted, rejected or cancelled); a head waiting for HBM wakes on release
+            srv->cv_work.wait(lk, [&] {
+                if (srv->stop || !srv->active.empty()) return true;
+                if (srv->pending.empty()) return false;
+                const ServerJob* h = srv->pending.front();
+                return h->cancel || h->footprint > srv->cfg.hbm_budget_bytes ||
+                       srv->stats.resident_bytes + h->footprint <= srv->cfg.hbm_budget_bytes;
+            });
+            if (srv->stop) break;
+            // drop cancelled active jobs
+            for (size_t i = 0; i < srv->active.size();) {
+                ServerJob* j = srv->active[i];
+                if (j->cancel) {
+                    release_scene_copy(j);
+                    j->flux.reset();
+                    j->hits.reset();
+                    j->stats_dev.reset();
+                    finish_job(srv, j, ST_ERR_CANCELLED);
+                    srv->active.erase(srv->active.begin() + i);
+                    srv->cv_done.notify_all();
+                } else {
+                    ++i;
+                }
+            }
+            // admission, FIFO
+            while (!srv->pending.empty() && (int)srv->active.size() < srv->cfg.max_active_jobs) {
+                ServerJob* j = srv->pending.front();
+                if (j->cancel) {
+                    srv->pending.pop_front();
+                    j->status = ST_ERR_CANCELLED;
+                    j->done = true;
+                    j->finish_s = now_s();
+                    srv->stats.cancelled++;
+                    srv->cv_done.notify_all();
+                    continue;
+                }
+                if (j->footprint > srv->cfg.hbm_budget_bytes) {
+                    srv->pending.pop_front();
+                    j->status = ST_ERR_REJECTED;
+                    j->done = true;
+                    j->finish_s = now_s();
+                    srv->stats.rejected++;
+                    srv->cv_done.notify_all();
+                    continue;
+                }
+                if (srv->stats.resident_bytes + j->footprint > srv->cfg.hbm_budget_bytes) break;  // wait for memory
+                srv->pending.pop_front();
+                srv->stats.resident_bytes += j->footprint;
+                srv->stats.peak_resident_bytes = std::max(srv->stats.peak_resident_bytes, srv->stats.resident_bytes);
+                srv->stats.admitted++;
+                j->admitted = true;
+                j->start_s = now_s();
+                srv->active.push_back(j);
+                fresh.push_back(j);
+            }
+        }
+        const double t0 = now_s();
+        for (ServerJob* j : fresh) {
+            int rc = setup_job(srv, j);
+            if (rc != ST_OK) {
+                std::lock_guard<std::mutex> lk(srv->mu);
+                release_scene_copy(j);
+                finish_job(srv, j, rc);
+                srv->active.erase(std::find(srv->active.begin(), srv->active.end(), j));
+                srv->cv_done.notify_all();
+            } else {
+                j->ready = true;
+            }
+        }
+        // one launch over a slice of every active job
+        std::vector<ServerJob*> round;
+        uint64_t round_rays = 0;
+        {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            for (ServerJob* j : srv->active)
+                if (j->ready && j->dispatched < j->desc.num_rays) round.push_back(j);
+        }
+        if (round.empty()) continue;
+        std::vector<uint64_t> slice(round.size());
+        for (size_t i = 0; i < round.size(); ++i) {
+            slice[i] = std::min<uint64_t>(srv->cfg.chunk_rays, round[i]->desc.num_rays - round[i]->dispatched);
+            round_rays += slice[i];
+        }
+        const LaunchShape sh = pick_shape(round_rays, srv->cfg.block_size, srv->cfg.grid_blocks, ST_KERNEL_MEGA);
+        srv->h_prefix[0] = 0;
+        for (size_t i = 0; i < round.size(); ++i) {
+            JobDev J = round[i]->dev;
+            J.ray_begin = round[i]->dispatched;
+            J.ray_count = slice[i];
+            J.record_base = J.ray_begin;
+            srv->h_jobs[i] = J;
+            srv->h_prefix[i + 1] = srv->h_prefix[i] + (slice[i] + sh.chunk - 1) / sh.chunk;
+        }
+        hipStream_t st = srv->stream;
+        hipError_t e = hipMemcpyAsync(srv->d_jobs.ptr, srv->h_jobs, sizeof(JobDev) * round.size(),
+                                      hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(srv->d_prefix.ptr, srv->h_prefix, sizeof(uint64_t) * (round.size() + 1),
+                               hipMemcpyHostToDevice, st);
+        JobDev dummy;
+        memset(&dummy, 0, sizeof(dummy));
+        if (e == hipSuccess)
+            e = launch_trace(st, (const JobDev*)srv->d_jobs.ptr, (const uint64_t*)srv->d_prefix.ptr,
+                             (int32_t)round.size(), srv->h_prefix[round.size()], (uint32_t*)srv->d_counter.ptr, sh,
+                             dummy);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        // retire finished jobs
+        std::vector<ServerJob*> finished;
+        for (size_t i = 0; i < round.size(); ++i) {
+            round[i]->dispatched += slice[i];
+            if (round[i]->dispatched >= round[i]->desc.num_rays) finished.push_back(round[i]);
+        }
+        for (ServerJob* j : finished) {
+            if (e != hipSuccess) break;
+            SunFrame F;
+            e = hipMemcpyAsync(j->stats, j->stats_dev.ptr, sizeof(j->stats), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(&F, j->sun.ptr, sizeof(F), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e == hipSuccess) j->power_per_ray = F.dni * F.area / (double)j->desc.num_rays;
+        }
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stats.launches++;
+        srv->stats.rays_traced += round_rays;
+        srv->stats.busy_s += now_s() - t0;
+        if (e != hipSuccess) {
+            srv->worker_error = hipGetErrorString(e);
+            for (ServerJob* j : srv->active) {
+                release_scene_copy(j);
+                finish_job(srv, j, ST_ERR_DEVICE);
+            }
+            srv->active.clear();
+            srv->cv_done.notify_all();
+            continue;
+        }
+        for (ServerJob* j : finished) {
+            release_scene_copy(j);
+            finish_job(srv, j, ST_OK);
+            srv->active.erase(std::find(srv->active.begin(), srv->active.end(), j));
+        }
+        if (!finished.empty()) srv->cv_done.notify_all();
+    }
+}
+
+void fill_result(const ServerJob* j, st_job_result* r) {
+    memset(r, 0, sizeof(*r));
+    r->status = j->status;
+    r->done = j->done ? 1 : 0;
+    r->submit_s = j->submit_s;
+    r->start_s = j->start_s;
+    r->finish_s = j->finish_s;
+    r->power_per_ray = j->power_per_ray;
+    memcpy(r->stats, j->stats, sizeof(r->stats));
+}
+
+}  // namespace
+
+#define HIP_TRY(expr, what)                                \
+    do {                                                   \
+        hipError_t e__ = (expr);                           \
+        if (e__ != hipSuccess) return hip_fail(e__, what); \
+    } while (0)
+
+extern "C" int st_server_create(st_context ctx, const st_server_config* cfg, st_server* out) {
+    if (!ctx || !cfg || !out) return fail(ST_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (cfg->max_active_jobs <= 0 || cfg->max_active_jobs > 4096)
+        return fail(ST_ERR_INVALID, "max_active_jobs must be in [1, 4096]");
+    if (cfg->hbm_budget_bytes == 0) return fail(ST_ERR_INVALID, "zero HBM budget");
+    if (cfg->block_size != 0 && cfg->block_size != 64 && cfg->block_size != 128 && cfg->block_size != 256)
+        return fail(ST_ERR_INVALID, "block_size must be 0, 64, 128 or 256");
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    auto srv = std::make_unique<st_server_s>();
+    srv->ctx = ctx;
+    srv->ctx_ref.hold(ctx);
+    srv->cfg = *cfg;
+    if (srv->cfg.chunk_rays == 0) srv->cfg.chunk_rays = 8u << 20;
+    HIP_TRY(hipStreamCreateWithFlags(&srv->stream, hipStreamNonBlocking), "server stream");
+    HIP_TRY(hipStreamCreateWithFlags(&srv->fetch_stream, hipStreamNonBlocking), "server fetch stream");
+    const size_t J = (size_t)srv->cfg.max_active_jobs;
+    int rc;
+    if ((rc = srv->d_jobs.ensure(sizeof(JobDev) * J, "server jobs")) != ST_OK) return rc;
+    if ((rc = srv->d_prefix.ensure(sizeof(uint64_t) * (J + 1), "server prefix")) != ST_OK) return rc;
+    if ((rc = srv->d_counter.ensure(sizeof(uint32_t) * 2, "server counter")) != ST_OK) return rc;
+    HIP_TRY(hipMemset(srv->d_counter.ptr, 0, sizeof(uint32_t) * 2), "counter clear");
+    HIP_TRY(hipHostMalloc((void**)&srv->h_jobs, sizeof(JobDev) * J, hipHostMallocDefault), "pinned jobs");
+    HIP_TRY(hipHostMalloc((void**)&srv->h_prefix, sizeof(uint64_t) * (J + 1), hipHostMallocDefault), "pinned prefix");
+    HIP_TRY(hipDeviceSynchronize(), "server init sync");
+    st_server s = srv.release();
+    s->worker = std::thread(worker_main, s);
+    *out = s;
+    return ST_OK;
+}
+
+extern "C" int st_server_destroy(st_server srv) {
+    if (!srv) return fail(ST_ERR_INVALID, "null server");
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stop = true;
+    }
+    srv->cv_work.notify_all();
+    if (srv->worker.joinable()) srv->worker.join();
+    (void)hipSetDevice(srv->ctx->device);
+    (void)hipStreamSynchronize(srv->stream);
+    srv->jobs.clear();
+    if (srv->h_jobs) (void)hipHostFree(srv->h_jobs);
+    if (srv->h_prefix) (void)hipHostFree(srv->h_prefix);
+    (void)hipStreamDestroy(srv->stream);
+    (void)hipStreamDestroy(srv->fetch_stream);
+    delete srv;
+    return ST_OK;
+}
+
+extern "C" int st_server_add_scene(st_server srv, st_scene scene, int32_t* slot) {
+    if (!srv || !scene || !slot) return fail(ST_ERR_INVALID, "null argument");
+    if (scene->ctx != srv->ctx) return fail(ST_ERR_INVALID, "scene belongs to another context");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->scenes.push_back(scene);
+    *slot = (int32_t)srv->scenes.size() - 1;
+    return ST_OK;
+}
+
+extern "C" int st_server_submit(st_server srv, const st_job_desc* job, int64_t* handle) {
+    if (!srv || !job || !handle) return fail(ST_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (job->scene_slot < 0 || job->scene_slot >= (int32_t)srv->scenes.size())
+        return fail(ST_ERR_INVALID, "unknown scene slot %d", job->scene_slot);
+    if (job->num_rays == 0) return fail(ST_ERR_INVALID, "num_rays must be positive");
+    if (job->max_depth < 1 || job->max_depth > kMaxDepthLimit) return fail(ST_ERR_INVALID, "bad max_depth %d", job->max_depth);
+    if (job->flux_nx < 0 || job->flux_ny < 0 || (int64_t)job->flux_nx * job->flux_ny > (1 << 26))
+        return fail(ST_ERR_INVALID, "bad flux grid");
+    if (job->flags & ST_TRACE_RECORDS) return fail(ST_ERR_INVALID, "records are not served");
+    const double* v = job->sun.vector;
+    if (!(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] > 0)) return fail(ST_ERR_INVALID, "zero sun vector");
+    if (job->sun.shape != ST_SUN_PILLBOX && job->sun.shape != ST_SUN_GAUSSIAN) return fail(ST_ERR_INVALID, "bad sun shape");
+    if (!(job->sun.param >= 0) || job->sun.param > 0.2) return fail(ST_ERR_INVALID, "bad sun angle");
+    st_scene sc = srv->scenes[job->scene_slot];
+    bool has_receiver = false;
+    for (const auto& e : sc->host_elems) has_receiver |= e.role == ST_ROLE_RECEIVER;
+    if (!has_receiver) return fail(ST_ERR_INVALID, "scene has no receiver to aim at");
+    auto j = std::make_unique<ServerJob>();
+    j->handle = srv->next_handle++;
+    j->desc = *job;
+    j->scene = sc;
+    j->submit_s = now_s();
+    j->footprint = job_footprint(sc, *job);
+    j->scene_bytes = job_scene_bytes(sc);
+    ServerJob* raw = j.get();
+    srv->jobs[j->handle] = std::move(j);
+    srv->pending.push_back(raw);
+    srv->stats.submitted++;
+    *handle = raw->handle;
+    srv->cv_work.notify_all();
+    return ST_OK;
+}
+
+extern "C" int st_server_poll(st_server srv, int64_t h, st_job_result* r) {
+    if (!srv || !r) return fail(ST_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->jobs.find(h);
+    if (it == srv->jobs.end()) return fail(ST_ERR_INVALID, "unknown job %lld", (long long)h);
+    fill_result(it->second.get(), r);
+    return ST_OK;
+}
+
+extern "C" int st_server_wait(st_server srv, int64_t h, st_job_result* r) {
+    if (!srv || !r) return fail(ST_ERR_INVALID, "null argument");
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->jobs.find(h);
+    if (it == srv->jobs.end()) return fail(ST_ERR_INVALID, "unknown job %lld", (long long)h);
+    ServerJob* j = it->second.get();
+    srv->cv_done.wait(lk, [&] { return j->done; });
+    fill_result(j, r);
+    return ST_OK;
+}
+
+extern "C" int st_server_fetch(st_server srv, int64_t h, int64_t* flux_host, uint64_t* hits_host) {
+    if (!srv) return fail(ST_ERR_INVALID, "null server");
+    ServerJob* j = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        auto it = srv->jobs.find(h);
+        if (it == srv->jobs.end()) return fail(ST_ERR_INVALID, "unknown job %lld", (long long)h);
+        j = it->second.get();
+        if (!j->done) return fail(ST_ERR_STATE, "job %lld has not finished", (long long)h);
+        if (j->status != ST_OK) return fail(j->status, "job %lld did not complete", (long long)h);
+    }
+    HIP_TRY(hipSetDevice(srv->ctx->device), "hipSetDevice");
+    const size_t nbins = (size_t)std::max(j->desc.flux_nx, 0) * (size_t)std::max(j->desc.flux_ny, 0);
+    if (flux_host && nbins)
+        HIP_TRY(hipMemcpyAsync(flux_host, j->flux.ptr, nbins * sizeof(int64_t), hipMemcpyDeviceToHost, srv->fetch_stream),
+                "fetch flux");
+    if (hits_host)
+        HIP_TRY(hipMemcpyAsync(hits_host, j->hits.ptr, (size_t)j->scene->n * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                               srv->fetch_stream),
+                "fetch hits");
+    HIP_TRY(hipStreamSynchronize(srv->fetch_stream), "fetch sync");
+    return ST_OK;
+}
+
+extern "C" int st_server_cancel(st_server srv, int64_t h) {
+    if (!srv) return fail(ST_ERR_INVALID, "null server");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->jobs.find(h);
+    if (it == srv->jobs.end()) return fail(ST_ERR_INVALID, "unknown job %lld", (long long)h);
+    if (!it->second->done) it->second->cancel = true;
+    srv->cv_work.notify_all();
+    return ST_OK;
+}
+
+extern "C" int st_server_release(st_server srv, int64_t h) {
+    if (!srv) return fail(ST_ERR_INVALID, "null server");
+    std::unique_ptr<ServerJob> j;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        auto it = srv->jobs.find(h);
+        if (it == srv->jobs.end()) return fail(ST_ERR_INVALID, "unknown job %lld", (long long)h);
+        if (!it->second->done) return fail(ST_ERR_STATE, "job %lld is still running; cancel it first", (long long)h);
+        if (it->second->status == ST_OK) srv->stats.resident_bytes -= (it->second->footprint - it->second->scene_bytes);
+        j = std::move(it->second);
+        srv->jobs.erase(it);
+        srv->cv_work.notify_all();  // memory freed: admission may proceed
+    }
+    return ST_OK;  // device buffers freed by ~ServerJob
+}
+
+extern "C" int st_server_get_stats(st_server srv, st_server_stats* out) {
+    if (!srv || !out) return fail(ST_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    *out = srv->stats;
+    return ST_OK;
+}

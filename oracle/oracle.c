@@ -473,6 +473,29 @@ void or_reflect(const double d[3], const double n[3], double out[3]) {
  * of the ray index or Philox uniforms; direction uniform over the cap of
  * half-angle alpha around -s (:72-91): z = cos(alpha) + (1-cos(alpha)) xi2,
  * phi = 2 pi xi1, evaluated as 1-z = vers(alpha) (1-xi2) for accuracy. */
+/* Stratified sun-plane cell of a ray (position sampling mode 2, new): groups of
+ * 64 consecutive rays share one cell of a 256 x 256 grid; within a sweep of
+ * 65,536 groups the cells follow a 4-round Feistel permutation of the group
+ * index (8-bit halves, round function ((R ^ k) * 0x9E3779B1) >> 24) keyed by a
+ * Philox draw of (sweep, seed, job). Every cell is visited once per sweep. */
+uint32_t or_stratified_cell(uint64_t ray, uint64_t seed, uint32_t job) {
+    const uint64_t group = ray >> 6;
+    const uint64_t sweep = group >> 16;
+    uint32_t q = (uint32_t)(group & 0xFFFFu);
+    uint32_t ctr[4] = {(uint32_t)sweep, (uint32_t)(sweep >> 32), 0x53545254u, job};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t k[4];
+    or_philox4x32_10(ctr, key, k);
+    uint32_t L = q >> 8, R = q & 0xFFu;
+    for (int r = 0; r < 4; ++r) {
+        uint32_t F = ((R ^ k[r]) * 0x9E3779B1u) >> 24;
+        uint32_t nl = R;
+        R = L ^ F;
+        L = nl;
+    }
+    return (L << 8) | R;
+}
+
 void or_sample_sun(const or_scene* s, uint64_t ray, uint64_t seed, uint32_t job, double pos[3], double dir[3]) {
     double u[4];
     draw4(ray, 0, job, seed, u);
@@ -480,6 +503,10 @@ void or_sample_sun(const or_scene* s, uint64_t ray, uint64_t seed, uint32_t job,
     if (s->sun.position_sampling == 1) {
         a = or_halton((uint32_t)ray, 2);
         b = or_halton((uint32_t)ray, 3);
+    } else if (s->sun.position_sampling == 2) {
+        uint32_t cell = or_stratified_cell(ray, seed, job);
+        a = ((double)(cell & 255u) + u[0]) / 256.0;
+        b = ((double)(cell >> 8) + u[1]) / 256.0;
     } else {
         a = u[0];
         b = u[1];

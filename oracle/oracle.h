@@ -48,7 +48,7 @@ typedef struct or_sun {
     double param;
     double dni;
     int32_t shape;             /* 0 pillbox, 1 gaussian */
-    int32_t position_sampling; /* 0 philox, 1 halton */
+    int32_t position_sampling; /* 0 philox, 1 halton, 2 stratified */
 } or_sun;
 
 typedef struct or_sun_plane {
@@ -99,6 +99,7 @@ typedef struct or_scene or_scene;
 void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 double or_uniform(uint32_t x);           /* ((x>>9)+0.5) * 2^-23, in (0,1) */
 double or_halton(uint32_t index, uint32_t base);
+uint32_t or_stratified_cell(uint64_t ray, uint64_t seed, uint32_t job); /* cx | cy << 8 */
 
 /* element frame: L2G rows (x,y,z basis vectors are the columns) */
 void or_element_frame(const or_element* e, double l2g[9]);

@@ -65,6 +65,8 @@ def lib():
         L.or_uniform.restype = C.c_double
         L.or_halton.argtypes = [C.c_uint32, C.c_uint32]
         L.or_halton.restype = C.c_double
+        L.or_stratified_cell.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32]
+        L.or_stratified_cell.restype = C.c_uint32
         L.or_element_frame.argtypes = [C.c_void_p, d3]
         L.or_element_frame.restype = None
         L.or_element_ref_aabb.argtypes = [C.c_void_p, d3, d3]
@@ -105,6 +107,10 @@ def uniform(x: int) -> float:
 
 def halton(i: int, base: int) -> float:
     return lib().or_halton(i, base)
+
+
+def stratified_cell(ray: int, seed: int = 0, job: int = 0) -> int:
+    return lib().or_stratified_cell(ray, seed, job)
 
 
 def _elem_ptr(elements: np.ndarray):

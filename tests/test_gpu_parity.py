@@ -62,7 +62,7 @@ def check_parity(w, g, o, n):
     assert ray_outcomes_balance(g["stats"])
 
 
-@pytest.mark.parametrize("pos", [_abi.POS_PHILOX, _abi.POS_HALTON])
+@pytest.mark.parametrize("pos", [_abi.POS_PHILOX, _abi.POS_HALTON, _abi.POS_STRATIFIED])
 def test_three_heliostats_parity(gpu_ctx, oracle_mod, pos):
     w = scenes.three_heliostats()
     n = 200_000
@@ -89,6 +89,13 @@ def test_field_10k_parity(gpu_ctx, oracle_mod):
     sc, _, g, o = run_both(gpu_ctx, oracle_mod, w, n)
     info = sc.bvh_info()
     assert info["depth"] < 32 and info["max_leaf"] <= 4
+    check_parity(w, g, o, n)
+
+
+def test_field_10k_parity_stratified(gpu_ctx, oracle_mod):
+    w = scenes.heliostat_field(10_000)
+    n = 200_000
+    _, _, g, o = run_both(gpu_ctx, oracle_mod, w, n, pos=_abi.POS_STRATIFIED, seed=21)
     check_parity(w, g, o, n)
 
 

@@ -221,3 +221,38 @@ def test_records_layout(oracle_mod):
     # tags: 1 for mirrors, 2 for the receiver (element 3)
     assert np.all(rec[ids == 3][:, 0] == 2.0)
     assert np.all(rec[(ids < 3)][:, 0] == 1.0)
+
+
+def test_stratified_cells_cover_the_grid_once_per_sweep(oracle_mod):
+    """Position sampling mode 2: the 65,536 groups of one sweep visit every cell of the
+    256 x 256 grid exactly once (a bijection), in an order that depends on seed, job, sweep."""
+    cells0 = [oracle_mod.stratified_cell(g * 64, seed=3, job=1) for g in range(65536)]
+    assert sorted(cells0) == list(range(65536))
+    cells1 = [oracle_mod.stratified_cell((65536 + g) * 64, seed=3, job=1) for g in range(65536)]
+    assert sorted(cells1) == list(range(65536)) and cells0 != cells1
+    assert cells0 != [oracle_mod.stratified_cell(g * 64, seed=4, job=1) for g in range(65536)]
+    # 64 consecutive rays share their group's cell
+    assert len({oracle_mod.stratified_cell(128 + k, 3, 1) for k in range(64)}) == 1
+
+
+def test_stratified_positions_uniform(oracle_mod):
+    from optixsoltracedemos_amd._abi import POS_STRATIFIED
+    w = scenes.three_heliostats()
+    s = oracle_mod.Scene(w.elements)
+    s.set_sun(w.sun_vector, w.sun_param, position_sampling=POS_STRATIFIED)
+    pl = s.sun_plane()
+    e1, e2 = pl["v1"] - pl["v0"], pl["v3"] - pl["v0"]
+    ab = []
+    for i in range(0, 64 * 4000, 61):          # spread over many groups
+        p, _ = s.sample_sun(i, seed=9)
+        r = p - pl["v0"]
+        ab.append((r @ e1 / (e1 @ e1), r @ e2 / (e2 @ e2)))
+    ab = np.array(ab)
+    for k in range(2):
+        assert sst.kstest(ab[:, k], "uniform").pvalue > 1e-3
+    # same flux statistics as i.i.d. sampling within Monte Carlo noise
+    a = s.trace(400_000, max_depth=5, flux_nx=8, flux_ny=8, threads=8)
+    s.set_sun(w.sun_vector, w.sun_param)
+    b = s.trace(400_000, max_depth=5, flux_nx=8, flux_ny=8, threads=8)
+    ha, hb = int(a["stats"][3]), int(b["stats"][3])
+    assert abs(ha - hb) < 5 * math.sqrt(ha + hb)
