@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=float, default=1e7)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched-vs-plain annual sweep")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--sampling", choices=["stratified", "philox", "halton"], default="stratified",
                     help="sun-plane position sampling (stratified: 64-ray wave-coherent cells)")
     ap.add_argument("--block", type=int, default=0)
@@ -108,6 +110,9 @@ def main():
     from optixsoltracedemos_amd import _abi
     pos = {"stratified": _abi.POS_STRATIFIED, "philox": _abi.POS_PHILOX, "halton": _abi.POS_HALTON}[args.sampling]
     sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=pos)
+    tuning = None
+    if not args.no_autotune and args.block == 0 and args.grid == 0:
+        tuning = sc.autotune(sample_rays=20_000_000, max_depth=w.max_depth)
     stream = torch.cuda.Stream(device=0)
     desc = sc.make_desc(rays, seed=args.seed, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny,
                         zero=False, block_size=args.block, grid_blocks=args.grid, stream=stream.cuda_stream)
@@ -155,6 +160,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s",
                      "frac": achieved / VALU_PEAK_TOPS, "traffic": read_traffic(),
                      "ops_per_ray": w_ops, "kernel_ms": kernel_ms},
+        "tuning": tuning,
         "segments_per_s": float(per_call[1]) * args.steps / wall,
         "receiver_hits_per_ray": float(per_call[3] / rays),
     }
@@ -164,6 +170,12 @@ def main():
             line["latency"] = serving.quick_latency(ctx)
         except Exception as ex:  # reported, never silently replaced
             line["latency"] = {"error": f"{type(ex).__name__}: {ex}"}
+    if not args.no_sweep:
+        try:
+            from optixsoltracedemos_amd import sweep
+            line["sweep"] = sweep.compare(ctx, steps=96, seed=args.seed)
+        except Exception as ex:  # reported, never silently replaced
+            line["sweep"] = {"error": f"{type(ex).__name__}: {ex}"}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, int(args.cpu_sample), pos)
     print(json.dumps(line), flush=True)

@@ -218,6 +218,19 @@ int st_scene_get_aabbs(st_scene scene, float* lo_hi /* n*6 reference AABBs */);
 int st_scene_bvh_info(st_scene scene, int32_t* n_nodes, int32_t* depth, int32_t* max_leaf);
 int st_scene_synchronize(st_scene scene);
 
+/* Launch-shape autotuner (new): times every trace-kernel variant (block size,
+ * traversal stack depth, waves per SIMD) at several persistent grid sizes on
+ * sample_rays rays of this scene with its current sun, keeps the fastest as
+ * the scene's default (used when st_trace_desc.block_size and grid_blocks are
+ * 0) and reports it. */
+typedef struct st_tune_result {
+    int32_t block, grid, stack, waves;
+    int32_t trials;
+    int32_t reserved;
+    double best_ms, default_ms;   /* kernel time of the sample, tuned and untuned */
+} st_tune_result;
+int st_scene_autotune(st_scene scene, uint64_t sample_rays, int32_t max_depth, st_tune_result* out);
+
 /* ---- tracing ----------------------------------------------------------- */
 /* Trace on desc->stream (or the scene stream). Synchronous unless stream != NULL. */
 int st_trace(st_scene scene, const st_trace_desc* desc, const st_outputs* out);

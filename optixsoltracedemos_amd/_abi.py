@@ -139,6 +139,11 @@ class BatchDesc(C.Structure):
     ]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("block", C.c_int32), ("grid", C.c_int32), ("stack", C.c_int32), ("waves", C.c_int32),
+                ("trials", C.c_int32), ("reserved", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [
         ("hbm_budget_bytes", C.c_uint64),
@@ -210,6 +215,7 @@ SIGNATURES = {
     "st_scene_get_aabbs": (C.c_int, [P, C.POINTER(C.c_float)]),
     "st_scene_bvh_info": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "st_scene_synchronize": (C.c_int, [P]),
+    "st_scene_autotune": (C.c_int, [P, C.c_uint64, C.c_int32, C.POINTER(TuneResult)]),
     "st_trace": (C.c_int, [P, C.POINTER(TraceDesc), C.POINTER(Outputs)]),
     "st_flux_to_watts": (C.c_int, [P, C.POINTER(TraceDesc), P, P, C.POINTER(C.c_double)]),
     "st_trace_batch": (C.c_int, [P, C.POINTER(BatchDesc), P, P, P]),

@@ -74,11 +74,23 @@ int DevBuf::ensure(size_t n, const char* what) {
     return rc;
 }
 
-LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel) {
+LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel, int32_t bvh_depth,
+                       const LaunchShape* tuned) {
     LaunchShape sh;
-    sh.block = block > 0 ? block : 256;
     const int cus = device_cu_count();
-    sh.grid = grid > 0 ? grid : cus * 8;
+    if (tuned && block <= 0 && grid <= 0) {
+        sh = *tuned;
+        if (sh.stack < bvh_depth) sh.stack = 32;
+    } else {
+        sh.block = block > 0 ? block : 256;
+        sh.grid = grid > 0 ? grid : cus * 8;
+        sh.stack = bvh_depth <= 16 && sh.block >= 128 ? 16 : 32;
+        sh.waves = sh.stack == 16 ? 6 : 4;
+    }
+    if (!trace_variant_exists(sh.block, sh.stack, sh.waves)) {
+        sh.stack = 32;
+        sh.waves = 4;
+    }
     const uint64_t waves = (uint64_t)sh.grid * (uint64_t)(sh.block / 64);
     uint64_t chunk = rays / (4 * waves);
     chunk = (chunk + 63) / 64 * 64;
@@ -550,12 +562,104 @@ extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* ou
         if (out->element_hits) HIP_TRY(launch_fill_u64(st, out->element_hits, (uint64_t)s->n, 0), "zero hits");
         if (out->stats) HIP_TRY(launch_fill_u64(st, out->stats, ST_STAT_COUNT, 0), "zero stats");
     }
-    const LaunchShape sh = pick_shape(J.ray_count, d->block_size, d->grid_blocks, d->kernel);
+    const LaunchShape sh = pick_shape(J.ray_count, d->block_size, d->grid_blocks, d->kernel, s->topo.depth,
+                                      s->tuned_valid ? &s->tuned : nullptr);
     const uint64_t chunks = (J.ray_count + sh.chunk - 1) / sh.chunk;
     if (chunks >= (1ull << 32)) return fail(ST_ERR_INVALID, "too many rays for one call");
     if (launch_injected()) return fail(ST_ERR_INJECTED, "injected launch failure (trace)");
     HIP_TRY(launch_trace(st, nullptr, nullptr, 1, chunks, (uint32_t*)s->d_counter.ptr, sh, J), "trace kernel");
     if (!d->stream) HIP_TRY(hipStreamSynchronize(st), "trace sync");
+    return ST_OK;
+}
+
+extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_depth, st_tune_result* out) {
+    if (!s || !out) return fail(ST_ERR_INVALID, "null argument");
+    if (!s->sun_set) return fail(ST_ERR_STATE, "st_scene_set_sun must be called before tuning");
+    if (sample_rays < 64 || sample_rays > (1ull << 32)) return fail(ST_ERR_INVALID, "sample_rays outside [64, 2^32]");
+    if (max_depth < 1 || max_depth > kMaxDepthLimit) return fail(ST_ERR_INVALID, "bad max_depth %d", max_depth);
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    hipStream_t st = s->ctx->stream;
+    DevBuf hits, stats, flux;
+    ST_TRY(hits.ensure(sizeof(uint64_t) * s->n, "tune hits"));
+    ST_TRY(stats.ensure(sizeof(uint64_t) * ST_STAT_COUNT, "tune stats"));
+    ST_TRY(flux.ensure(sizeof(int64_t) * 256 * 128, "tune flux"));
+    st_trace_desc d;
+    memset(&d, 0, sizeof(d));
+    d.num_rays = sample_rays;
+    d.max_depth = max_depth;
+    d.flux_element = -1;
+    d.flux_nx = 256;
+    d.flux_ny = 128;
+    st_outputs o;
+    memset(&o, 0, sizeof(o));
+    o.flux = (int64_t*)flux.ptr;
+    o.element_hits = (uint64_t*)hits.ptr;
+    o.stats = (uint64_t*)stats.ptr;
+    JobDev J;
+    int32_t fslot = -1;
+    ST_TRY(resolve_flux_element(s, -1, 256, 128, &fslot));
+    if (fslot < 0) d.flux_nx = d.flux_ny = 0;
+    ST_TRY(fill_job(s, &d, &o, s->d_prims.ptr, s->d_nodes.ptr, s->d_sun.ptr, &J));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "event");
+    HIP_TRY(hipEventCreate(&e1), "event");
+    auto time_shape = [&](const LaunchShape& sh, double* ms) -> int {
+        const uint64_t chunks = (sample_rays + sh.chunk - 1) / sh.chunk;
+        float best = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) {  // first run warms caches and code
+            HIP_TRY(hipEventRecord(e0, st), "event");
+            HIP_TRY(launch_trace(st, nullptr, nullptr, 1, chunks, (uint32_t*)s->d_counter.ptr, sh, J), "trace");
+            HIP_TRY(hipEventRecord(e1, st), "event");
+            HIP_TRY(hipEventSynchronize(e1), "event sync");
+            float t = 0;
+            HIP_TRY(hipEventElapsedTime(&t, e0, e1), "event time");
+            if (rep > 0) best = std::min(best, t);
+        }
+        *ms = best;
+        return ST_OK;
+    };
+    const int cus = device_cu_count();
+    const LaunchShape def = pick_shape(sample_rays, 0, 0, ST_KERNEL_MEGA, s->topo.depth, nullptr);
+    double def_ms = 0;
+    int rc = time_shape(def, &def_ms);
+    LaunchShape best = def;
+    double best_ms = def_ms;
+    int trials = 1;
+    const int blocks[] = {128, 256}, stacks[] = {16, 32}, wavesv[] = {4, 5, 6, 8}, per_cu[] = {4, 8, 16};
+    for (int b : blocks)
+        for (int stk : stacks)
+            for (int w : wavesv)
+                for (int pc : per_cu) {
+                    if (rc != ST_OK) break;
+                    if (stk < s->topo.depth || !trace_variant_exists(b, stk, w)) continue;
+                    LaunchShape sh = def;
+                    sh.block = b;
+                    sh.stack = stk;
+                    sh.waves = w;
+                    sh.grid = cus * pc * 256 / b;
+                    const uint64_t wv = (uint64_t)sh.grid * (uint64_t)(b / 64);
+                    uint64_t chunk = (sample_rays / (4 * wv) + 63) / 64 * 64;
+                    sh.chunk = (int)std::min<uint64_t>(std::max<uint64_t>(chunk, 64), 2048);
+                    double ms = 0;
+                    rc = time_shape(sh, &ms);
+                    ++trials;
+                    if (rc == ST_OK && ms < best_ms) {
+                        best_ms = ms;
+                        best = sh;
+                    }
+                }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != ST_OK) return rc;
+    s->tuned = best;
+    s->tuned_valid = true;
+    out->block = best.block;
+    out->grid = best.grid;
+    out->stack = best.stack;
+    out->waves = best.waves;
+    out->trials = trials;
+    out->best_ms = best_ms;
+    out->default_ms = def_ms;
     return ST_OK;
 }
 
@@ -643,6 +747,10 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
     if (d->block_size != 0 && d->block_size != 64 && d->block_size != 128 && d->block_size != 256)
         return fail(ST_ERR_INVALID, "block_size must be 0, 64, 128 or 256");
     if (d->flags & ST_TRACE_RECORDS) return fail(ST_ERR_INVALID, "records are not available in batched sweeps");
+    if (d->sun.shape != ST_SUN_PILLBOX && d->sun.shape != ST_SUN_GAUSSIAN) return fail(ST_ERR_INVALID, "bad sun shape");
+    if (!(d->sun.param >= 0) || d->sun.param > 0.2) return fail(ST_ERR_INVALID, "bad sun angle");
+    if (d->sun.position_sampling < ST_POS_PHILOX || d->sun.position_sampling > ST_POS_STRATIFIED)
+        return fail(ST_ERR_INVALID, "bad position sampling");
     for (int k = 0; k < d->steps; ++k) {
         const double* v = d->sun_vectors + 3 * k;
         if (!(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] > 0)) return fail(ST_ERR_INVALID, "step %d: zero sun vector", k);
@@ -700,7 +808,8 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
         ST_TRY(b.d_counter.ensure(sizeof(uint32_t) * 2, "batch counter"));
         uint64_t total_rays = 0;
         for (int k = 0; k < K; ++k) total_rays += d->budgets[k];
-        b.shape = pick_shape(total_rays / K, d->block_size, d->grid_blocks, ST_KERNEL_MEGA);
+        b.shape = pick_shape(total_rays / K, d->block_size, d->grid_blocks, ST_KERNEL_MEGA, s->topo.depth,
+                             s->tuned_valid ? &s->tuned : nullptr);
         std::vector<JobDev> jobs(K);
         std::vector<uint64_t> prefix(K + 1, 0);
         for (int k = 0; k < K; ++k) {

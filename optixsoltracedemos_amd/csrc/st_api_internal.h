@@ -30,7 +30,10 @@ struct DevBuf {
     int ensure(size_t n, const char* what);  // (re)allocate to at least n bytes
 };
 
-LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel);
+// launch shape for `rays` rays; block/grid > 0 override, else the scene's tuned
+// shape when there is one, else defaults (16-entry stack when the BVH allows)
+LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel, int32_t bvh_depth,
+                       const LaunchShape* tuned);
 
 struct BatchCache {
     bool valid = false;
@@ -97,6 +100,8 @@ struct st_scene_s {
     st_sun_plane plane{};
     st::BatchCache batch;
     hipGraphExec_t batch_graph_exec = nullptr;
+    bool tuned_valid = false;
+    st::LaunchShape tuned;
     ~st_scene_s();
 };
 

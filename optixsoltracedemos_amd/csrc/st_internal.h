@@ -40,7 +40,10 @@ struct LaunchShape {
     int grid = 0;        // persistent blocks (0 -> 8 per CU)
     int chunk = 2048;    // rays per chunk (multiple of 64)
     int kernel = ST_KERNEL_MEGA;
+    int stack = 32;      // traversal stack entries per lane (16 or 32, >= BVH depth)
+    int waves = 4;       // waves per SIMD the register allocation is built for
 };
+bool trace_variant_exists(int block, int stack, int waves);
 
 // ---- launchers (all asynchronous on `stream`) -----------------------------
 // frames + packing for one scene: elements in element order -> prims in slot order

@@ -273,6 +273,22 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
+// base[idx] += 1 for every lane that calls it (lanes may diverge around the
+// call). One round of wave aggregation: the lanes sharing the first caller's
+// index add their count with a single atomic, the rest add one each. With
+// wave-coherent rays most callers share the index.
+__device__ __forceinline__ void add_one_aggregated(unsigned long long* base, uint32_t idx) {
+    const unsigned long long live = __ballot(1);
+    const int leader = __ffsll((long long)live) - 1;
+    const uint32_t lidx = __shfl(idx, leader, 64);
+    const unsigned long long same = __ballot(idx == lidx);
+    if (idx == lidx) {
+        if ((int)(threadIdx.x & 63) == leader) atomicAdd(base + lidx, (unsigned long long)__popcll(same));
+    } else {
+        atomicAdd(base + idx, 1ull);
+    }
+}
+
 __device__ __forceinline__ void record(const JobDev& J, uint64_t ray, int slot, float tag, V3 p, uint32_t elem) {
     if (J.records) {
         const uint64_t i = (ray - J.record_base) * (uint64_t)J.max_depth + (uint64_t)slot;
@@ -370,7 +386,7 @@ __device__ __forceinline__ void trace_ray(const JobDev& J, const SunFrame& S, ui
                     int ix = (int)floorf(fx), iy = (int)floorf(fy);
                     ix = min(max(ix, 0), J.flux_nx - 1);
                     iy = min(max(iy, 0), J.flux_ny - 1);
-                    atomicAdd((unsigned long long*)&J.flux[iy * J.flux_nx + ix], 1ull);
+                    add_one_aggregated((unsigned long long*)J.flux, (uint32_t)(iy * J.flux_nx + ix));
                     cnt.c[ST_STAT_FLUX_BINNED]++;
                 }
             } else {
@@ -385,7 +401,7 @@ __device__ __forceinline__ void trace_ray(const JobDev& J, const SunFrame& S, ui
         }
         record(J, ray, nd, 1.f, p, elem);
         last = nd;
-        if (J.hits) atomicAdd((unsigned long long*)&J.hits[elem], 1ull);
+        if (J.hits) add_one_aggregated((unsigned long long*)J.hits, elem);
         cnt.c[ST_STAT_MIRROR_HITS]++;
         const float4 op = J.optics[elem];
         if (op.x < 1.f) {
@@ -434,12 +450,14 @@ __device__ __forceinline__ void trace_ray(const JobDev& J, const SunFrame& S, ui
 // of work; the last wave to leave resets both to zero, so the next launch on
 // the stream starts from a clean counter without a memset (graph-safe).
 // jobs == nullptr: a single job passed by value in `single`.
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_trace(const JobDev* __restrict__ jobs,
-                                                 const uint64_t* __restrict__ chunk_prefix, int32_t n_jobs,
-                                                 uint64_t total_chunks, uint32_t* __restrict__ work_counter,
-                                                 int32_t chunk_rays, JobDev single) {
-    __shared__ int32_t lds_stack[kStackDepth * BLOCK];
+// STACK: traversal stack entries per lane (>= BVH depth); MINW: waves per SIMD
+// the register allocation must allow (LDS: STACK * BLOCK * 4 bytes per block)
+template <int BLOCK, int STACK, int MINW>
+__global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict__ jobs,
+                                                       const uint64_t* __restrict__ chunk_prefix, int32_t n_jobs,
+                                                       uint64_t total_chunks, uint32_t* __restrict__ work_counter,
+                                                       int32_t chunk_rays, JobDev single) {
+    __shared__ int32_t lds_stack[STACK * BLOCK];
     int32_t* stk = lds_stack + threadIdx.x;
     const int lane = threadIdx.x & 63;
     for (;;) {
@@ -768,6 +786,19 @@ hipError_t launch_sun_plane(hipStream_t s, const double* ref_aabb, int32_t n, in
     return hipGetLastError();
 }
 
+template <int BLOCK, int STACK, int MINW>
+static void launch_variant(hipStream_t s, int grid, const JobDev* jobs, const uint64_t* chunk_prefix, int32_t n_jobs,
+                           uint64_t total_chunks, uint32_t* work_counter, int chunk, const JobDev& single) {
+    hipLaunchKernelGGL((k_trace<BLOCK, STACK, MINW>), dim3(grid), dim3(BLOCK), 0, s, jobs, chunk_prefix, n_jobs,
+                       total_chunks, work_counter, chunk, single);
+}
+
+bool trace_variant_exists(int block, int stack, int waves) {
+    if (stack == 32) return (block == 64 || block == 128 || block == 256) && (waves == 4 || waves == 5);
+    if (stack == 16) return (block == 128 || block == 256) && (waves == 4 || waves == 6 || waves == 8);
+    return false;
+}
+
 hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk_prefix, int32_t n_jobs,
                         uint64_t total_chunks, uint32_t* work_counter, const LaunchShape& shape,
                         const JobDev& single) {
@@ -776,23 +807,26 @@ hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk
     const int waves_per_block = shape.block / 64;
     const uint64_t max_blocks = (waves_needed + waves_per_block - 1) / waves_per_block;
     if ((uint64_t)grid > max_blocks) grid = (int)(max_blocks > 0 ? max_blocks : 1);
-    switch (shape.block) {
-        case 64:
-            hipLaunchKernelGGL(k_trace<64>, dim3(grid), dim3(64), 0, s, jobs, chunk_prefix, n_jobs, total_chunks,
-                               work_counter, shape.chunk, single);
-            break;
-        case 128:
-            hipLaunchKernelGGL(k_trace<128>, dim3(grid), dim3(128), 0, s, jobs, chunk_prefix, n_jobs, total_chunks,
-                               work_counter, shape.chunk, single);
-            break;
-        case 256:
-            hipLaunchKernelGGL(k_trace<256>, dim3(grid), dim3(256), 0, s, jobs, chunk_prefix, n_jobs, total_chunks,
-                               work_counter, shape.chunk, single);
-            break;
-        default:
-            return hipErrorInvalidValue;
+    const int st = shape.stack, w = shape.waves, b = shape.block, c = shape.chunk;
+#define ST_VARIANT(B, S, W)                                                                             \
+    if (b == B && st == S && w == W) {                                                                 \
+        launch_variant<B, S, W>(s, grid, jobs, chunk_prefix, n_jobs, total_chunks, work_counter, c, single); \
+        return hipGetLastError();                                                                      \
     }
-    return hipGetLastError();
+    ST_VARIANT(64, 32, 4)
+    ST_VARIANT(64, 32, 5)
+    ST_VARIANT(128, 32, 4)
+    ST_VARIANT(128, 32, 5)
+    ST_VARIANT(256, 32, 4)
+    ST_VARIANT(256, 32, 5)
+    ST_VARIANT(128, 16, 4)
+    ST_VARIANT(128, 16, 6)
+    ST_VARIANT(128, 16, 8)
+    ST_VARIANT(256, 16, 4)
+    ST_VARIANT(256, 16, 6)
+    ST_VARIANT(256, 16, 8)
+#undef ST_VARIANT
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_fill_u64(hipStream_t s, void* p, uint64_t n_u64, uint64_t value) {

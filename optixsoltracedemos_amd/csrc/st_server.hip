@@ -276,7 +276,10 @@ void worker_main(st_server srv) {
             slice[i] = std::min<uint64_t>(srv->cfg.chunk_rays, round[i]->desc.num_rays - round[i]->dispatched);
             round_rays += slice[i];
         }
-        const LaunchShape sh = pick_shape(round_rays, srv->cfg.block_size, srv->cfg.grid_blocks, ST_KERNEL_MEGA);
+        int32_t depth = 0;
+        for (ServerJob* j : round) depth = std::max(depth, j->scene->topo.depth);
+        const LaunchShape sh =
+            pick_shape(round_rays, srv->cfg.block_size, srv->cfg.grid_blocks, ST_KERNEL_MEGA, depth, nullptr);
         srv->h_prefix[0] = 0;
         for (size_t i = 0; i < round.size(); ++i) {
             JobDev J = round[i]->dev;

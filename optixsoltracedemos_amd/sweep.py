@@ -1,0 +1,62 @@
+"""Batched annual sweep over the 10k-heliostat field: 96 sun positions with
+uneven ray budgets, mirrors re-aimed and the BVH refit per step.
+
+Two ways to run the same work:
+  * batched: one st_trace_batch call - FP64 GPU re-aim of every step, one
+    level-wise refit over all steps, all sun planes, and ONE trace launch over
+    the chunks of every step, captured in a hipGraph and replayed;
+  * plain per-scene loop, the reference's update()/run() cycle
+    (lib/soltrace_system.cpp:172-184, demos/demo_dynamic_scene.cpp:82-97):
+    per step re-aim + refit, new sun plane, one synchronous trace, flux copied
+    back to the host.
+Both produce identical per-step results (same Philox keys: seed, job = step).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _abi, scenes, tracer
+
+
+def run_batched(ctx, w, suns, budgets, seed=0, flux=(256, 128), use_graph=True, reps=2, position_sampling=_abi.POS_STRATIFIED):
+    import torch
+    sc = tracer.Scene(ctx, w.elements)
+    kw = dict(seed=seed, max_depth=w.max_depth, flux_nx=flux[0], flux_ny=flux[1], use_graph=use_graph)
+    sc.set_sun(suns[0], w.sun_param, position_sampling=position_sampling)
+    out = sc.trace_batch(suns, budgets, w.extra["target"], w.sun_param, position_sampling=position_sampling, **kw)
+    torch.cuda.synchronize()                                   # first call: set-up + graph capture
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = sc.trace_batch(suns, budgets, w.extra["target"], w.sun_param, position_sampling=position_sampling,
+                             out=out, **kw)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    return min(times), out
+
+
+def run_plain_loop(ctx, w, suns, budgets, seed=0, flux=(256, 128), position_sampling=_abi.POS_STRATIFIED):
+    sc = tracer.Scene(ctx, w.elements)
+    flux_maps = []
+    t0 = time.perf_counter()
+    for k in range(len(budgets)):
+        sc.set_sun(suns[k], w.sun_param, position_sampling=position_sampling)
+        sc.aim_heliostats(suns[k], w.extra["target"])
+        r = sc.trace(int(budgets[k]), seed=seed, job_id=k, max_depth=w.max_depth, flux_nx=flux[0], flux_ny=flux[1])
+        flux_maps.append(r.flux.cpu().numpy())
+    return time.perf_counter() - t0, flux_maps
+
+
+def compare(ctx, steps=96, seed=0):
+    w = scenes.heliostat_field(10_000)
+    suns, budgets = scenes.annual_sweep(steps, seed=seed)
+    total = int(budgets.sum())
+    tb, out = run_batched(ctx, w, suns, budgets, seed=seed)
+    tp, plain = run_plain_loop(ctx, w, suns, budgets, seed=seed)
+    fb = out["flux"].cpu().numpy()
+    identical = all(np.array_equal(fb[k], plain[k]) for k in range(steps))
+    return {"steps": steps, "total_rays": total, "batched_s": tb, "plain_loop_s": tp,
+            "batched_rays_per_s": total / tb, "plain_rays_per_s": total / tp, "speedup": tp / tb,
+            "identical_flux": bool(identical)}

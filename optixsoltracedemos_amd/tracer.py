@@ -115,6 +115,11 @@ class Scene:
         t = (C.c_double * 3)(*[float(x) for x in target])
         call("st_scene_aim_heliostats", self.h, s, t)
 
+    def autotune(self, sample_rays: int = 20_000_000, max_depth: int = 5) -> dict:
+        r = _abi.TuneResult()
+        call("st_scene_autotune", self.h, int(sample_rays), int(max_depth), C.byref(r))
+        return {k: getattr(r, k) for k, _ in _abi.TuneResult._fields_ if k != "reserved"}
+
     def synchronize(self):
         call("st_scene_synchronize", self.h)
 
@@ -175,7 +180,7 @@ class Scene:
 
     def trace_batch(self, sun_vectors: np.ndarray, budgets: np.ndarray, target, sun_param, sun_shape=_abi.SUN_PILLBOX,
                     seed=0, max_depth=5, flux_nx=0, flux_ny=0, flux_element=-1, use_graph=True, dni=1000.0,
-                    include_direct_sun=False, out=None, stream=None):
+                    include_direct_sun=False, out=None, stream=None, position_sampling=_abi.POS_PHILOX):
         torch = _torch()
         K = len(budgets)
         suns = np.ascontiguousarray(sun_vectors, dtype=np.float64).reshape(K, 3)
@@ -186,7 +191,7 @@ class Scene:
         d.budgets = bud.ctypes.data
         for k in range(3):
             d.target[k] = float(target[k])
-        d.sun = _abi.sun_desc([0, 0, 1], sun_param, sun_shape, dni)
+        d.sun = _abi.sun_desc([0, 0, 1], sun_param, sun_shape, dni, position_sampling)
         d.seed = int(seed)
         d.flux_element, d.flux_nx, d.flux_ny = int(flux_element), int(flux_nx), int(flux_ny)
         d.flags = _abi.TRACE_FLUX_DIRECT_SUN if include_direct_sun else 0

@@ -260,3 +260,33 @@ def test_injected_faults_fail_cleanly(gpu_ctx):
     assert ei.value.code == _abi.ST_ERR_INJECTED
     r = sc.trace(1000).numpy()                         # next call works
     assert r["stats"][0] == 1000
+
+
+def test_autotune_keeps_results_bitwise(gpu_ctx):
+    w = scenes.heliostat_field(10_000)
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param, position_sampling=_abi.POS_STRATIFIED)
+    kw = dict(max_depth=5, flux_nx=w.flux_nx, flux_ny=w.flux_ny)
+    before = sc.trace(3_000_000, **kw).numpy()
+    t = sc.autotune(sample_rays=4_000_000)
+    assert t["trials"] > 1 and 0 < t["best_ms"] <= t["default_ms"]
+    assert t["block"] in (128, 256) and t["stack"] >= sc.bvh_info()["depth"]
+    after = sc.trace(3_000_000, **kw).numpy()
+    for k in ("flux", "element_hits", "stats"):
+        assert np.array_equal(before[k], after[k]), k
+
+
+def test_batch_stratified_matches_individual(gpu_ctx):
+    w = scenes.heliostat_field(1000)
+    suns, _ = scenes.annual_sweep(3, seed=8)
+    budgets = np.array([500_000, 70_000, 1_300_000], dtype=np.uint64)
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    out = sc.trace_batch(suns, budgets, w.extra["target"], w.sun_param, seed=4, flux_nx=32, flux_ny=16,
+                         position_sampling=_abi.POS_STRATIFIED)
+    ind = tracer.Scene(gpu_ctx, w.elements)
+    for k in range(3):
+        ind.set_sun(suns[k], w.sun_param, position_sampling=_abi.POS_STRATIFIED)
+        ind.aim_heliostats(suns[k], w.extra["target"])
+        r = ind.trace(int(budgets[k]), seed=4, job_id=k, flux_nx=32, flux_ny=16).numpy()
+        assert np.array_equal(r["flux"], out["flux"][k].cpu().numpy())
+        assert np.array_equal(r["stats"], out["stats"][k].cpu().numpy())
