@@ -112,7 +112,7 @@ def main():
     sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=pos)
     tuning = None
     if not args.no_autotune and args.block == 0 and args.grid == 0:
-        tuning = sc.autotune(sample_rays=20_000_000, max_depth=w.max_depth)
+        tuning = sc.autotune(sample_rays=rays, max_depth=w.max_depth)   # tune at the measured size
     stream = torch.cuda.Stream(device=0)
     desc = sc.make_desc(rays, seed=args.seed, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny,
                         zero=False, block_size=args.block, grid_blocks=args.grid, stream=stream.cuda_stream)

@@ -606,7 +606,7 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
     auto time_shape = [&](const LaunchShape& sh, double* ms) -> int {
         const uint64_t chunks = (sample_rays + sh.chunk - 1) / sh.chunk;
         float best = 1e30f;
-        for (int rep = 0; rep < 3; ++rep) {  // first run warms caches and code
+        for (int rep = 0; rep < 4; ++rep) {  // first run warms caches and code
             HIP_TRY(hipEventRecord(e0, st), "event");
             HIP_TRY(launch_trace(st, nullptr, nullptr, 1, chunks, (uint32_t*)s->d_counter.ptr, sh, J), "trace");
             HIP_TRY(hipEventRecord(e1, st), "event");
@@ -643,7 +643,7 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
                     double ms = 0;
                     rc = time_shape(sh, &ms);
                     ++trials;
-                    if (rc == ST_OK && ms < best_ms) {
+                    if (rc == ST_OK && ms < 0.98 * best_ms) {
                         best_ms = ms;
                         best = sh;
                     }
