@@ -91,7 +91,8 @@ enum st_position_sampling {
 /* trace kernels */
 #define ST_KERNEL_AUTO 0
 #define ST_KERNEL_MEGA 1       /* one lane per ray, all bounces in registers        */
-#define ST_KERNEL_PERSISTENT 2 /* persistent waves, dead lanes refilled by ballot    */
+#define ST_KERNEL_QUEUED 2     /* sun rays in full waves; continuing rays parked in  
+                                  LDS and traced in full waves of 64                */
 
 /* statistics counters (uint64 each) */
 enum st_stat {
@@ -226,7 +227,7 @@ int st_scene_synchronize(st_scene scene);
 typedef struct st_tune_result {
     int32_t block, grid, stack, waves;
     int32_t trials;
-    int32_t reserved;
+    int32_t queued;               /* 1: ST_KERNEL_QUEUED form chosen                */
     double best_ms, default_ms;   /* kernel time of the sample, tuned and untuned */
 } st_tune_result;
 int st_scene_autotune(st_scene scene, uint64_t sample_rays, int32_t max_depth, st_tune_result* out);

@@ -34,7 +34,7 @@ TRACE_RECORDS = 0x1
 TRACE_FLUX_DIRECT_SUN = 0x2
 TRACE_ZERO_OUTPUTS = 0x4
 
-KERNEL_AUTO, KERNEL_MEGA, KERNEL_PERSISTENT = 0, 1, 2
+KERNEL_AUTO, KERNEL_MEGA, KERNEL_QUEUED = 0, 1, 2
 
 STAT_NAMES = [
     "rays", "segments", "mirror_hits", "receiver_hits", "direct_sun", "absorbed",
@@ -141,7 +141,7 @@ class BatchDesc(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("block", C.c_int32), ("grid", C.c_int32), ("stack", C.c_int32), ("waves", C.c_int32),
-                ("trials", C.c_int32), ("reserved", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double)]
+                ("trials", C.c_int32), ("queued", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double)]
 
 
 class ServerConfig(C.Structure):

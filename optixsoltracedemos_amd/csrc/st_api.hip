@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -78,24 +79,30 @@ LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kerne
                        const LaunchShape* tuned) {
     LaunchShape sh;
     const int cus = device_cu_count();
-    if (tuned && block <= 0 && grid <= 0) {
+    // ST_KERNEL_MEGA: whole ray per lane; ST_KERNEL_QUEUED: continuing rays
+    // regrouped in full waves; AUTO keeps the tuned (or default) form
+    const bool use_tuned = tuned && block <= 0 && grid <= 0 &&
+                           (kernel == ST_KERNEL_AUTO || (kernel == ST_KERNEL_QUEUED) == tuned->queued);
+    if (use_tuned) {
         sh = *tuned;
-        if (sh.stack < bvh_depth) sh.stack = 32;
     } else {
+        sh.queued = kernel == ST_KERNEL_AUTO ? kDefaultQueued : kernel == ST_KERNEL_QUEUED;
         sh.block = block > 0 ? block : 256;
         sh.grid = grid > 0 ? grid : cus * 8;
         sh.stack = bvh_depth <= 16 && sh.block >= 128 ? 16 : 32;
-        sh.waves = sh.stack == 16 ? 6 : 4;
+        sh.waves = sh.queued ? (sh.stack == 16 ? 5 : 3) : (sh.stack == 16 ? 6 : 4);
     }
-    if (!trace_variant_exists(sh.block, sh.stack, sh.waves)) {
+    if (sh.stack < bvh_depth) sh.stack = 32;
+    if (!trace_variant_exists(sh.block, sh.stack, sh.waves, sh.queued)) {
         sh.stack = 32;
-        sh.waves = 4;
+        sh.waves = sh.queued ? 3 : 4;
+        if (!trace_variant_exists(sh.block, sh.stack, sh.waves, sh.queued)) sh.block = 128;
     }
     const uint64_t waves = (uint64_t)sh.grid * (uint64_t)(sh.block / 64);
     uint64_t chunk = rays / (4 * waves);
     chunk = (chunk + 63) / 64 * 64;
     sh.chunk = (int)std::min<uint64_t>(std::max<uint64_t>(chunk, 64), 2048);
-    sh.kernel = kernel == ST_KERNEL_AUTO ? ST_KERNEL_MEGA : kernel;
+    sh.kernel = sh.queued ? ST_KERNEL_QUEUED : ST_KERNEL_MEGA;
     return sh;
 }
 
@@ -542,7 +549,7 @@ int validate_trace(st_scene s, const st_trace_desc* d, const st_outputs* out) {
     if (d->block_size != 0 && d->block_size != 64 && d->block_size != 128 && d->block_size != 256)
         return fail(ST_ERR_INVALID, "block_size must be 0, 64, 128 or 256");
     if (d->grid_blocks < 0) return fail(ST_ERR_INVALID, "negative grid_blocks");
-    if (d->kernel < 0 || d->kernel > ST_KERNEL_PERSISTENT) return fail(ST_ERR_INVALID, "unknown kernel %d", d->kernel);
+    if (d->kernel < 0 || d->kernel > ST_KERNEL_QUEUED) return fail(ST_ERR_INVALID, "unknown kernel %d", d->kernel);
     if ((d->flags & ST_TRACE_RECORDS) && (!out || !out->records))
         return fail(ST_ERR_INVALID, "ST_TRACE_RECORDS needs outputs->records");
     if (d->flux_nx > 0 && d->flux_ny > 0 && (!out || !out->flux))
@@ -619,23 +626,29 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
         return ST_OK;
     };
     const int cus = device_cu_count();
-    const LaunchShape def = pick_shape(sample_rays, 0, 0, ST_KERNEL_MEGA, s->topo.depth, nullptr);
+    const LaunchShape def = pick_shape(sample_rays, 0, 0, ST_KERNEL_AUTO, s->topo.depth, nullptr);
     double def_ms = 0;
     int rc = time_shape(def, &def_ms);
     LaunchShape best = def;
     double best_ms = def_ms;
     int trials = 1;
-    const int blocks[] = {128, 256}, stacks[] = {16, 32}, wavesv[] = {4, 5, 6, 8}, per_cu[] = {4, 8, 16};
+    const bool log_trials = getenv("ST_TUNE_LOG") != nullptr;  // one stderr line per trial
+    if (log_trials)
+        fprintf(stderr, "st_scene_autotune: default block %d stack %d waves %d queued %d grid %d chunk %d: %.3f ms\n",
+                def.block, def.stack, def.waves, (int)def.queued, def.grid, def.chunk, def_ms);
+    const int blocks[] = {64, 128, 256}, stacks[] = {16, 32}, wavesv[] = {3, 4, 5, 6, 8}, per_cu[] = {4, 8, 16};
+    for (int rf = 0; rf < 2; ++rf)
     for (int b : blocks)
         for (int stk : stacks)
             for (int w : wavesv)
                 for (int pc : per_cu) {
                     if (rc != ST_OK) break;
-                    if (stk < s->topo.depth || !trace_variant_exists(b, stk, w)) continue;
+                    if (stk < s->topo.depth || !trace_variant_exists(b, stk, w, rf != 0)) continue;
                     LaunchShape sh = def;
                     sh.block = b;
                     sh.stack = stk;
                     sh.waves = w;
+                    sh.queued = rf != 0;
                     sh.grid = cus * pc * 256 / b;
                     const uint64_t wv = (uint64_t)sh.grid * (uint64_t)(b / 64);
                     uint64_t chunk = (sample_rays / (4 * wv) + 63) / 64 * 64;
@@ -643,6 +656,9 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
                     double ms = 0;
                     rc = time_shape(sh, &ms);
                     ++trials;
+                    if (log_trials)
+                        fprintf(stderr, "st_scene_autotune: block %d stack %d waves %d queued %d grid %d chunk %d: %.3f ms\n",
+                                b, stk, w, rf, sh.grid, sh.chunk, ms);
                     if (rc == ST_OK && ms < 0.98 * best_ms) {
                         best_ms = ms;
                         best = sh;
@@ -657,6 +673,7 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
     out->grid = best.grid;
     out->stack = best.stack;
     out->waves = best.waves;
+    out->queued = best.queued ? 1 : 0;
     out->trials = trials;
     out->best_ms = best_ms;
     out->default_ms = def_ms;
@@ -808,7 +825,7 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
         ST_TRY(b.d_counter.ensure(sizeof(uint32_t) * 2, "batch counter"));
         uint64_t total_rays = 0;
         for (int k = 0; k < K; ++k) total_rays += d->budgets[k];
-        b.shape = pick_shape(total_rays / K, d->block_size, d->grid_blocks, ST_KERNEL_MEGA, s->topo.depth,
+        b.shape = pick_shape(total_rays / K, d->block_size, d->grid_blocks, ST_KERNEL_AUTO, s->topo.depth,
                              s->tuned_valid ? &s->tuned : nullptr);
         std::vector<JobDev> jobs(K);
         std::vector<uint64_t> prefix(K + 1, 0);

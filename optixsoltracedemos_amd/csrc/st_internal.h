@@ -42,8 +42,10 @@ struct LaunchShape {
     int kernel = ST_KERNEL_MEGA;
     int stack = 32;      // traversal stack entries per lane (16 or 32, >= BVH depth)
     int waves = 4;       // waves per SIMD the register allocation is built for
+    bool queued = false; // continuing rays parked in LDS, traced in full waves
 };
-bool trace_variant_exists(int block, int stack, int waves);
+bool trace_variant_exists(int block, int stack, int waves, bool queued);
+constexpr bool kDefaultQueued = true;  // form used when the caller and the tuner leave it open
 
 // ---- launchers (all asynchronous on `stream`) -----------------------------
 // frames + packing for one scene: elements in element order -> prims in slot order

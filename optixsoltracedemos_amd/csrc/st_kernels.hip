@@ -40,6 +40,34 @@ __device__ __forceinline__ V3 normalize(V3 a) { return rsqrtf(dot(a, a)) * a; }
 __device__ __forceinline__ V3 fma3(float s, V3 a, V3 b) { return v3(fmaf(s, a.x, b.x), fmaf(s, a.y, b.y), fmaf(s, a.z, b.z)); }
 
 // ---------------------------------------------------------------------------
+// Global-memory accessors. The job's pointers reach the kernel inside a struct
+// (by value or from a device array), so the compiler cannot prove they are
+// global and would emit flat instructions, whose completion is tracked
+// together with LDS traffic (the traversal stack). These casts make every
+// scene read, record write and histogram atomic a global-segment instruction.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+#define ST_GLOBAL __attribute__((address_space(1)))
+#define ST_CONST __attribute__((address_space(4)))  // read-only for the whole launch: scalar loads
+__device__ __forceinline__ float4 ld4(const float4* p) {
+    const f4v v = *(const ST_GLOBAL f4v*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float2 ld2(const void* p) {
+    const f2v v = *(const ST_GLOBAL f2v*)p;
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ uint32_t ld1(const uint32_t* p) { return *(const ST_GLOBAL uint32_t*)p; }
+__device__ __forceinline__ void st4(float4* p, float4 v) { *(ST_GLOBAL f4v*)p = f4v{v.x, v.y, v.z, v.w}; }
+__device__ __forceinline__ void st1(uint32_t* p, uint32_t v) { *(ST_GLOBAL uint32_t*)p = v; }
+__device__ __forceinline__ void gadd(void* p, unsigned long long v) {
+    __hip_atomic_fetch_add((ST_GLOBAL unsigned long long*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t gadd32(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_add((ST_GLOBAL uint32_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
 // Sampling
 
 // basis of cuda/sun.cu:77-80: u = normalize(cross(|w.x| > 0.99 ? y : x, w)), v = w x u
@@ -190,8 +218,8 @@ __device__ __forceinline__ bool isect_cylinder(const float4& q0, const float4& q
 
 __device__ __forceinline__ bool isect_prim(const float4* __restrict__ prims, uint32_t kind, int slot, V3 o, V3 d,
                                           float tmin, float tmax, float& t, int& cap) {
-    const float4 q0 = prims[4 * slot + 0], q1 = prims[4 * slot + 1], q2 = prims[4 * slot + 2],
-                 q3 = prims[4 * slot + 3];
+    const float4 q0 = ld4(prims + 4 * slot + 0), q1 = ld4(prims + 4 * slot + 1), q2 = ld4(prims + 4 * slot + 2),
+                 q3 = ld4(prims + 4 * slot + 3);
     cap = 0;
     if (kind == kKindFlat) return isect_flat(q0, q1, q2, q3, o, d, tmin, tmax, t);
     if (kind == kKindParabolic) return isect_parabolic(q0, q1, q2, q3, o, d, tmin, tmax, t);
@@ -214,8 +242,9 @@ __device__ __forceinline__ void traverse(const float4* __restrict__ nodes, const
     int32_t node = 0;
     for (;;) {
         while (node >= 0) {
-            const float4 n0 = nodes[4 * node + 0], n1 = nodes[4 * node + 1], n2 = nodes[4 * node + 2],
-                         n3 = nodes[4 * node + 3];
+            const float4 n0 = ld4(nodes + 4 * node + 0), n1 = ld4(nodes + 4 * node + 1),
+                         n2 = ld4(nodes + 4 * node + 2);
+            const float2 n3 = ld2(nodes + 4 * node + 3);
             float a0 = fmaf(n0.x, ix, -ox), a1 = fmaf(n0.y, ix, -ox);
             float b0 = fmaf(n0.z, iy, -oy), b1 = fmaf(n0.w, iy, -oy);
             float c0 = fmaf(n2.x, iz, -oz), c1 = fmaf(n2.y, iz, -oz);
@@ -245,12 +274,12 @@ __device__ __forceinline__ void traverse(const float4* __restrict__ nodes, const
         const int32_t start = leaf_start(node), cnt = leaf_count(node);
         for (int k = 0; k < cnt; ++k) {
             const int slot = start + k;
-            const uint32_t m = meta[slot];
+            const uint32_t m = ld1(meta + slot);
             float t;
             int cap;
             const float lim = bslot >= 0 ? next_up(tbest) : tbest;
             if (isect_prim(prims, meta_kind(m), slot, o, d, tmin, lim, t, cap)) {
-                if (bslot < 0 || t < tbest || (t == tbest && meta_elem(m) < meta_elem(meta[bslot]))) {
+                if (bslot < 0 || t < tbest || (t == tbest && meta_elem(m) < meta_elem(ld1(meta + bslot)))) {
                     tbest = t;
                     bslot = slot;
                     bcap = cap;
@@ -263,15 +292,19 @@ __device__ __forceinline__ void traverse(const float4* __restrict__ nodes, const
 }
 
 // ---------------------------------------------------------------------------
-struct Counters {
-    uint32_t c[12];  // ST_STAT_* 0..10, [11] = receiver hits on the flux element
-};
+// Event counters: ST_STAT_* 0..10, [11] = hits on the receiver kept aside
+// (agg_slot). Each wave owns kNumCounters u32 in LDS; every lane that reaches
+// a counting site contributes one and the wave adds the number of such lanes
+// with a single LDS atomic, so the counters cost no registers during traversal.
+constexpr int kNumCounters = 12;
+constexpr int kAggCounter = 11;
 
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+__device__ __forceinline__ void count_if(uint32_t* wc, int k, bool pred) {
+    const unsigned long long m = __ballot(pred);
+    const unsigned long long live = __ballot(1);
+    if (m && (int)(threadIdx.x & 63) == __ffsll((long long)live) - 1) atomicAdd(wc + k, (uint32_t)__popcll(m));
 }
+__device__ __forceinline__ void count(uint32_t* wc, int k) { count_if(wc, k, true); }
 
 // base[idx] += 1 for every lane that calls it (lanes may diverge around the
 // call). One round of wave aggregation: the lanes sharing the first caller's
@@ -283,24 +316,24 @@ __device__ __forceinline__ void add_one_aggregated(unsigned long long* base, uin
     const uint32_t lidx = __shfl(idx, leader, 64);
     const unsigned long long same = __ballot(idx == lidx);
     if (idx == lidx) {
-        if ((int)(threadIdx.x & 63) == leader) atomicAdd(base + lidx, (unsigned long long)__popcll(same));
+        if ((int)(threadIdx.x & 63) == leader) gadd(base + lidx, (unsigned long long)__popcll(same));
     } else {
-        atomicAdd(base + idx, 1ull);
+        gadd(base + idx, 1ull);
     }
 }
 
 __device__ __forceinline__ void record(const JobDev& J, uint64_t ray, int slot, float tag, V3 p, uint32_t elem) {
     if (J.records) {
         const uint64_t i = (ray - J.record_base) * (uint64_t)J.max_depth + (uint64_t)slot;
-        J.records[i] = make_float4(tag, p.x, p.y, p.z);
-        if (J.rec_elem) J.rec_elem[i] = elem;
+        st4(J.records + i, make_float4(tag, p.x, p.y, p.z));
+        if (J.rec_elem) st1(J.rec_elem + i, elem);
     }
 }
 
-template <int STRIDE>
-__device__ __forceinline__ void trace_ray(const JobDev& J, const SunFrame& S, uint64_t local, int32_t* stk,
-                                          Counters& cnt) {
-    const uint64_t ray = J.ray_begin + local;
+// Sun sample of one ray: origin on the sun plane, direction in the sun cone
+// (cuda/sun.cu:94-134). Counts the ray and writes its record slot 0.
+__device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame& S, uint64_t ray, uint32_t* wc,
+                                        V3& o, V3& d) {
     const uint32_t rlo = (uint32_t)ray, rhi = (uint32_t)(ray >> 32);
     // ---- sun sample: cuda/sun.cu:94-134
     const U4 u = philox4x32_10(U4{rlo, rhi, 0u, J.job_id}, J.seed_lo, J.seed_hi);
@@ -316,9 +349,8 @@ __device__ __forceinline__ void trace_ray(const JobDev& J, const SunFrame& S, ui
         a = u01(u.x);
         b = u01(u.y);
     }
-    V3 o = v3(fmaf(a, S.e1[0], fmaf(b, S.e2[0], S.o[0])), fmaf(a, S.e1[1], fmaf(b, S.e2[1], S.o[1])),
-              fmaf(a, S.e1[2], fmaf(b, S.e2[2], S.o[2])));
-    V3 d;
+    o = v3(fmaf(a, S.e1[0], fmaf(b, S.e2[0], S.o[0])), fmaf(a, S.e1[1], fmaf(b, S.e2[1], S.o[1])),
+           fmaf(a, S.e1[2], fmaf(b, S.e2[2], S.o[2])));
     {
         const V3 w = v3(S.w[0], S.w[1], S.w[2]), su = v3(S.u[0], S.u[1], S.u[2]), sv = v3(S.v[0], S.v[1], S.v[2]);
         float sp, cp, st, ct;
@@ -332,116 +364,125 @@ __device__ __forceinline__ void trace_ray(const JobDev& J, const SunFrame& S, ui
         }
         d = normalize(fma3(ct, w, st * fma3(cp, su, sp * sv)));
     }
-    cnt.c[ST_STAT_RAYS]++;
+    count(wc, ST_STAT_RAYS);
     record(J, ray, 0, 0.f, o, 0xFFFFFFFEu);
-    int depth = 0, last = 0;
-    float tmin = kTminPrimary;
-    for (;;) {
-        cnt.c[ST_STAT_SEGMENTS]++;
-        float t = kTmax;
-        int slot = -1, cap = 0;
-        traverse<STRIDE>(J.nodes, J.prims, J.meta, o, d, tmin, t, slot, cap, stk);
-        if (slot < 0) {  // __miss__ms, cuda/materials.cu:236-253
-            cnt.c[ST_STAT_ESCAPED]++;
-            break;
-        }
-        const int nd = depth + 1;
-        const uint32_t m = J.meta[slot];
-        const uint32_t kind = meta_kind(m), elem = meta_elem(m);
-        const float4 q0 = J.prims[4 * slot + 0], q1 = J.prims[4 * slot + 1], q2 = J.prims[4 * slot + 2],
-                     q3 = J.prims[4 * slot + 3];
-        const V3 p = fma3(t, d, o);
-        const V3 r = o - v3(q0);
-        if (meta_receiver(m)) {
-            // cuda/materials.cu:86-121 (flat: front face only) and :123-163 (cylinder)
-            if (kind == kKindFlat && !(dot(d, v3(q3)) < 0.f)) {
-                cnt.c[ST_STAT_BACK_FACE]++;
-                break;
-            }
-            if (nd < J.max_depth) {
-                record(J, ray, nd, 2.f, p, elem);
-                last = nd;
-                const bool fe = slot == J.flux_slot;
-                if (slot == J.agg_slot) cnt.c[11]++;
-                else if (J.hits) atomicAdd((unsigned long long*)&J.hits[elem], 1ull);
-                cnt.c[ST_STAT_RECEIVER_HITS]++;
-                const bool direct = depth == 0;
-                cnt.c[ST_STAT_DIRECT_SUN] += direct;
-                cnt.c[ST_STAT_CAP_HITS] += cap;
-                if (fe && J.flux && !cap && (!direct || (J.flags & ST_TRACE_FLUX_DIRECT_SUN))) {
-                    // scripts/post_processing/flux_map_calculator.py:95-179
-                    const float lx = fmaf(t, dot(d, v3(q1)), dot(r, v3(q1)));
-                    const float ly = fmaf(t, dot(d, v3(q2)), dot(r, v3(q2)));
-                    float fx, fy;
-                    if (kind == kKindCylinder) {
-                        const float lz = fmaf(t, dot(d, v3(q3)), dot(r, v3(q3)));
-                        float th = atan2f(-lz, lx);
-                        if (th < 0.f) th += 6.283185307179586f;
-                        fx = th * (float)J.flux_nx * 0.15915494309189535f;
-                        fy = (ly + q1.w) * (float)J.flux_ny / (2.f * q1.w);
-                    } else {
-                        fx = (lx + q0.w) * (float)J.flux_nx / (2.f * q0.w);
-                        fy = (ly + q1.w) * (float)J.flux_ny / (2.f * q1.w);
-                    }
-                    int ix = (int)floorf(fx), iy = (int)floorf(fy);
-                    ix = min(max(ix, 0), J.flux_nx - 1);
-                    iy = min(max(iy, 0), J.flux_ny - 1);
-                    add_one_aggregated((unsigned long long*)J.flux, (uint32_t)(iy * J.flux_nx + ix));
-                    cnt.c[ST_STAT_FLUX_BINNED]++;
-                }
-            } else {
-                cnt.c[ST_STAT_TRUNCATED]++;
-            }
-            break;
-        }
-        // mirror: cuda/materials.cu:26-84, 170-231
-        if (nd >= J.max_depth) {
-            cnt.c[ST_STAT_TRUNCATED]++;
-            break;
-        }
-        record(J, ray, nd, 1.f, p, elem);
-        last = nd;
-        if (J.hits) add_one_aggregated((unsigned long long*)J.hits, elem);
-        cnt.c[ST_STAT_MIRROR_HITS]++;
-        const float4 op = J.optics[elem];
-        if (op.x < 1.f) {
-            const U4 ur = philox4x32_10(U4{rlo, rhi, 2u * (uint32_t)nd, J.job_id}, J.seed_lo, J.seed_hi);
-            if (u01(ur.x) >= op.x) {
-                cnt.c[ST_STAT_ABSORBED]++;
-                break;
-            }
-        }
-        V3 n;
-        if (kind == kKindFlat) {
-            n = v3(q3);
-        } else {
-            const V3 ax = v3(q1), ay = v3(q2), az = v3(q3);
-            const float lx = fmaf(t, dot(d, ax), dot(r, ax)), ly = fmaf(t, dot(d, ay), dot(r, ay));
-            V3 nl;
-            if (kind == kKindParabolic) {
-                nl = normalize(v3(-q2.w * lx, -q3.w * ly, 1.f));
-            } else {
-                const float lz = fmaf(t, dot(d, az), dot(r, az));
-                nl = cap ? v3(0.f, ly < 0.f ? -1.f : 1.f, 0.f) : normalize(v3(lx, 0.f, lz));
-            }
-            n = normalize(fma3(nl.x, ax, fma3(nl.y, ay, nl.z * az)));
-        }
-        if (dot(d, n) > 0.f) n = -n;  // faceforward(n, -d, n)
-        V3 rd;
-        if (op.y > 0.f || op.z > 0.f) {
-            const U4 ue = philox4x32_10(U4{rlo, rhi, 2u * (uint32_t)nd - 1u, J.job_id}, J.seed_lo, J.seed_hi);
-            if (op.y > 0.f) n = perturb(n, op.y, u01(ue.x), u01(ue.y));
-            rd = normalize(fma3(-2.f * dot(n, d), n, d));
-            if (op.z > 0.f) rd = perturb(rd, op.z, u01(ue.z), u01(ue.w));
-        } else {
-            rd = normalize(fma3(-2.f * dot(n, d), n, d));
-        }
-        o = p;
-        d = rd;
-        tmin = kTminBounce;
-        depth = nd;
+}
+
+// One segment of a ray: closest hit from o along d, then the hit element's
+// program. Returns true when the ray continues (o, d, tmin, depth updated for
+// the next segment), false when it ended.
+template <int STRIDE>
+__device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int32_t* stk, uint32_t* wc, V3& o,
+                                              V3& d, float& tmin, int& depth, int& last) {
+    const uint32_t rlo = (uint32_t)ray, rhi = (uint32_t)(ray >> 32);
+    count(wc, ST_STAT_SEGMENTS);
+    float t = kTmax;
+    int slot = -1, cap = 0;
+    traverse<STRIDE>(J.nodes, J.prims, J.meta, o, d, tmin, t, slot, cap, stk);
+    if (slot < 0) {  // __miss__ms, cuda/materials.cu:236-253
+        count(wc, ST_STAT_ESCAPED);
+        return false;
     }
-    // empty slots are all-zero (the reference memsets the buffer, lib/soltrace_system.cpp:125)
+    const int nd = depth + 1;
+    const uint32_t m = ld1(J.meta + slot);
+    const uint32_t kind = meta_kind(m), elem = meta_elem(m);
+    const float4 q0 = ld4(J.prims + 4 * slot + 0), q1 = ld4(J.prims + 4 * slot + 1),
+                 q2 = ld4(J.prims + 4 * slot + 2), q3 = ld4(J.prims + 4 * slot + 3);
+    const V3 p = fma3(t, d, o);
+    const V3 r = o - v3(q0);
+    if (meta_receiver(m)) {
+        // cuda/materials.cu:86-121 (flat: front face only) and :123-163 (cylinder)
+        if (kind == kKindFlat && !(dot(d, v3(q3)) < 0.f)) {
+            count(wc, ST_STAT_BACK_FACE);
+            return false;
+        }
+        if (nd < J.max_depth) {
+            record(J, ray, nd, 2.f, p, elem);
+            last = nd;
+            const bool fe = slot == J.flux_slot;
+            count_if(wc, kAggCounter, slot == J.agg_slot);
+            if (slot != J.agg_slot && J.hits) gadd(J.hits + elem, 1ull);
+            count(wc, ST_STAT_RECEIVER_HITS);
+            const bool direct = depth == 0;
+            count_if(wc, ST_STAT_DIRECT_SUN, direct);
+            count_if(wc, ST_STAT_CAP_HITS, cap != 0);
+            if (fe && J.flux && !cap && (!direct || (J.flags & ST_TRACE_FLUX_DIRECT_SUN))) {
+                // scripts/post_processing/flux_map_calculator.py:95-179
+                const float lx = fmaf(t, dot(d, v3(q1)), dot(r, v3(q1)));
+                const float ly = fmaf(t, dot(d, v3(q2)), dot(r, v3(q2)));
+                float fx, fy;
+                if (kind == kKindCylinder) {
+                    const float lz = fmaf(t, dot(d, v3(q3)), dot(r, v3(q3)));
+                    float th = atan2f(-lz, lx);
+                    if (th < 0.f) th += 6.283185307179586f;
+                    fx = th * (float)J.flux_nx * 0.15915494309189535f;
+                    fy = (ly + q1.w) * (float)J.flux_ny / (2.f * q1.w);
+                } else {
+                    fx = (lx + q0.w) * (float)J.flux_nx / (2.f * q0.w);
+                    fy = (ly + q1.w) * (float)J.flux_ny / (2.f * q1.w);
+                }
+                int ix = (int)floorf(fx), iy = (int)floorf(fy);
+                ix = min(max(ix, 0), J.flux_nx - 1);
+                iy = min(max(iy, 0), J.flux_ny - 1);
+                add_one_aggregated((unsigned long long*)J.flux, (uint32_t)(iy * J.flux_nx + ix));
+                count(wc, ST_STAT_FLUX_BINNED);
+            }
+        } else {
+            count(wc, ST_STAT_TRUNCATED);
+        }
+        return false;
+    }
+    // mirror: cuda/materials.cu:26-84, 170-231
+    if (nd >= J.max_depth) {
+        count(wc, ST_STAT_TRUNCATED);
+        return false;
+    }
+    record(J, ray, nd, 1.f, p, elem);
+    last = nd;
+    if (J.hits) add_one_aggregated((unsigned long long*)J.hits, elem);
+    count(wc, ST_STAT_MIRROR_HITS);
+    const float4 op = ld4(J.optics + elem);
+    if (op.x < 1.f) {
+        const U4 ur = philox4x32_10(U4{rlo, rhi, 2u * (uint32_t)nd, J.job_id}, J.seed_lo, J.seed_hi);
+        if (u01(ur.x) >= op.x) {
+            count(wc, ST_STAT_ABSORBED);
+            return false;
+        }
+    }
+    V3 n;
+    if (kind == kKindFlat) {
+        n = v3(q3);
+    } else {
+        const V3 ax = v3(q1), ay = v3(q2), az = v3(q3);
+        const float lx = fmaf(t, dot(d, ax), dot(r, ax)), ly = fmaf(t, dot(d, ay), dot(r, ay));
+        V3 nl;
+        if (kind == kKindParabolic) {
+            nl = normalize(v3(-q2.w * lx, -q3.w * ly, 1.f));
+        } else {
+            const float lz = fmaf(t, dot(d, az), dot(r, az));
+            nl = cap ? v3(0.f, ly < 0.f ? -1.f : 1.f, 0.f) : normalize(v3(lx, 0.f, lz));
+        }
+        n = normalize(fma3(nl.x, ax, fma3(nl.y, ay, nl.z * az)));
+    }
+    if (dot(d, n) > 0.f) n = -n;  // faceforward(n, -d, n)
+    V3 rd;
+    if (op.y > 0.f || op.z > 0.f) {
+        const U4 ue = philox4x32_10(U4{rlo, rhi, 2u * (uint32_t)nd - 1u, J.job_id}, J.seed_lo, J.seed_hi);
+        if (op.y > 0.f) n = perturb(n, op.y, u01(ue.x), u01(ue.y));
+        rd = normalize(fma3(-2.f * dot(n, d), n, d));
+        if (op.z > 0.f) rd = perturb(rd, op.z, u01(ue.z), u01(ue.w));
+    } else {
+        rd = normalize(fma3(-2.f * dot(n, d), n, d));
+    }
+    o = p;
+    d = rd;
+    tmin = kTminBounce;
+    depth = nd;
+    return true;
+}
+
+// empty slots are all-zero (the reference memsets the buffer, lib/soltrace_system.cpp:125)
+__device__ __forceinline__ void finish_ray(const JobDev& J, uint64_t ray, int last) {
     if (J.records)
         for (int k = last + 1; k < J.max_depth; ++k) record(J, ray, k, 0.f, v3(0.f, 0.f, 0.f), 0xFFFFFFFFu);
 }
@@ -451,18 +492,53 @@ __device__ __forceinline__ void trace_ray(const JobDev& J, const SunFrame& S, ui
 // the stream starts from a clean counter without a memset (graph-safe).
 // jobs == nullptr: a single job passed by value in `single`.
 // STACK: traversal stack entries per lane (>= BVH depth); MINW: waves per SIMD
-// the register allocation must allow (LDS: STACK * BLOCK * 4 bytes per block)
-template <int BLOCK, int STACK, int MINW>
+// the register allocation must allow (LDS: STACK * BLOCK * 4 bytes per block).
+// QUEUED = false: each lane carries one ray through all its segments and the
+// wave moves to the next 64 rays when its slowest ray ends, so after the first
+// segment the lanes whose ray ended sit idle. QUEUED = true: sun rays are traced
+// 64 at a time; a ray that continues after a mirror is parked in the wave's LDS
+// queue (hit point, reflected direction, ray offset, depth) and the queue is
+// traced in full waves of 64 continuing rays (the rest at the chunk's end).
+// Both give identical results: every per-ray quantity depends on the ray index
+// and its own path only, and all accumulations are integer sums.
+constexpr int kQueueCap = 128;  // <= 63 waiting + 64 newly parked
+
+// park the lanes with `cont` set at queue[qn ...]; returns the new count
+__device__ __forceinline__ uint32_t queue_push(float* q, uint32_t qn, bool cont, V3 o, V3 d, uint32_t tag) {
+    const unsigned long long m = __ballot(cont);
+    if (cont) {
+        const uint32_t i = qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        q[0 * kQueueCap + i] = o.x;
+        q[1 * kQueueCap + i] = o.y;
+        q[2 * kQueueCap + i] = o.z;
+        q[3 * kQueueCap + i] = d.x;
+        q[4 * kQueueCap + i] = d.y;
+        q[5 * kQueueCap + i] = d.z;
+        q[6 * kQueueCap + i] = __uint_as_float(tag);
+    }
+    return qn + (uint32_t)__popcll(m);
+}
+
+// tag of a parked ray: offset in the chunk (< 2^16) | depth << 16 | last << 20
+__device__ __forceinline__ uint32_t queue_tag(uint64_t offset, int depth, int last) {
+    return (uint32_t)offset | ((uint32_t)depth << 16) | ((uint32_t)last << 20);
+}
+
+template <int BLOCK, int STACK, int MINW, bool QUEUED>
 __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict__ jobs,
                                                        const uint64_t* __restrict__ chunk_prefix, int32_t n_jobs,
                                                        uint64_t total_chunks, uint32_t* __restrict__ work_counter,
                                                        int32_t chunk_rays, JobDev single) {
     __shared__ int32_t lds_stack[STACK * BLOCK];
+    __shared__ uint32_t lds_counters[BLOCK / 64][kNumCounters];
+    __shared__ float lds_queue[QUEUED ? BLOCK / 64 : 1][QUEUED ? 7 * kQueueCap : 1];
     int32_t* stk = lds_stack + threadIdx.x;
     const int lane = threadIdx.x & 63;
+    uint32_t* wc = lds_counters[threadIdx.x >> 6];
+    if (lane < kNumCounters) wc[lane] = 0;
     for (;;) {
         uint32_t c = 0;
-        if (lane == 0) c = atomicAdd(&work_counter[0], 1u);
+        if (lane == 0) c = gadd32(&work_counter[0], 1u);
         c = __builtin_amdgcn_readfirstlane(c);
         if ((uint64_t)c >= total_chunks) break;
         int lo = 0;
@@ -477,29 +553,78 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
             base = chunk_prefix[lo];
         }
         const JobDev& J = jobs ? jobs[lo] : single;
-        const SunFrame& S = *J.sun;
+        const ST_CONST SunFrame& S = *(const ST_CONST SunFrame*)J.sun;
         const uint64_t first = ((uint64_t)c - base) * (uint64_t)chunk_rays;
         const uint64_t end = min(first + (uint64_t)chunk_rays, J.ray_count);
-        Counters cnt;
-#pragma unroll
-        for (int k = 0; k < 12; ++k) cnt.c[k] = 0;
-        for (uint64_t i = first + lane; i < end; i += 64) trace_ray<BLOCK>(J, S, i, stk, cnt);
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            const uint32_t v = wave_sum(cnt.c[k]);
-            if (lane == 0 && v) {
-                if (k < 11) {
-                    if (J.stats) atomicAdd((unsigned long long*)&J.stats[k], (unsigned long long)v);
-                } else if (J.hits && J.agg_slot >= 0) {
-                    atomicAdd((unsigned long long*)&J.hits[meta_elem(J.meta[J.agg_slot])], (unsigned long long)v);
+        if (QUEUED) {
+            float* q = lds_queue[threadIdx.x >> 6];
+            uint32_t qn = 0;  // wave-uniform queue length
+            const uint64_t base_ray = J.ray_begin + first;
+            // one full (or, at the end, partial) wave of parked rays
+            auto trace_parked = [&](uint32_t take) {
+                qn -= take;
+                bool cont = false;
+                V3 o = v3(0.f, 0.f, 0.f), d = o;
+                uint32_t off = 0;
+                int depth = 0, last = 0;
+                if ((uint32_t)lane < take) {
+                    const uint32_t i = qn + (uint32_t)lane;
+                    o = v3(q[0 * kQueueCap + i], q[1 * kQueueCap + i], q[2 * kQueueCap + i]);
+                    d = v3(q[3 * kQueueCap + i], q[4 * kQueueCap + i], q[5 * kQueueCap + i]);
+                    const uint32_t tag = __float_as_uint(q[6 * kQueueCap + i]);
+                    off = tag & 0xFFFFu;
+                    depth = (int)((tag >> 16) & 15u);
+                    last = (int)((tag >> 20) & 15u);
+                    float tmin = kTminBounce;
+                    const uint64_t ray = base_ray + off;
+                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last);
+                    if (!cont) finish_ray(J, ray, last);
                 }
+                qn = queue_push(q, qn, cont, o, d, queue_tag(off, depth, last));
+            };
+            for (uint64_t b = first; b < end; b += 64) {
+                const uint64_t i = b + (uint64_t)lane;
+                bool cont = false;
+                V3 o = v3(0.f, 0.f, 0.f), d = o;
+                int depth = 0, last = 0;
+                if (i < end) {
+                    const uint64_t ray = J.ray_begin + i;
+                    sun_ray(J, S, ray, wc, o, d);
+                    float tmin = kTminPrimary;
+                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last);
+                    if (!cont) finish_ray(J, ray, last);
+                }
+                qn = queue_push(q, qn, cont, o, d, queue_tag(i - first, depth, last));
+                while (qn >= 64) trace_parked(64);
+            }
+            while (qn > 0) trace_parked(min(qn, 64u));
+        } else {
+            for (uint64_t i = first + lane; i < end; i += 64) {
+                const uint64_t ray = J.ray_begin + i;
+                V3 o, d;
+                sun_ray(J, S, ray, wc, o, d);
+                float tmin = kTminPrimary;
+                int depth = 0, last = 0;
+                while (trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last)) {
+                }
+                finish_ray(J, ray, last);
+            }
+        }
+        // flush this chunk's counters (same wave wrote them: LDS order suffices)
+        const uint32_t v = lane < kNumCounters ? wc[lane] : 0u;
+        if (lane < kNumCounters) wc[lane] = 0;
+        if (v) {
+            if (lane < kAggCounter) {
+                if (J.stats) gadd(J.stats + lane, (unsigned long long)v);
+            } else if (lane == kAggCounter && J.hits && J.agg_slot >= 0) {
+                gadd(J.hits + meta_elem(ld1(J.meta + J.agg_slot)), (unsigned long long)v);
             }
         }
     }
     if (lane == 0) {
         __threadfence();
         const uint32_t waves = gridDim.x * (BLOCK / 64);
-        if (atomicAdd(&work_counter[1], 1u) == waves - 1u) {
+        if (gadd32(&work_counter[1], 1u) == waves - 1u) {
             atomicExch(&work_counter[0], 0u);
             atomicExch(&work_counter[1], 0u);
         }
@@ -786,15 +911,23 @@ hipError_t launch_sun_plane(hipStream_t s, const double* ref_aabb, int32_t n, in
     return hipGetLastError();
 }
 
-template <int BLOCK, int STACK, int MINW>
+template <int BLOCK, int STACK, int MINW, bool QUEUED>
 static void launch_variant(hipStream_t s, int grid, const JobDev* jobs, const uint64_t* chunk_prefix, int32_t n_jobs,
                            uint64_t total_chunks, uint32_t* work_counter, int chunk, const JobDev& single) {
-    hipLaunchKernelGGL((k_trace<BLOCK, STACK, MINW>), dim3(grid), dim3(BLOCK), 0, s, jobs, chunk_prefix, n_jobs,
-                       total_chunks, work_counter, chunk, single);
+    hipLaunchKernelGGL((k_trace<BLOCK, STACK, MINW, QUEUED>), dim3(grid), dim3(BLOCK), 0, s, jobs, chunk_prefix,
+                       n_jobs, total_chunks, work_counter, chunk, single);
 }
 
-bool trace_variant_exists(int block, int stack, int waves) {
-    if (stack == 32) return (block == 64 || block == 128 || block == 256) && (waves == 4 || waves == 5);
+// Compiled shapes. The queued form keeps 3.5 KB of parked rays per wave in
+// LDS on top of the stack, which caps it at 5 waves per SIMD (stack 16) or 3
+// (stack 32); its register budgets are chosen to match.
+bool trace_variant_exists(int block, int stack, int waves, bool queued) {
+    if (queued) {
+        if (stack == 16) return (block == 128 || block == 256) && (waves == 4 || waves == 5);
+        if (stack == 32) return (block == 64 || block == 128) && waves == 3;
+        return false;
+    }
+    if (stack == 32) return ((block == 64 || block == 128) && (waves == 4 || waves == 5)) || (block == 256 && waves == 4);
     if (stack == 16) return (block == 128 || block == 256) && (waves == 4 || waves == 6 || waves == 8);
     return false;
 }
@@ -808,23 +941,30 @@ hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk
     const uint64_t max_blocks = (waves_needed + waves_per_block - 1) / waves_per_block;
     if ((uint64_t)grid > max_blocks) grid = (int)(max_blocks > 0 ? max_blocks : 1);
     const int st = shape.stack, w = shape.waves, b = shape.block, c = shape.chunk;
-#define ST_VARIANT(B, S, W)                                                                             \
-    if (b == B && st == S && w == W) {                                                                 \
-        launch_variant<B, S, W>(s, grid, jobs, chunk_prefix, n_jobs, total_chunks, work_counter, c, single); \
-        return hipGetLastError();                                                                      \
+    const bool r = shape.queued;
+    if (r && (c <= 0 || c > 65535)) return hipErrorInvalidValue;  // parked rays keep a 16-bit chunk offset
+#define ST_VARIANT(B, S, W, Q)                                                                             \
+    if (b == B && st == S && w == W && r == Q) {                                                          \
+        launch_variant<B, S, W, Q>(s, grid, jobs, chunk_prefix, n_jobs, total_chunks, work_counter, c, single); \
+        return hipGetLastError();                                                                         \
     }
-    ST_VARIANT(64, 32, 4)
-    ST_VARIANT(64, 32, 5)
-    ST_VARIANT(128, 32, 4)
-    ST_VARIANT(128, 32, 5)
-    ST_VARIANT(256, 32, 4)
-    ST_VARIANT(256, 32, 5)
-    ST_VARIANT(128, 16, 4)
-    ST_VARIANT(128, 16, 6)
-    ST_VARIANT(128, 16, 8)
-    ST_VARIANT(256, 16, 4)
-    ST_VARIANT(256, 16, 6)
-    ST_VARIANT(256, 16, 8)
+    ST_VARIANT(64, 32, 4, false)
+    ST_VARIANT(64, 32, 5, false)
+    ST_VARIANT(128, 32, 4, false)
+    ST_VARIANT(128, 32, 5, false)
+    ST_VARIANT(256, 32, 4, false)
+    ST_VARIANT(128, 16, 4, false)
+    ST_VARIANT(128, 16, 6, false)
+    ST_VARIANT(128, 16, 8, false)
+    ST_VARIANT(256, 16, 4, false)
+    ST_VARIANT(256, 16, 6, false)
+    ST_VARIANT(256, 16, 8, false)
+    ST_VARIANT(128, 16, 4, true)
+    ST_VARIANT(128, 16, 5, true)
+    ST_VARIANT(256, 16, 4, true)
+    ST_VARIANT(256, 16, 5, true)
+    ST_VARIANT(64, 32, 3, true)
+    ST_VARIANT(128, 32, 3, true)
 #undef ST_VARIANT
     return hipErrorInvalidValue;
 }

@@ -279,7 +279,7 @@ void worker_main(st_server srv) {
         int32_t depth = 0;
         for (ServerJob* j : round) depth = std::max(depth, j->scene->topo.depth);
         const LaunchShape sh =
-            pick_shape(round_rays, srv->cfg.block_size, srv->cfg.grid_blocks, ST_KERNEL_MEGA, depth, nullptr);
+            pick_shape(round_rays, srv->cfg.block_size, srv->cfg.grid_blocks, ST_KERNEL_AUTO, depth, nullptr);
         srv->h_prefix[0] = 0;
         for (size_t i = 0; i < round.size(); ++i) {
             JobDev J = round[i]->dev;

@@ -137,6 +137,29 @@ def test_chunked_and_block_sizes_bitwise(gpu_ctx):
             assert np.array_equal(r[k], full[k]), (bs, grid, k)
 
 
+@pytest.mark.parametrize("scene", ["field", "dish"])
+def test_queued_and_whole_ray_loops_bitwise(gpu_ctx, scene):
+    """ST_KERNEL_QUEUED (sun rays in full waves, continuing rays parked in LDS
+    and traced in full waves) and ST_KERNEL_MEGA (one ray per lane through all
+    bounces) give identical records, flux, element counts and statistics, for
+    every block shape."""
+    if scene == "field":
+        w = scenes.heliostat_field(10_000)
+        n, pos = 1_000_000, _abi.POS_STRATIFIED
+    else:
+        w = scenes.parabolic_dish()
+        n, pos = 400_000, _abi.POS_PHILOX
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=pos)
+    kw = dict(max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny, records=True, seed=11)
+    ref = sc.trace(n, kernel=_abi.KERNEL_MEGA, **kw).numpy()
+    for bs, grid in ((256, 0), (128, 5), (64, 0)):
+        r = sc.trace(n, kernel=_abi.KERNEL_QUEUED, block_size=bs, grid_blocks=grid, **kw).numpy()
+        for k in ("flux", "element_hits", "stats", "records", "record_elements"):
+            assert np.array_equal(r[k], ref[k]), (scene, bs, grid, k)
+    assert ref["stats"][0] == n and ray_outcomes_balance(ref["stats"].astype(np.int64))
+
+
 def test_full_size_field_properties(gpu_ctx):
     """1e8 rays on the 10k field: outcomes balance, flux = binned hits, element
     counts add up, reflectivity 0.93 within binomial noise, run-to-run identical."""
