@@ -84,6 +84,8 @@ enum st_position_sampling {
                                           (reference CSV keeps them; SolTrace drops them,
                                           scripts/post_processing/clean_direct_sun.py:6-16) */
 #define ST_TRACE_ZERO_OUTPUTS 0x4u     /* zero flux / element_hits / stats before tracing */
+#define ST_TRACE_BVH_ONLY 0x8u         /* find sun-ray hits by BVH traversal instead of the
+                                          sun-plane candidate lists (identical results) */
 #define ST_JOB_KEEP_AIM 0x100u         /* server job: trace the scene's stored aims instead of
                                           re-aiming every mirror at the job's sun (bisector,
                                           target = centre of the scene's last receiver) */

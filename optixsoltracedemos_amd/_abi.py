@@ -33,6 +33,7 @@ POS_PHILOX, POS_HALTON, POS_STRATIFIED = 0, 1, 2
 TRACE_RECORDS = 0x1
 TRACE_FLUX_DIRECT_SUN = 0x2
 TRACE_ZERO_OUTPUTS = 0x4
+TRACE_BVH_ONLY = 0x8
 
 KERNEL_AUTO, KERNEL_MEGA, KERNEL_QUEUED = 0, 1, 2
 

@@ -279,6 +279,45 @@ int scene_sun_plane(st_scene s) {
     s->plane.distance = F.distance;
     s->plane.area = F.area;
     s->plane.box_half_angle = F.box_half_angle;
+    return scene_grid(s);
+}
+
+// Largest angle between a sampled sun ray and the sun axis. Pillbox: the half
+// angle. Gaussian: sigma * sqrt(-2 ln u) with u >= 2^-24 (the smallest
+// uniform) = 5.768 sigma. Widened for FP32 direction rounding.
+static double sun_cone_tan(const st_sun_desc& sun) {
+    const double th = sun.shape == ST_SUN_GAUSSIAN ? 5.7686 * sun.param : sun.param;
+    return tan(th * 1.001 + 1e-6);
+}
+
+int scene_grid(st_scene s) {
+    constexpr uint32_t kCells = kGridRes * kGridRes;
+    constexpr uint32_t kMaxAverage = 6, kMaxLongest = 64;  // beyond these the BVH is cheaper
+    hipStream_t st = s->ctx->stream;
+    s->grid_valid = false;
+    ST_TRY(s->d_grid_counts.ensure(sizeof(uint32_t) * kCells, "grid counts"));
+    ST_TRY(s->d_grid_cursor.ensure(sizeof(uint32_t) * kCells, "grid cursor"));
+    ST_TRY(s->d_grid_start.ensure(sizeof(uint32_t) * (kCells + 1), "grid start"));
+    ST_TRY(s->d_grid_stats.ensure(sizeof(uint32_t) * 2, "grid stats"));
+    const double tmax = sun_cone_tan(s->sun);
+    HIP_TRY(launch_fill_u64(st, s->d_grid_counts.ptr, kCells / 2, 0), "grid clear");
+    HIP_TRY(launch_grid_count(st, (const float*)s->d_bounds.ptr, s->n, (const SunFrame*)s->d_sun.ptr, tmax,
+                              (uint32_t*)s->d_grid_counts.ptr, (uint32_t*)s->d_grid_start.ptr,
+                              (uint32_t*)s->d_grid_stats.ptr),
+            "grid count kernels");
+    uint32_t stats[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(stats, s->d_grid_stats.ptr, sizeof(stats), hipMemcpyDeviceToHost, st), "grid stats");
+    HIP_TRY(hipStreamSynchronize(st), "grid sync");
+    s->grid_items = stats[0];
+    s->grid_longest = stats[1];
+    if (stats[0] == 0 || stats[0] > kMaxAverage * kCells || stats[1] > kMaxLongest) return ST_OK;
+    ST_TRY(s->d_grid_items.ensure(sizeof(int32_t) * stats[0], "grid items"));
+    HIP_TRY(launch_fill_u64(st, s->d_grid_cursor.ptr, kCells / 2, 0), "grid cursor clear");
+    HIP_TRY(launch_grid_fill(st, (const float*)s->d_bounds.ptr, s->n, (const SunFrame*)s->d_sun.ptr, tmax,
+                             (const uint32_t*)s->d_grid_start.ptr, (uint32_t*)s->d_grid_cursor.ptr,
+                             (int32_t*)s->d_grid_items.ptr),
+            "grid fill kernel");
+    s->grid_valid = true;
     return ST_OK;
 }
 }  // namespace st
@@ -505,6 +544,10 @@ int fill_job(st_scene s, const st_trace_desc* d, const st_outputs* out, const vo
     J->optics = (const float4*)s->d_optics.ptr;
     J->nodes = (const float4*)nodes;
     J->sun = (const SunFrame*)sun;
+    if (s->grid_valid && prims == s->d_prims.ptr && sun == s->d_sun.ptr && !(d->flags & ST_TRACE_BVH_ONLY)) {
+        J->grid_start = (const uint32_t*)s->d_grid_start.ptr;
+        J->grid_items = (const int32_t*)s->d_grid_items.ptr;
+    }
     J->ray_begin = d->ray_begin;
     J->ray_count = d->ray_count ? d->ray_count : d->num_rays - d->ray_begin;
     J->record_base = d->ray_begin;

@@ -94,6 +94,11 @@ struct st_scene_s {
     std::vector<int32_t> level_offsets;  // host: nodes of level l are level_nodes[off[l], off[l+1])
     st::DevBuf d_elems, d_elem_slot, d_frames, d_ref_aabb, d_bounds, d_prims, d_meta, d_optics, d_nodes, d_child,
         d_level_nodes, d_sun, d_sun_unit, d_aim, d_zrot, d_counter;
+    // sun-plane candidate grid (rebuilt with the sun plane; unused when its
+    // lists would be long, e.g. at grazing sun elevations)
+    st::DevBuf d_grid_counts, d_grid_start, d_grid_cursor, d_grid_items, d_grid_stats;
+    bool grid_valid = false;
+    uint32_t grid_items = 0, grid_longest = 0;
     bool sun_set = false;
     st_sun_desc sun{};
     double sun_unit[3] = {0, 0, 1};
@@ -107,7 +112,8 @@ struct st_scene_s {
 
 namespace st {
 int scene_refresh(st_scene s, bool use_overrides);
-int scene_sun_plane(st_scene s);
+int scene_sun_plane(st_scene s);  // also rebuilds the sun-plane candidate grid
+int scene_grid(st_scene s);
 int resolve_flux_element(st_scene s, int32_t requested, int32_t nx, int32_t ny, int32_t* slot);
 int fill_job(st_scene s, const st_trace_desc* d, const st_outputs* out, const void* prims, const void* nodes,
              const void* sun, JobDev* J);

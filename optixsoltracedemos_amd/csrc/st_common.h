@@ -180,6 +180,7 @@ ST_INLINE void bisector_aim_point(const double origin[3], const double sun_unit[
 // FP32 copies for the per-ray work).
 struct SunFrame {
     double v0[3], v1[3], v2[3], v3[3];
+    double sun[3];                  // unit vector towards the sun
     double distance, area, box_half_angle, dni;
     float o[3], e1[3], e2[3];       // v0, v1 - v0, v3 - v0
     float w[3], u[3], v[3];         // sampling basis, cuda/sun.cu:77-80

@@ -33,7 +33,11 @@ struct JobDev {
     uint64_t* stats;         // [ST_STAT_COUNT]
     float4* records;         // [rays][max_depth]
     uint32_t* rec_elem;      // [rays][max_depth]
+    const uint32_t* grid_start;  // [kGridRes^2 + 1] sun-plane cell lists, or null (BVH for sun rays)
+    const int32_t* grid_items;   // primitive slots of every cell
 };
+
+constexpr int kGridRes = 256;  // sun-plane candidate grid, the stratified-sampling cells
 
 struct LaunchShape {
     int block = 256;     // threads per block (multiple of 64)
@@ -72,6 +76,13 @@ hipError_t launch_sun_plane(hipStream_t s, const double* ref_aabb, int32_t n, in
 hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk_prefix, int32_t n_jobs,
                         uint64_t total_chunks, uint32_t* work_counter, const LaunchShape& shape,
                         const JobDev& single);
+// sun-plane candidate grid: counts (zeroed, kGridRes^2 u32) -> start
+// (kGridRes^2 + 1), stats[0] = total items, stats[1] = longest cell list
+hipError_t launch_grid_count(hipStream_t s, const float* bounds, int32_t n, const SunFrame* sun, double tan_max,
+                             uint32_t* counts, uint32_t* start, uint32_t* stats);
+// cursor zeroed (kGridRes^2 u32); items sized stats[0]
+hipError_t launch_grid_fill(hipStream_t s, const float* bounds, int32_t n, const SunFrame* sun, double tan_max,
+                            const uint32_t* start, uint32_t* cursor, int32_t* items);
 hipError_t launch_fill_u64(hipStream_t s, void* p, uint64_t n_u64, uint64_t value);
 hipError_t launch_flux_watts(hipStream_t s, const int64_t* flux, int32_t nbins, const SunFrame* sun,
                              double num_rays, double bin_area, double* out, double* power_per_ray_dev);

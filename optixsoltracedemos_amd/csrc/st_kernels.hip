@@ -230,6 +230,33 @@ __device__ __forceinline__ bool isect_prim(const float4* __restrict__ prims, uin
 // Closest hit over the BVH: smallest t, equal t -> lower element index.
 __device__ __forceinline__ float next_up(float t) { return __int_as_float(__float_as_int(t) + 1); }
 
+// One candidate of the closest-hit search. The result over a set of candidates
+// (smallest t, equal t -> lower element index; next_up keeps equal-t hits in
+// play) does not depend on the order the candidates are tested in.
+__device__ __forceinline__ void test_prim(const float4* __restrict__ prims, const uint32_t* __restrict__ meta,
+                                          int slot, V3 o, V3 d, float tmin, float& tbest, int& bslot, int& bcap) {
+    const uint32_t m = ld1(meta + slot);
+    float t;
+    int cap;
+    const float lim = bslot >= 0 ? next_up(tbest) : tbest;
+    if (isect_prim(prims, meta_kind(m), slot, o, d, tmin, lim, t, cap)) {
+        if (bslot < 0 || t < tbest || (t == tbest && meta_elem(m) < meta_elem(ld1(meta + bslot)))) {
+            tbest = t;
+            bslot = slot;
+            bcap = cap;
+        }
+    }
+}
+
+// Closest hit of a ray leaving the sun plane: every primitive such a ray can
+// reach is listed under its sun-plane cell (conservative binning, k_grid_*),
+// so testing the cell's list equals a full BVH traversal.
+__device__ __forceinline__ void closest_in_cell(const JobDev& J, uint32_t cell, V3 o, V3 d, float tmin, float& tbest,
+                                                int& bslot, int& bcap) {
+    const uint32_t b = ld1(J.grid_start + cell), e = ld1(J.grid_start + cell + 1);
+    for (uint32_t k = b; k < e; ++k) test_prim(J.prims, J.meta, (int)ld1((const uint32_t*)J.grid_items + k), o, d, tmin, tbest, bslot, bcap);
+}
+
 template <int STRIDE>
 __device__ __forceinline__ void traverse(const float4* __restrict__ nodes, const float4* __restrict__ prims,
                                          const uint32_t* __restrict__ meta, V3 o, V3 d, float tmin, float& tbest,
@@ -272,20 +299,7 @@ __device__ __forceinline__ void traverse(const float4* __restrict__ nodes, const
             }
         }
         const int32_t start = leaf_start(node), cnt = leaf_count(node);
-        for (int k = 0; k < cnt; ++k) {
-            const int slot = start + k;
-            const uint32_t m = ld1(meta + slot);
-            float t;
-            int cap;
-            const float lim = bslot >= 0 ? next_up(tbest) : tbest;
-            if (isect_prim(prims, meta_kind(m), slot, o, d, tmin, lim, t, cap)) {
-                if (bslot < 0 || t < tbest || (t == tbest && meta_elem(m) < meta_elem(ld1(meta + bslot)))) {
-                    tbest = t;
-                    bslot = slot;
-                    bcap = cap;
-                }
-            }
-        }
+        for (int k = 0; k < cnt; ++k) test_prim(prims, meta, start + k, o, d, tmin, tbest, bslot, bcap);
         if (sp == 0) return;
         node = stk[--sp * STRIDE];
     }
@@ -333,21 +347,25 @@ __device__ __forceinline__ void record(const JobDev& J, uint64_t ray, int slot, 
 // Sun sample of one ray: origin on the sun plane, direction in the sun cone
 // (cuda/sun.cu:94-134). Counts the ray and writes its record slot 0.
 __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame& S, uint64_t ray, uint32_t* wc,
-                                        V3& o, V3& d) {
+                                        V3& o, V3& d, uint32_t& cell) {
     const uint32_t rlo = (uint32_t)ray, rhi = (uint32_t)(ray >> 32);
     // ---- sun sample: cuda/sun.cu:94-134
     const U4 u = philox4x32_10(U4{rlo, rhi, 0u, J.job_id}, J.seed_lo, J.seed_hi);
     float a, b;
     if (S.position_sampling == ST_POS_STRATIFIED) {
-        const uint32_t cell = stratified_cell(ray, J.seed_lo, J.seed_hi, J.job_id);
+        cell = stratified_cell(ray, J.seed_lo, J.seed_hi, J.job_id);
         a = ((float)(cell & 255u) + u01(u.x)) * (1.f / 256.f);
         b = ((float)(cell >> 8) + u01(u.y)) * (1.f / 256.f);
-    } else if (S.position_sampling == ST_POS_HALTON) {
-        a = halton2(rlo);
-        b = halton3(rlo);
     } else {
-        a = u01(u.x);
-        b = u01(u.y);
+        if (S.position_sampling == ST_POS_HALTON) {
+            a = halton2(rlo);
+            b = halton3(rlo);
+        } else {
+            a = u01(u.x);
+            b = u01(u.y);
+        }
+        // a * 256 is exact in FP32: the cell of the sampled plane coordinates
+        cell = (min((uint32_t)(b * 256.f), 255u) << 8) | min((uint32_t)(a * 256.f), 255u);
     }
     o = v3(fmaf(a, S.e1[0], fmaf(b, S.e2[0], S.o[0])), fmaf(a, S.e1[1], fmaf(b, S.e2[1], S.o[1])),
            fmaf(a, S.e1[2], fmaf(b, S.e2[2], S.o[2])));
@@ -373,12 +391,13 @@ __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame
 // the next segment), false when it ended.
 template <int STRIDE>
 __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int32_t* stk, uint32_t* wc, V3& o,
-                                              V3& d, float& tmin, int& depth, int& last) {
+                                              V3& d, float& tmin, int& depth, int& last, int cell) {
     const uint32_t rlo = (uint32_t)ray, rhi = (uint32_t)(ray >> 32);
     count(wc, ST_STAT_SEGMENTS);
     float t = kTmax;
     int slot = -1, cap = 0;
-    traverse<STRIDE>(J.nodes, J.prims, J.meta, o, d, tmin, t, slot, cap, stk);
+    if (cell >= 0) closest_in_cell(J, (uint32_t)cell, o, d, tmin, t, slot, cap);
+    else traverse<STRIDE>(J.nodes, J.prims, J.meta, o, d, tmin, t, slot, cap, stk);
     if (slot < 0) {  // __miss__ms, cuda/materials.cu:236-253
         count(wc, ST_STAT_ESCAPED);
         return false;
@@ -577,7 +596,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                     last = (int)((tag >> 20) & 15u);
                     float tmin = kTminBounce;
                     const uint64_t ray = base_ray + off;
-                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last);
+                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, -1);
                     if (!cont) finish_ray(J, ray, last);
                 }
                 qn = queue_push(q, qn, cont, o, d, queue_tag(off, depth, last));
@@ -589,9 +608,11 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                 int depth = 0, last = 0;
                 if (i < end) {
                     const uint64_t ray = J.ray_begin + i;
-                    sun_ray(J, S, ray, wc, o, d);
+                    uint32_t cell;
+                    sun_ray(J, S, ray, wc, o, d, cell);
                     float tmin = kTminPrimary;
-                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last);
+                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last,
+                                                J.grid_start ? (int)cell : -1);
                     if (!cont) finish_ray(J, ray, last);
                 }
                 qn = queue_push(q, qn, cont, o, d, queue_tag(i - first, depth, last));
@@ -602,11 +623,13 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
             for (uint64_t i = first + lane; i < end; i += 64) {
                 const uint64_t ray = J.ray_begin + i;
                 V3 o, d;
-                sun_ray(J, S, ray, wc, o, d);
+                uint32_t cell;
+                sun_ray(J, S, ray, wc, o, d, cell);
                 float tmin = kTminPrimary;
                 int depth = 0, last = 0;
-                while (trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last)) {
-                }
+                if (trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, J.grid_start ? (int)cell : -1))
+                    while (trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, -1)) {
+                    }
                 finish_ray(J, ray, last);
             }
         }
@@ -820,6 +843,7 @@ __global__ __launch_bounds__(256) void k_sun_plane(const double* __restrict__ re
         e1[j] = F.v1[j] - F.v0[j];
         e2[j] = F.v3[j] - F.v0[j];
     }
+    for (int j = 0; j < 3; ++j) F.sun[j] = s[j];
     F.distance = d;
     F.area = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]) * sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
     F.box_half_angle = angle;
@@ -845,6 +869,104 @@ __global__ __launch_bounds__(256) void k_sun_plane(const double* __restrict__ re
     F.shape = sun.shape;
     F.position_sampling = sun.position_sampling;
     out[k] = F;
+}
+
+// ---------------------------------------------------------------------------
+// Sun-plane candidate grid: kGridRes x kGridRes cells over the sun plane's
+// (a, b) parameters. A primitive is listed in every cell from which a sun ray
+// can reach its traversal box: the box corners projected along the sun onto
+// the plane, widened by the largest lateral drift of a ray in the sun cone over
+// the axial distance from the plane (tan_max * (distance - corner . sun)) plus
+// 1 mm and 1e-6 of the plane's extent for FP32 ray origins.
+__device__ __forceinline__ bool grid_range(const float* __restrict__ bounds, int slot, const SunFrame& F,
+                                           double tan_max, int r[4]) {
+    const double e1[3] = {F.v1[0] - F.v0[0], F.v1[1] - F.v0[1], F.v1[2] - F.v0[2]};
+    const double e2[3] = {F.v3[0] - F.v0[0], F.v3[1] - F.v0[1], F.v3[2] - F.v0[2]};
+    const double l1 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+    const double l2 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+    if (!(l1 > 0.0) || !(l2 > 0.0)) return false;
+    double amin = 1e300, amax = -1e300, bmin = 1e300, bmax = -1e300, smin = 1e300;
+    for (int c = 0; c < 8; ++c) {
+        const double p[3] = {(double)bounds[6 * slot + ((c & 1) ? 3 : 0)], (double)bounds[6 * slot + 1 + ((c & 2) ? 3 : 0)],
+                             (double)bounds[6 * slot + 2 + ((c & 4) ? 3 : 0)]};
+        const double q[3] = {p[0] - F.v0[0], p[1] - F.v0[1], p[2] - F.v0[2]};
+        const double a = (q[0] * e1[0] + q[1] * e1[1] + q[2] * e1[2]) / l1;
+        const double b = (q[0] * e2[0] + q[1] * e2[1] + q[2] * e2[2]) / l2;
+        amin = fmin(amin, a);
+        amax = fmax(amax, a);
+        bmin = fmin(bmin, b);
+        bmax = fmax(bmax, b);
+        smin = fmin(smin, p[0] * F.sun[0] + p[1] * F.sun[1] + p[2] * F.sun[2]);
+    }
+    const double extent = sqrt(l1) + sqrt(l2) + fabs(F.v0[0]) + fabs(F.v0[1]) + fabs(F.v0[2]);
+    const double m = fmax(F.distance - smin, 0.0) * tan_max + 1e-3 + 1e-6 * extent;
+    const double ma = m / sqrt(l1), mb = m / sqrt(l2);
+    if (amax + ma < 0.0 || amin - ma > 1.0 || bmax + mb < 0.0 || bmin - mb > 1.0) return false;
+    r[0] = (int)fmax(0.0, floor((amin - ma) * kGridRes));
+    r[1] = (int)fmin(kGridRes - 1.0, floor((amax + ma) * kGridRes));
+    r[2] = (int)fmax(0.0, floor((bmin - mb) * kGridRes));
+    r[3] = (int)fmin(kGridRes - 1.0, floor((bmax + mb) * kGridRes));
+    return r[0] <= r[1] && r[2] <= r[3];
+}
+
+__global__ void k_grid_count(const float* __restrict__ bounds, int32_t n, const SunFrame* __restrict__ sun,
+                             double tan_max, uint32_t* __restrict__ counts) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= n) return;
+    int r[4];
+    if (!grid_range(bounds, slot, *sun, tan_max, r)) return;
+    for (int b = r[2]; b <= r[3]; ++b)
+        for (int a = r[0]; a <= r[1]; ++a) atomicAdd(&counts[b * kGridRes + a], 1u);
+}
+
+// exclusive prefix sum of kGridRes^2 counts (one block of 1024); start[cells]
+// = total, stats[0] = total, stats[1] = longest list
+__global__ __launch_bounds__(1024) void k_grid_scan(const uint32_t* __restrict__ counts, uint32_t* __restrict__ start,
+                                                    uint32_t* __restrict__ stats) {
+    constexpr int kCells = kGridRes * kGridRes, kPer = kCells / 1024;
+    __shared__ uint32_t part[1024], top[1024];
+    const int t = threadIdx.x;
+    uint32_t sum = 0, mx = 0;
+    for (int i = 0; i < kPer; ++i) {
+        const uint32_t c = counts[t * kPer + i];
+        sum += c;
+        mx = max(mx, c);
+    }
+    part[t] = sum;
+    top[t] = mx;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        const uint32_t w = t >= off ? top[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        top[t] = max(top[t], w);
+        __syncthreads();
+    }
+    uint32_t run = t ? part[t - 1] : 0u;
+    for (int i = 0; i < kPer; ++i) {
+        start[t * kPer + i] = run;
+        run += counts[t * kPer + i];
+    }
+    if (t == 1023) {
+        start[kCells] = part[1023];
+        stats[0] = part[1023];
+        stats[1] = top[1023];
+    }
+}
+
+__global__ void k_grid_fill(const float* __restrict__ bounds, int32_t n, const SunFrame* __restrict__ sun,
+                            double tan_max, const uint32_t* __restrict__ start, uint32_t* __restrict__ cursor,
+                            int32_t* __restrict__ items) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= n) return;
+    int r[4];
+    if (!grid_range(bounds, slot, *sun, tan_max, r)) return;
+    for (int b = r[2]; b <= r[3]; ++b)
+        for (int a = r[0]; a <= r[1]; ++a) {
+            const int c = b * kGridRes + a;
+            items[start[c] + atomicAdd(&cursor[c], 1u)] = slot;
+        }
 }
 
 __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t value) {
@@ -967,6 +1089,22 @@ hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk
     ST_VARIANT(128, 32, 3, true)
 #undef ST_VARIANT
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_grid_count(hipStream_t s, const float* bounds, int32_t n, const SunFrame* sun, double tan_max,
+                             uint32_t* counts, uint32_t* start, uint32_t* stats) {
+    hipLaunchKernelGGL(k_grid_count, dim3((n + 127) / 128), dim3(128), 0, s, bounds, n, sun, tan_max, counts);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_grid_scan, dim3(1), dim3(1024), 0, s, counts, start, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_fill(hipStream_t s, const float* bounds, int32_t n, const SunFrame* sun, double tan_max,
+                            const uint32_t* start, uint32_t* cursor, int32_t* items) {
+    hipLaunchKernelGGL(k_grid_fill, dim3((n + 127) / 128), dim3(128), 0, s, bounds, n, sun, tan_max, start, cursor,
+                       items);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_u64(hipStream_t s, void* p, uint64_t n_u64, uint64_t value) {
