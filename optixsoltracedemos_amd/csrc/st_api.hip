@@ -279,15 +279,61 @@ int scene_sun_plane(st_scene s) {
     s->plane.distance = F.distance;
     s->plane.area = F.area;
     s->plane.box_half_angle = F.box_half_angle;
-    return scene_grid(s);
+    ST_TRY(scene_grid(s));
+    return scene_bounce_lists(s);
 }
 
 // Largest angle between a sampled sun ray and the sun axis. Pillbox: the half
 // angle. Gaussian: sigma * sqrt(-2 ln u) with u >= 2^-24 (the smallest
 // uniform) = 5.768 sigma. Widened for FP32 direction rounding.
-static double sun_cone_tan(const st_sun_desc& sun) {
+static double sun_cone_angle(const st_sun_desc& sun) {
     const double th = sun.shape == ST_SUN_GAUSSIAN ? 5.7686 * sun.param : sun.param;
-    return tan(th * 1.001 + 1e-6);
+    return th * 1.001 + 1e-6;
+}
+static double sun_cone_tan(const st_sun_desc& sun) { return tan(sun_cone_angle(sun)); }
+
+int scene_bounce_lists(st_scene s) {
+    hipStream_t st = s->ctx->stream;
+    s->bounce_valid = false;
+    const size_t n = (size_t)s->n;
+    ST_TRY(s->d_bounce_counts.ensure(sizeof(uint32_t) * n, "bounce counts"));
+    ST_TRY(s->d_bounce_range.ensure(sizeof(uint2) * n, "bounce ranges"));
+    const double theta = sun_cone_angle(s->sun);
+    HIP_TRY(launch_bounce_lists(st, (const st_element_f64*)s->d_elems.ptr, s->n, (const int32_t*)s->d_elem_slot.ptr,
+                                (const double*)s->d_frames.ptr, (const float*)s->d_bounds.ptr,
+                                (const float4*)s->d_nodes.ptr, (const SunFrame*)s->d_sun.ptr, theta,
+                                (uint32_t*)s->d_bounce_counts.ptr, nullptr, nullptr),
+            "bounce count kernel");
+    s->h_bounce_counts.resize(n);
+    s->h_bounce_range.resize(n);
+    HIP_TRY(hipMemcpyAsync(s->h_bounce_counts.data(), s->d_bounce_counts.ptr, sizeof(uint32_t) * n,
+                           hipMemcpyDeviceToHost, st),
+            "bounce counts download");
+    HIP_TRY(hipStreamSynchronize(st), "bounce sync");
+    uint64_t total = 0;
+    for (size_t e = 0; e < n; ++e) {
+        const uint32_t c = s->h_bounce_counts[e];
+        if (c == kNoBounceList || total + c > 0x7FFFFFFFull) {
+            s->h_bounce_range[e] = make_uint2(1u, 0u);
+        } else {
+            s->h_bounce_range[e] = make_uint2((uint32_t)total, (uint32_t)(total + c));
+            total += c;
+        }
+    }
+    s->bounce_items = total;
+    if (total == 0) return ST_OK;
+    ST_TRY(s->d_bounce_items.ensure(sizeof(int32_t) * total, "bounce items"));
+    HIP_TRY(hipMemcpyAsync(s->d_bounce_range.ptr, s->h_bounce_range.data(), sizeof(uint2) * n, hipMemcpyHostToDevice,
+                           st),
+            "bounce ranges upload");
+    HIP_TRY(launch_bounce_lists(st, (const st_element_f64*)s->d_elems.ptr, s->n, (const int32_t*)s->d_elem_slot.ptr,
+                                (const double*)s->d_frames.ptr, (const float*)s->d_bounds.ptr,
+                                (const float4*)s->d_nodes.ptr, (const SunFrame*)s->d_sun.ptr, theta, nullptr,
+                                (const uint2*)s->d_bounce_range.ptr, (int32_t*)s->d_bounce_items.ptr),
+            "bounce fill kernel");
+    HIP_TRY(hipStreamSynchronize(st), "bounce sync");  // the host range array is reused by the next rebuild
+    s->bounce_valid = true;
+    return ST_OK;
 }
 
 int scene_grid(st_scene s) {
@@ -547,6 +593,10 @@ int fill_job(st_scene s, const st_trace_desc* d, const st_outputs* out, const vo
     if (s->grid_valid && prims == s->d_prims.ptr && sun == s->d_sun.ptr && !(d->flags & ST_TRACE_BVH_ONLY)) {
         J->grid_start = (const uint32_t*)s->d_grid_start.ptr;
         J->grid_items = (const int32_t*)s->d_grid_items.ptr;
+    }
+    if (s->bounce_valid && prims == s->d_prims.ptr && sun == s->d_sun.ptr && !(d->flags & ST_TRACE_BVH_ONLY)) {
+        J->bounce_range = (const uint2*)s->d_bounce_range.ptr;
+        J->bounce_items = (const int32_t*)s->d_bounce_items.ptr;
     }
     J->ray_begin = d->ray_begin;
     J->ray_count = d->ray_count ? d->ray_count : d->num_rays - d->ray_begin;

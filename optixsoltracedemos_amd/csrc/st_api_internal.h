@@ -99,6 +99,12 @@ struct st_scene_s {
     st::DevBuf d_grid_counts, d_grid_start, d_grid_cursor, d_grid_items, d_grid_stats;
     bool grid_valid = false;
     uint32_t grid_items = 0, grid_longest = 0;
+    // candidate lists of first reflections, per element (rebuilt with the sun plane)
+    st::DevBuf d_bounce_counts, d_bounce_range, d_bounce_items;
+    std::vector<uint32_t> h_bounce_counts;
+    std::vector<uint2> h_bounce_range;
+    bool bounce_valid = false;
+    uint64_t bounce_items = 0;
     bool sun_set = false;
     st_sun_desc sun{};
     double sun_unit[3] = {0, 0, 1};
@@ -114,6 +120,7 @@ namespace st {
 int scene_refresh(st_scene s, bool use_overrides);
 int scene_sun_plane(st_scene s);  // also rebuilds the sun-plane candidate grid
 int scene_grid(st_scene s);
+int scene_bounce_lists(st_scene s);
 int resolve_flux_element(st_scene s, int32_t requested, int32_t nx, int32_t ny, int32_t* slot);
 int fill_job(st_scene s, const st_trace_desc* d, const st_outputs* out, const void* prims, const void* nodes,
              const void* sun, JobDev* J);

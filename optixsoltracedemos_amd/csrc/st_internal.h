@@ -35,6 +35,9 @@ struct JobDev {
     uint32_t* rec_elem;      // [rays][max_depth]
     const uint32_t* grid_start;  // [kGridRes^2 + 1] sun-plane cell lists, or null (BVH for sun rays)
     const int32_t* grid_items;   // primitive slots of every cell
+    const uint2* bounce_range;   // [n_elements] (begin, end) list of a first reflection off the element;
+                                 // begin > end: none (BVH). Null: BVH for every reflected ray
+    const int32_t* bounce_items; // primitive slots
 };
 
 constexpr int kGridRes = 256;  // sun-plane candidate grid, the stratified-sampling cells
@@ -83,6 +86,14 @@ hipError_t launch_grid_count(hipStream_t s, const float* bounds, int32_t n, cons
 // cursor zeroed (kGridRes^2 u32); items sized stats[0]
 hipError_t launch_grid_fill(hipStream_t s, const float* bounds, int32_t n, const SunFrame* sun, double tan_max,
                             const uint32_t* start, uint32_t* cursor, int32_t* items);
+// candidate lists of first reflections: count pass (range == null) writes
+// counts[e] (kNoBounceList: none); fill pass writes items[range[e].x ...]
+constexpr uint32_t kNoBounceList = 0xFFFFFFFFu;
+constexpr uint32_t kMaxBounceList = 64;
+hipError_t launch_bounce_lists(hipStream_t s, const st_element_f64* elems, int32_t n, const int32_t* elem_slot,
+                               const double* frames, const float* bounds_slot, const float4* nodes,
+                               const SunFrame* sun, double theta_sun, uint32_t* counts, const uint2* range,
+                               int32_t* items);
 hipError_t launch_fill_u64(hipStream_t s, void* p, uint64_t n_u64, uint64_t value);
 hipError_t launch_flux_watts(hipStream_t s, const int64_t* flux, int32_t nbins, const SunFrame* sun,
                              double num_rays, double bin_area, double* out, double* power_per_ray_dev);

@@ -248,13 +248,32 @@ __device__ __forceinline__ void test_prim(const float4* __restrict__ prims, cons
     }
 }
 
-// Closest hit of a ray leaving the sun plane: every primitive such a ray can
-// reach is listed under its sun-plane cell (conservative binning, k_grid_*),
-// so testing the cell's list equals a full BVH traversal.
-__device__ __forceinline__ void closest_in_cell(const JobDev& J, uint32_t cell, V3 o, V3 d, float tmin, float& tbest,
-                                                int& bslot, int& bcap) {
-    const uint32_t b = ld1(J.grid_start + cell), e = ld1(J.grid_start + cell + 1);
-    for (uint32_t k = b; k < e; ++k) test_prim(J.prims, J.meta, (int)ld1((const uint32_t*)J.grid_items + k), o, d, tmin, tbest, bslot, bcap);
+// Closest hit over a candidate list (items[b, e)): a sun ray's sun-plane cell
+// list or a first reflection's list of its mirror. Every primitive such a ray
+// can reach is listed (conservative binning, k_grid_* / k_bounce_lists), so
+// the result equals a full BVH traversal.
+__device__ __forceinline__ void closest_in_list(const JobDev& J, const int32_t* __restrict__ items, uint32_t b,
+                                                uint32_t e, V3 o, V3 d, float tmin, float& tbest, int& bslot,
+                                                int& bcap) {
+    for (uint32_t k = b; k < e; ++k)
+        test_prim(J.prims, J.meta, (int)ld1((const uint32_t*)items + k), o, d, tmin, tbest, bslot, bcap);
+}
+
+// candidate list of a segment: lb > le means none (BVH traversal)
+struct SegList {
+    const int32_t* items;
+    uint32_t lb, le;
+};
+__device__ __forceinline__ SegList no_list() { return SegList{nullptr, 1u, 0u}; }
+__device__ __forceinline__ SegList cell_list(const JobDev& J, uint32_t cell) {
+    if (!J.grid_start) return no_list();
+    return SegList{J.grid_items, ld1(J.grid_start + cell), ld1(J.grid_start + cell + 1)};
+}
+// first reflection off element `from` (sun-cone incidence); later ones: BVH
+__device__ __forceinline__ SegList bounce_list(const JobDev& J, int depth, int from) {
+    if (!J.bounce_range || depth != 1 || from < 0) return no_list();
+    const float2 r = ld2(J.bounce_range + from);
+    return SegList{J.bounce_items, __float_as_uint(r.x), __float_as_uint(r.y)};
 }
 
 template <int STRIDE>
@@ -391,12 +410,12 @@ __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame
 // the next segment), false when it ended.
 template <int STRIDE>
 __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int32_t* stk, uint32_t* wc, V3& o,
-                                              V3& d, float& tmin, int& depth, int& last, int cell) {
+                                              V3& d, float& tmin, int& depth, int& last, SegList L, int& from) {
     const uint32_t rlo = (uint32_t)ray, rhi = (uint32_t)(ray >> 32);
     count(wc, ST_STAT_SEGMENTS);
     float t = kTmax;
     int slot = -1, cap = 0;
-    if (cell >= 0) closest_in_cell(J, (uint32_t)cell, o, d, tmin, t, slot, cap);
+    if (L.lb <= L.le) closest_in_list(J, L.items, L.lb, L.le, o, d, tmin, t, slot, cap);
     else traverse<STRIDE>(J.nodes, J.prims, J.meta, o, d, tmin, t, slot, cap, stk);
     if (slot < 0) {  // __miss__ms, cuda/materials.cu:236-253
         count(wc, ST_STAT_ESCAPED);
@@ -497,6 +516,7 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
     d = rd;
     tmin = kTminBounce;
     depth = nd;
+    from = (int)elem;
     return true;
 }
 
@@ -538,9 +558,14 @@ __device__ __forceinline__ uint32_t queue_push(float* q, uint32_t qn, bool cont,
     return qn + (uint32_t)__popcll(m);
 }
 
-// tag of a parked ray: offset in the chunk (< 2^16) | depth << 16 | last << 20
-__device__ __forceinline__ uint32_t queue_tag(uint64_t offset, int depth, int last) {
-    return (uint32_t)offset | ((uint32_t)depth << 16) | ((uint32_t)last << 20);
+// tag of a parked ray: offset in the chunk (< 2^11) | depth << 11 | (from + 1)
+// << 15, from = element it reflected off (0: none, or an index >= 2^17 - 1,
+// whose continuation is then traced through the BVH). A parked ray's last
+// record slot is its depth.
+constexpr uint32_t kMaxChunkRays = 2048;
+__device__ __forceinline__ uint32_t queue_tag(uint64_t offset, int depth, int from) {
+    const uint32_t f = (uint32_t)(from + 1) < (1u << 17) ? (uint32_t)(from + 1) : 0u;
+    return (uint32_t)offset | ((uint32_t)depth << 11) | (f << 15);
 }
 
 template <int BLOCK, int STACK, int MINW, bool QUEUED>
@@ -585,37 +610,38 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                 bool cont = false;
                 V3 o = v3(0.f, 0.f, 0.f), d = o;
                 uint32_t off = 0;
-                int depth = 0, last = 0;
+                int depth = 0, from = -1;
                 if ((uint32_t)lane < take) {
                     const uint32_t i = qn + (uint32_t)lane;
                     o = v3(q[0 * kQueueCap + i], q[1 * kQueueCap + i], q[2 * kQueueCap + i]);
                     d = v3(q[3 * kQueueCap + i], q[4 * kQueueCap + i], q[5 * kQueueCap + i]);
                     const uint32_t tag = __float_as_uint(q[6 * kQueueCap + i]);
-                    off = tag & 0xFFFFu;
-                    depth = (int)((tag >> 16) & 15u);
-                    last = (int)((tag >> 20) & 15u);
+                    off = tag & 0x7FFu;
+                    depth = (int)((tag >> 11) & 15u);
+                    from = (int)(tag >> 15) - 1;
+                    int last = depth;
                     float tmin = kTminBounce;
                     const uint64_t ray = base_ray + off;
-                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, -1);
+                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, bounce_list(J, depth, from),
+                                                from);
                     if (!cont) finish_ray(J, ray, last);
                 }
-                qn = queue_push(q, qn, cont, o, d, queue_tag(off, depth, last));
+                qn = queue_push(q, qn, cont, o, d, queue_tag(off, depth, from));
             };
             for (uint64_t b = first; b < end; b += 64) {
                 const uint64_t i = b + (uint64_t)lane;
                 bool cont = false;
                 V3 o = v3(0.f, 0.f, 0.f), d = o;
-                int depth = 0, last = 0;
+                int depth = 0, last = 0, from = -1;
                 if (i < end) {
                     const uint64_t ray = J.ray_begin + i;
                     uint32_t cell;
                     sun_ray(J, S, ray, wc, o, d, cell);
                     float tmin = kTminPrimary;
-                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last,
-                                                J.grid_start ? (int)cell : -1);
+                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, cell_list(J, cell), from);
                     if (!cont) finish_ray(J, ray, last);
                 }
-                qn = queue_push(q, qn, cont, o, d, queue_tag(i - first, depth, last));
+                qn = queue_push(q, qn, cont, o, d, queue_tag(i - first, depth, from));
                 while (qn >= 64) trace_parked(64);
             }
             while (qn > 0) trace_parked(min(qn, 64u));
@@ -626,9 +652,10 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                 uint32_t cell;
                 sun_ray(J, S, ray, wc, o, d, cell);
                 float tmin = kTminPrimary;
-                int depth = 0, last = 0;
-                if (trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, J.grid_start ? (int)cell : -1))
-                    while (trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, -1)) {
+                int depth = 0, last = 0, from = -1;
+                if (trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, cell_list(J, cell), from))
+                    while (trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, bounce_list(J, depth, from),
+                                                from)) {
                     }
                 finish_ray(J, ray, last);
             }
@@ -969,6 +996,117 @@ __global__ void k_grid_fill(const float* __restrict__ bounds, int32_t n, const S
         }
 }
 
+// ---------------------------------------------------------------------------
+// Candidate lists of first reflections. A sun ray reflected off mirror e
+// leaves from e's box B_e in a direction within phi of a = reflect(-sun, n_e):
+// phi = sun-cone angle + 2 * (slope-error bound + surface-normal spread) +
+// specular-error bound, each error bounded by sigma * sqrt(-2 ln 2^-24) =
+// 5.7686 sigma (the smallest uniform). Such a ray can reach a box B only if the
+// cone {t u : t >= 0, angle(u, a) <= phi} meets B - B_e (Minkowski
+// difference); the cone is covered by the boxes of 32 pieces [t_i, t_i+1]
+// (geometric in t) and tested against the BVH, so the list holds every
+// primitive the ray can hit. Elements whose cone is wide (cylinders, large
+// errors) or whose list exceeds kMaxBounceList get none.
+__device__ bool bounce_cone(const st_element_f64& E, double theta_sun, double* phi) {
+    if (E.role != ST_ROLE_MIRROR || E.surface == ST_SURFACE_CYLINDER) return false;
+    double spread = 0.0;
+    if (E.surface == ST_SURFACE_PARABOLIC) {
+        const double sx = E.curv_x * 0.5 * E.dim_x, sy = E.curv_y * 0.5 * E.dim_y;
+        spread = atan(sqrt(sx * sx + sy * sy));
+    }
+    const double p = (theta_sun + 2.0 * (5.7686 * E.slope_error + spread) + 5.7686 * E.spec_error) * 1.001 + 1e-5;
+    *phi = p;
+    return p < 0.35;
+}
+
+__device__ bool cone_meets_box(const double a[3], double tan_phi, const double lo[3], const double hi[3]) {
+    double far2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const double m = fmax(fabs(lo[k]), fabs(hi[k]));
+        far2 += m * m;
+    }
+    const double T = sqrt(far2);
+    constexpr int kPieces = 32;
+    const double first = 0.25;
+    const double ratio = T > first ? pow(T / first, 1.0 / (kPieces - 1)) : 1.0;
+    double t0 = 0.0, t1 = fmin(T, first);
+    for (int i = 0; i < kPieces; ++i) {
+        const double r0 = t0 * tan_phi, r1 = t1 * tan_phi;
+        bool meet = true;
+        for (int k = 0; k < 3 && meet; ++k) {
+            const double c0 = a[k] * t0, c1 = a[k] * t1;
+            const double slo = fmin(c0 - r0, c1 - r1), shi = fmax(c0 + r0, c1 + r1);
+            meet = !(shi < lo[k] || slo > hi[k]);
+        }
+        if (meet) return true;
+        if (t1 >= T) break;
+        t0 = t1;
+        t1 = i == kPieces - 2 ? T : fmin(T, t1 * ratio);
+    }
+    return false;
+}
+
+__global__ void k_bounce_lists(const st_element_f64* __restrict__ elems, int32_t n, const int32_t* __restrict__ elem_slot,
+                               const double* __restrict__ frames, const float* __restrict__ bounds_slot,
+                               const float4* __restrict__ nodes, const SunFrame* __restrict__ sun, double theta_sun,
+                               uint32_t* __restrict__ counts, const uint2* __restrict__ range,
+                               int32_t* __restrict__ items) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const st_element_f64 E = elems[e];
+    double phi = 0.0;
+    if (!bounce_cone(E, theta_sun, &phi)) {
+        if (counts) counts[e] = kNoBounceList;
+        return;
+    }
+    uint32_t out = 0;
+    if (range) {
+        if (range[e].x > range[e].y) return;
+        out = range[e].x;
+    }
+    const double nz[3] = {frames[9 * (size_t)e + 2], frames[9 * (size_t)e + 5], frames[9 * (size_t)e + 8]};
+    const double din[3] = {-sun->sun[0], -sun->sun[1], -sun->sun[2]};
+    const double dn = din[0] * nz[0] + din[1] * nz[1] + din[2] * nz[2];
+    double a[3] = {din[0] - 2.0 * dn * nz[0], din[1] - 2.0 * dn * nz[1], din[2] - 2.0 * dn * nz[2]};
+    const double la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    for (int k = 0; k < 3; ++k) a[k] /= la;
+    const double tp = tan(phi);
+    const float* hb = bounds_slot + 6 * (size_t)elem_slot[e];
+    auto meets = [&](const float lo[3], const float hi[3]) {
+        const double dlo[3] = {(double)lo[0] - hb[3], (double)lo[1] - hb[4], (double)lo[2] - hb[5]};
+        const double dhi[3] = {(double)hi[0] - hb[0], (double)hi[1] - hb[1], (double)hi[2] - hb[2]};
+        return cone_meets_box(a, tp, dlo, dhi);
+    };
+    uint32_t count = 0;
+    int32_t stack[kStackDepth + 1];  // depth <= 31 levels, one pending sibling each
+    int sp = 0;
+    int32_t node = 0;
+    for (;;) {
+        const float4 n0 = nodes[4 * node + 0], n1 = nodes[4 * node + 1], n2 = nodes[4 * node + 2], n3 = nodes[4 * node + 3];
+        const float lo0[3] = {n0.x, n0.z, n2.x}, hi0[3] = {n0.y, n0.w, n2.y};
+        const float lo1[3] = {n1.x, n1.z, n2.z}, hi1[3] = {n1.y, n1.w, n2.w};
+        const int32_t ch[2] = {__float_as_int(n3.x), __float_as_int(n3.y)};
+        const bool hit[2] = {meets(lo0, hi0), meets(lo1, hi1)};
+        for (int c = 0; c < 2; ++c) {
+            if (!hit[c] || (c == 1 && ch[1] == ch[0])) continue;  // a lone primitive sits under both children
+            if (ch[c] >= 0) {
+                stack[sp++] = ch[c];
+            } else {
+                const int32_t st = leaf_start(ch[c]), cnt = leaf_count(ch[c]);
+                for (int i = 0; i < cnt; ++i) {
+                    const float* b = bounds_slot + 6 * (size_t)(st + i);
+                    if (!meets(b, b + 3)) continue;
+                    if (range) items[out + count] = st + i;
+                    ++count;
+                }
+            }
+        }
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+    if (counts) counts[e] = count > kMaxBounceList ? kNoBounceList : count;
+}
+
 __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t value) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = value;
@@ -1064,7 +1202,7 @@ hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk
     if ((uint64_t)grid > max_blocks) grid = (int)(max_blocks > 0 ? max_blocks : 1);
     const int st = shape.stack, w = shape.waves, b = shape.block, c = shape.chunk;
     const bool r = shape.queued;
-    if (r && (c <= 0 || c > 65535)) return hipErrorInvalidValue;  // parked rays keep a 16-bit chunk offset
+    if (r && (c <= 0 || c > (int)kMaxChunkRays)) return hipErrorInvalidValue;  // parked rays keep an 11-bit offset
 #define ST_VARIANT(B, S, W, Q)                                                                             \
     if (b == B && st == S && w == W && r == Q) {                                                          \
         launch_variant<B, S, W, Q>(s, grid, jobs, chunk_prefix, n_jobs, total_chunks, work_counter, c, single); \
@@ -1104,6 +1242,15 @@ hipError_t launch_grid_fill(hipStream_t s, const float* bounds, int32_t n, const
                             const uint32_t* start, uint32_t* cursor, int32_t* items) {
     hipLaunchKernelGGL(k_grid_fill, dim3((n + 127) / 128), dim3(128), 0, s, bounds, n, sun, tan_max, start, cursor,
                        items);
+    return hipGetLastError();
+}
+
+hipError_t launch_bounce_lists(hipStream_t s, const st_element_f64* elems, int32_t n, const int32_t* elem_slot,
+                               const double* frames, const float* bounds_slot, const float4* nodes,
+                               const SunFrame* sun, double theta_sun, uint32_t* counts, const uint2* range,
+                               int32_t* items) {
+    hipLaunchKernelGGL(k_bounce_lists, dim3((n + 63) / 64), dim3(64), 0, s, elems, n, elem_slot, frames, bounds_slot,
+                       nodes, sun, theta_sun, counts, range, items);
     return hipGetLastError();
 }
 
