@@ -21,22 +21,30 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "traced rays/sec at 10k-heliostat field + p50/p99 job latency, 1 MI355X"
-# FP32 VALU issue rate of MI355X: 256 CU x 64 lanes x 2.4 GHz (FMA counted
-# once, no packed FP32) = published 157.3 TFLOP/s / (2 FMA x 2 packed)
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
-# lane operations per unit of work (counting rules: add/mul/fma/min/max/cmp = 1,
-# sqrt/rcp = 4, sincos = 20, atan2 = 30, log = 10)
-OPS_PER_RAY_FIXED = 160      # Philox draw (~100 integer ops) + sun sample (~60)
-OPS_PER_SEGMENT = 520        # BVH traversal of the 10k field (~28 box tests x 12 + stack) + leaf tests
-OPS_PER_MIRROR_HIT = 178     # normal (15), reflectivity draw (~103: Philox + compare), slope error (~60)
-OPS_PER_RECEIVER_HIT = 140   # cylinder re-intersection (~90) + cylindrical bin (~50)
+# VALU issue peak of MI355X: 256 CU x 128 lane-ops per clock x 2.4 GHz = the
+# published FP32 vector peak 157.3 TFLOP/s with an FMA counted as one op (a
+# plain 32-bit add/xor/fma chain issues at ~71e12 lane-ops/s on the card)
+VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
+# Algorithmic lane operations per event (add/mul/fma/min/max/cmp/select/rcp/
+# rsqrt/sqrt = 1, a 32x32->64 multiply = 1, sincos/sincospi = 20, atan2 = 30),
+# counted from the source of each routine:
+OPS_SUN_RAY = 125       # Philox4x32-10 (60), 4 uniforms (12), plane point (6), cone direction (45)
+OPS_SEGMENT = 10        # hit point and local offset of a closest-hit query
+OPS_NODE_VISIT = 34     # two slab tests (12 fma + 18 min/max/cmp) + order / stack
+OPS_FLAT_TEST = 30      # plane t, in-rectangle test, closest-hit update
+OPS_PARABOLIC_TEST = 60 # quadratic in the element frame, root choice, aperture test
+OPS_CYLINDER_TEST = 70  # wall quadratic about closest approach + two caps
+OPS_MIRROR_HIT = 172    # Philox (60), faceforward, slope-error cone (81), reflection (14), counts
+OPS_RECEIVER_HIT = 70   # local coordinates, atan2 bin, aggregated flux add
 HBM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_field10k_hbm_traffic.txt")
 
 
 def ops_per_ray(stats) -> float:
-    rays = float(stats[0])
-    return (OPS_PER_RAY_FIXED + OPS_PER_SEGMENT * stats[1] / rays + OPS_PER_MIRROR_HIT * stats[2] / rays
-            + OPS_PER_RECEIVER_HIT * stats[3] / rays)
+    """Algorithmic lane operations per ray from a ST_TRACE_COUNT_WORK launch."""
+    s = [float(x) / float(stats[0]) for x in stats]
+    return (OPS_SUN_RAY + OPS_SEGMENT * s[1] + OPS_NODE_VISIT * s[11] + OPS_FLAT_TEST * s[12]
+            + OPS_PARABOLIC_TEST * s[13] + OPS_CYLINDER_TEST * s[14] + OPS_MIRROR_HIT * s[2]
+            + OPS_RECEIVER_HIT * s[3])
 
 
 def read_traffic():
@@ -78,6 +86,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--sampling", choices=["stratified", "philox", "halton"], default="stratified",
                     help="sun-plane position sampling (stratified: 64-ray wave-coherent cells)")
+    ap.add_argument("--kernel", choices=["auto", "mega", "queued"], default="auto",
+                    help="trace loop form (auto: the tuned one)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -114,8 +124,10 @@ def main():
     if not args.no_autotune and args.block == 0 and args.grid == 0:
         tuning = sc.autotune(sample_rays=rays, max_depth=w.max_depth)   # tune at the measured size
     stream = torch.cuda.Stream(device=0)
+    kernel = {"auto": _abi.KERNEL_AUTO, "mega": _abi.KERNEL_MEGA, "queued": _abi.KERNEL_QUEUED}[args.kernel]
     desc = sc.make_desc(rays, seed=args.seed, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny,
-                        zero=False, block_size=args.block, grid_blocks=args.grid, stream=stream.cuda_stream)
+                        zero=False, block_size=args.block, grid_blocks=args.grid, stream=stream.cuda_stream,
+                        kernel=kernel)
     res = sc.alloc_outputs(desc)
     torch.cuda.synchronize()
     for _ in range(args.warmup):
@@ -138,7 +150,16 @@ def main():
     assert stats[0] == calls * rays, "every launched ray must be counted once"
     per_call = stats / calls
     value = args.steps * rays / wall
-    w_ops = ops_per_ray(per_call)
+    # one more launch of the same work, untimed, counting node visits and tests
+    wdesc = sc.make_desc(rays, seed=args.seed, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny,
+                         zero=True, block_size=args.block, grid_blocks=args.grid, stream=stream.cuda_stream,
+                         kernel=kernel, count_work=True)
+    wres = sc.alloc_outputs(wdesc)
+    sc.trace_into(wdesc, wres)
+    torch.cuda.synchronize()
+    work = wres.stats.cpu().numpy().astype(np.int64)
+    assert np.array_equal(work[:11], stats[:11] // calls), "work-count launch must trace the same rays"
+    w_ops = ops_per_ray(work)
     achieved = w_ops * rays / (kernel_ms * 1e-3) / 1e12
     line = {
         "metric": METRIC,
@@ -159,7 +180,11 @@ def main():
                    "seed": args.seed, "sampling": args.sampling, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s",
                      "frac": achieved / VALU_PEAK_TOPS, "traffic": read_traffic(),
-                     "ops_per_ray": w_ops, "kernel_ms": kernel_ms},
+                     "ops_per_ray": w_ops, "kernel_ms": kernel_ms,
+                     "work_per_ray": {"segments": float(work[1]) / rays, "node_visits": float(work[11]) / rays,
+                                      "flat_tests": float(work[12]) / rays,
+                                      "parabolic_tests": float(work[13]) / rays,
+                                      "cylinder_tests": float(work[14]) / rays}},
         "tuning": tuning,
         "segments_per_s": float(per_call[1]) * args.steps / wall,
         "receiver_hits_per_ray": float(per_call[3] / rays),

@@ -84,8 +84,10 @@ enum st_position_sampling {
                                           (reference CSV keeps them; SolTrace drops them,
                                           scripts/post_processing/clean_direct_sun.py:6-16) */
 #define ST_TRACE_ZERO_OUTPUTS 0x4u     /* zero flux / element_hits / stats before tracing */
-#define ST_TRACE_BVH_ONLY 0x8u         /* find sun-ray hits by BVH traversal instead of the
-                                          sun-plane candidate lists (identical results) */
+#define ST_TRACE_BVH_ONLY 0x8u         /* find every hit by BVH traversal instead of the
+                                          candidate lists (identical results) */
+#define ST_TRACE_COUNT_WORK 0x10u      /* also count node visits and primitive tests
+                                          (stats 11-14; slows the trace down) */
 #define ST_JOB_KEEP_AIM 0x100u         /* server job: trace the scene's stored aims instead of
                                           re-aiming every mirror at the job's sun (bisector,
                                           target = centre of the scene's last receiver) */
@@ -109,6 +111,11 @@ enum st_stat {
     ST_STAT_TRUNCATED = 8,     /* hit something at the depth cap (not recorded)      */
     ST_STAT_CAP_HITS = 9,      /* receiver hits on a cylinder cap                    */
     ST_STAT_FLUX_BINNED = 10,  /* hits added to the flux map                         */
+    /* work counters, filled only with ST_TRACE_COUNT_WORK                             */
+    ST_STAT_NODE_VISITS = 11,  /* BVH inner nodes visited (two child-box tests each)  */
+    ST_STAT_FLAT_TESTS = 12,   /* flat-primitive intersection tests                  */
+    ST_STAT_PARABOLIC_TESTS = 13, /* parabolic-primitive intersection tests          */
+    ST_STAT_CYLINDER_TESTS = 14,  /* cylinder intersection tests                      */
     ST_STAT_COUNT = 16
 };
 

@@ -34,12 +34,14 @@ TRACE_RECORDS = 0x1
 TRACE_FLUX_DIRECT_SUN = 0x2
 TRACE_ZERO_OUTPUTS = 0x4
 TRACE_BVH_ONLY = 0x8
+TRACE_COUNT_WORK = 0x10
 
 KERNEL_AUTO, KERNEL_MEGA, KERNEL_QUEUED = 0, 1, 2
 
 STAT_NAMES = [
     "rays", "segments", "mirror_hits", "receiver_hits", "direct_sun", "absorbed",
     "escaped", "back_face", "truncated", "cap_hits", "flux_binned",
+    "node_visits", "flat_tests", "parabolic_tests", "cylinder_tests",
 ]
 STAT_COUNT = 16
 

@@ -292,47 +292,74 @@ static double sun_cone_angle(const st_sun_desc& sun) {
 }
 static double sun_cone_tan(const st_sun_desc& sun) { return tan(sun_cone_angle(sun)); }
 
-int scene_bounce_lists(st_scene s) {
+// count pass -> host prefix sum (lists over kMaxBounceList or marked none get
+// an empty-but-invalid range) -> fill pass. Returns the number of items.
+template <class Launch>
+static int build_cone_lists(st_scene s, size_t tasks, DevBuf& counts, DevBuf& range, DevBuf& items,
+                            std::vector<uint2>& h_range, uint64_t* total_out, Launch launch) {
     hipStream_t st = s->ctx->stream;
-    s->bounce_valid = false;
-    const size_t n = (size_t)s->n;
-    ST_TRY(s->d_bounce_counts.ensure(sizeof(uint32_t) * n, "bounce counts"));
-    ST_TRY(s->d_bounce_range.ensure(sizeof(uint2) * n, "bounce ranges"));
-    const double theta = sun_cone_angle(s->sun);
-    HIP_TRY(launch_bounce_lists(st, (const st_element_f64*)s->d_elems.ptr, s->n, (const int32_t*)s->d_elem_slot.ptr,
-                                (const double*)s->d_frames.ptr, (const float*)s->d_bounds.ptr,
-                                (const float4*)s->d_nodes.ptr, (const SunFrame*)s->d_sun.ptr, theta,
-                                (uint32_t*)s->d_bounce_counts.ptr, nullptr, nullptr),
-            "bounce count kernel");
-    s->h_bounce_counts.resize(n);
-    s->h_bounce_range.resize(n);
-    HIP_TRY(hipMemcpyAsync(s->h_bounce_counts.data(), s->d_bounce_counts.ptr, sizeof(uint32_t) * n,
-                           hipMemcpyDeviceToHost, st),
-            "bounce counts download");
-    HIP_TRY(hipStreamSynchronize(st), "bounce sync");
+    *total_out = 0;
+    if (tasks == 0) return ST_OK;
+    ST_TRY(counts.ensure(sizeof(uint32_t) * tasks, "list counts"));
+    ST_TRY(range.ensure(sizeof(uint2) * tasks, "list ranges"));
+    HIP_TRY(launch((uint32_t*)counts.ptr, nullptr, nullptr), "list count kernel");
+    s->h_counts.resize(tasks);
+    h_range.resize(tasks);
+    HIP_TRY(hipMemcpyAsync(s->h_counts.data(), counts.ptr, sizeof(uint32_t) * tasks, hipMemcpyDeviceToHost, st),
+            "list counts download");
+    HIP_TRY(hipStreamSynchronize(st), "list sync");
     uint64_t total = 0;
-    for (size_t e = 0; e < n; ++e) {
-        const uint32_t c = s->h_bounce_counts[e];
+    for (size_t t = 0; t < tasks; ++t) {
+        const uint32_t c = s->h_counts[t];
         if (c == kNoBounceList || total + c > 0x7FFFFFFFull) {
-            s->h_bounce_range[e] = make_uint2(1u, 0u);
+            h_range[t] = make_uint2(1u, 0u);
         } else {
-            s->h_bounce_range[e] = make_uint2((uint32_t)total, (uint32_t)(total + c));
+            h_range[t] = make_uint2((uint32_t)total, (uint32_t)(total + c));
             total += c;
         }
     }
-    s->bounce_items = total;
+    *total_out = total;
     if (total == 0) return ST_OK;
-    ST_TRY(s->d_bounce_items.ensure(sizeof(int32_t) * total, "bounce items"));
-    HIP_TRY(hipMemcpyAsync(s->d_bounce_range.ptr, s->h_bounce_range.data(), sizeof(uint2) * n, hipMemcpyHostToDevice,
-                           st),
-            "bounce ranges upload");
-    HIP_TRY(launch_bounce_lists(st, (const st_element_f64*)s->d_elems.ptr, s->n, (const int32_t*)s->d_elem_slot.ptr,
-                                (const double*)s->d_frames.ptr, (const float*)s->d_bounds.ptr,
-                                (const float4*)s->d_nodes.ptr, (const SunFrame*)s->d_sun.ptr, theta, nullptr,
-                                (const uint2*)s->d_bounce_range.ptr, (int32_t*)s->d_bounce_items.ptr),
-            "bounce fill kernel");
-    HIP_TRY(hipStreamSynchronize(st), "bounce sync");  // the host range array is reused by the next rebuild
+    ST_TRY(items.ensure(sizeof(int32_t) * total, "list items"));
+    HIP_TRY(hipMemcpyAsync(range.ptr, h_range.data(), sizeof(uint2) * tasks, hipMemcpyHostToDevice, st),
+            "list ranges upload");
+    HIP_TRY(launch(nullptr, (const uint2*)range.ptr, (int32_t*)items.ptr), "list fill kernel");
+    HIP_TRY(hipStreamSynchronize(st), "list sync");  // h_range is reused by the next rebuild
+    return ST_OK;
+}
+
+int scene_bounce_lists(st_scene s) {
+    hipStream_t st = s->ctx->stream;
+    s->bounce_valid = s->pair_valid = false;
+    const double theta = sun_cone_angle(s->sun);
+    auto common = [&](int32_t tasks, const int32_t* pitems, const int32_t* powner, uint32_t* c, const uint2* r,
+                      int32_t* it) {
+        return launch_cone_lists(st, (const st_element_f64*)s->d_elems.ptr, tasks, (const int32_t*)s->d_elem_slot.ptr,
+                                 (const uint32_t*)s->d_meta.ptr, (const double*)s->d_frames.ptr,
+                                 (const float*)s->d_bounds.ptr, (const float4*)s->d_nodes.ptr,
+                                 (const SunFrame*)s->d_sun.ptr, theta, pitems, powner, c, r, it);
+    };
+    // first reflections: one list per element
+    ST_TRY(build_cone_lists(s, (size_t)s->n, s->d_bounce_counts, s->d_bounce_range, s->d_bounce_items,
+                            s->h_bounce_range, &s->bounce_items,
+                            [&](uint32_t* c, const uint2* r, int32_t* it) { return common(s->n, nullptr, nullptr, c, r, it); }));
+    if (s->bounce_items == 0) return ST_OK;
     s->bounce_valid = true;
+    // second reflections: one list per first-reflection list entry
+    s->h_pair_owner.assign(s->bounce_items, 0);
+    for (int32_t e = 0; e < s->n; ++e)
+        for (uint32_t j = s->h_bounce_range[e].x; j < s->h_bounce_range[e].y; ++j) s->h_pair_owner[j] = e;
+    ST_TRY(s->d_pair_owner.ensure(sizeof(int32_t) * s->bounce_items, "pair owners"));
+    HIP_TRY(hipMemcpyAsync(s->d_pair_owner.ptr, s->h_pair_owner.data(), sizeof(int32_t) * s->bounce_items,
+                           hipMemcpyHostToDevice, st),
+            "pair owners upload");
+    const int32_t tasks = (int32_t)std::min<uint64_t>(s->bounce_items, 0x7FFFFFFF);
+    ST_TRY(build_cone_lists(s, (size_t)tasks, s->d_pair_counts, s->d_pair_range, s->d_pair_items, s->h_pair_range,
+                            &s->pair_items, [&](uint32_t* c, const uint2* r, int32_t* it) {
+                                return common(tasks, (const int32_t*)s->d_bounce_items.ptr,
+                                              (const int32_t*)s->d_pair_owner.ptr, c, r, it);
+                            }));
+    s->pair_valid = s->pair_items > 0;
     return ST_OK;
 }
 
@@ -597,6 +624,10 @@ int fill_job(st_scene s, const st_trace_desc* d, const st_outputs* out, const vo
     if (s->bounce_valid && prims == s->d_prims.ptr && sun == s->d_sun.ptr && !(d->flags & ST_TRACE_BVH_ONLY)) {
         J->bounce_range = (const uint2*)s->d_bounce_range.ptr;
         J->bounce_items = (const int32_t*)s->d_bounce_items.ptr;
+        if (s->pair_valid) {
+            J->pair_range = (const uint2*)s->d_pair_range.ptr;
+            J->pair_items = (const int32_t*)s->d_pair_items.ptr;
+        }
     }
     J->ray_begin = d->ray_begin;
     J->ray_count = d->ray_count ? d->ray_count : d->num_rays - d->ray_begin;

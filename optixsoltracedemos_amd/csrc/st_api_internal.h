@@ -99,12 +99,15 @@ struct st_scene_s {
     st::DevBuf d_grid_counts, d_grid_start, d_grid_cursor, d_grid_items, d_grid_stats;
     bool grid_valid = false;
     uint32_t grid_items = 0, grid_longest = 0;
-    // candidate lists of first reflections, per element (rebuilt with the sun plane)
-    st::DevBuf d_bounce_counts, d_bounce_range, d_bounce_items;
-    std::vector<uint32_t> h_bounce_counts;
-    std::vector<uint2> h_bounce_range;
-    bool bounce_valid = false;
-    uint64_t bounce_items = 0;
+    // candidate lists of first reflections (per element) and second
+    // reflections (per first-reflection list entry), rebuilt with the sun plane
+    st::DevBuf d_bounce_counts, d_bounce_range, d_bounce_items, d_pair_counts, d_pair_range, d_pair_items,
+        d_pair_owner;
+    std::vector<uint32_t> h_counts;
+    std::vector<uint2> h_bounce_range, h_pair_range;
+    std::vector<int32_t> h_pair_owner;
+    bool bounce_valid = false, pair_valid = false;
+    uint64_t bounce_items = 0, pair_items = 0;
     bool sun_set = false;
     st_sun_desc sun{};
     double sun_unit[3] = {0, 0, 1};

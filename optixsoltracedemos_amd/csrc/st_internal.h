@@ -38,6 +38,9 @@ struct JobDev {
     const uint2* bounce_range;   // [n_elements] (begin, end) list of a first reflection off the element;
                                  // begin > end: none (BVH). Null: BVH for every reflected ray
     const int32_t* bounce_items; // primitive slots
+    const uint2* pair_range;     // [bounce items] list of a second reflection whose first mirror's
+                                 // list entry was that item (begin > end: none). Null: BVH
+    const int32_t* pair_items;
 };
 
 constexpr int kGridRes = 256;  // sun-plane candidate grid, the stratified-sampling cells
@@ -86,14 +89,17 @@ hipError_t launch_grid_count(hipStream_t s, const float* bounds, int32_t n, cons
 // cursor zeroed (kGridRes^2 u32); items sized stats[0]
 hipError_t launch_grid_fill(hipStream_t s, const float* bounds, int32_t n, const SunFrame* sun, double tan_max,
                             const uint32_t* start, uint32_t* cursor, int32_t* items);
-// candidate lists of first reflections: count pass (range == null) writes
-// counts[e] (kNoBounceList: none); fill pass writes items[range[e].x ...]
+// candidate lists of reflected rays (k_cone_lists). Task t: first
+// reflections off element t (pair_items null), or second reflections of rays
+// that left element pair_owner[t] and hit primitive pair_items[t]. Count pass
+// (range null) writes counts[t] (kNoBounceList: none); fill pass writes
+// items[range[t].x ...].
 constexpr uint32_t kNoBounceList = 0xFFFFFFFFu;
 constexpr uint32_t kMaxBounceList = 64;
-hipError_t launch_bounce_lists(hipStream_t s, const st_element_f64* elems, int32_t n, const int32_t* elem_slot,
-                               const double* frames, const float* bounds_slot, const float4* nodes,
-                               const SunFrame* sun, double theta_sun, uint32_t* counts, const uint2* range,
-                               int32_t* items);
+hipError_t launch_cone_lists(hipStream_t s, const st_element_f64* elems, int32_t tasks, const int32_t* elem_slot,
+                             const uint32_t* meta, const double* frames, const float* bounds_slot,
+                             const float4* nodes, const SunFrame* sun, double theta_sun, const int32_t* pair_items,
+                             const int32_t* pair_owner, uint32_t* counts, const uint2* range, int32_t* items);
 hipError_t launch_fill_u64(hipStream_t s, void* p, uint64_t n_u64, uint64_t value);
 hipError_t launch_flux_watts(hipStream_t s, const int64_t* flux, int32_t nbins, const SunFrame* sun,
                              double num_rays, double bin_area, double* out, double* power_per_ray_dev);

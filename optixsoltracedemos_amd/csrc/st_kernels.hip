@@ -104,8 +104,9 @@ __device__ __forceinline__ float halton3(uint32_t i) {
 
 // cell of the stratified sun-plane sampling: 64 consecutive rays per cell of a
 // 256 x 256 grid, cells in a keyed 16-bit Feistel order within each sweep
-__device__ __forceinline__ uint32_t stratified_cell(uint64_t ray, uint32_t seed_lo, uint32_t seed_hi, uint32_t job) {
-    const uint64_t group = ray >> 6, sweep = group >> 16;
+__device__ __forceinline__ uint32_t stratified_cell_of_group(uint64_t group, uint32_t seed_lo, uint32_t seed_hi,
+                                                             uint32_t job) {
+    const uint64_t sweep = group >> 16;
     const U4 k = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), 0x53545254u, job}, seed_lo, seed_hi);
     const uint32_t keys[4] = {k.x, k.y, k.z, k.w};
     uint32_t L = ((uint32_t)group >> 8) & 0xFFu, R = (uint32_t)group & 0xFFu;
@@ -230,12 +231,28 @@ __device__ __forceinline__ bool isect_prim(const float4* __restrict__ prims, uin
 // Closest hit over the BVH: smallest t, equal t -> lower element index.
 __device__ __forceinline__ float next_up(float t) { return __int_as_float(__float_as_int(t) + 1); }
 
+// Event counters: ST_STAT_* 0..14, [15] = hits on the receiver kept aside
+// (agg_slot). Each wave owns kNumCounters u32 in LDS; every lane that reaches
+// a counting site contributes one and the wave adds the number of such lanes
+// with a single LDS atomic, so the counters cost no registers during traversal.
+constexpr int kNumCounters = 16;
+constexpr int kAggCounter = 15;
+
+__device__ __forceinline__ void count_if(uint32_t* wc, int k, bool pred) {
+    const unsigned long long m = __ballot(pred);
+    const unsigned long long live = __ballot(1);
+    if (m && (int)(threadIdx.x & 63) == __ffsll((long long)live) - 1) atomicAdd(wc + k, (uint32_t)__popcll(m));
+}
+__device__ __forceinline__ void count(uint32_t* wc, int k) { count_if(wc, k, true); }
+
 // One candidate of the closest-hit search. The result over a set of candidates
 // (smallest t, equal t -> lower element index; next_up keeps equal-t hits in
 // play) does not depend on the order the candidates are tested in.
 __device__ __forceinline__ void test_prim(const float4* __restrict__ prims, const uint32_t* __restrict__ meta,
-                                          int slot, V3 o, V3 d, float tmin, float& tbest, int& bslot, int& bcap) {
+                                          int slot, V3 o, V3 d, float tmin, float& tbest, int& bslot, int& bcap,
+                                          uint32_t* wc) {
     const uint32_t m = ld1(meta + slot);
+    if (wc) count(wc, ST_STAT_FLAT_TESTS + (int)meta_kind(m));
     float t;
     int cap;
     const float lim = bslot >= 0 ? next_up(tbest) : tbest;
@@ -249,17 +266,26 @@ __device__ __forceinline__ void test_prim(const float4* __restrict__ prims, cons
 }
 
 // Closest hit over a candidate list (items[b, e)): a sun ray's sun-plane cell
-// list or a first reflection's list of its mirror. Every primitive such a ray
-// can reach is listed (conservative binning, k_grid_* / k_bounce_lists), so
-// the result equals a full BVH traversal.
+// list, a first reflection's list of its mirror or a second reflection's list
+// of its (first, second) mirror pair. Every primitive such a ray can reach is
+// listed (conservative binning, k_grid_* / k_cone_lists), so the result equals
+// a full BVH traversal. bk = list index of the winner.
 __device__ __forceinline__ void closest_in_list(const JobDev& J, const int32_t* __restrict__ items, uint32_t b,
                                                 uint32_t e, V3 o, V3 d, float tmin, float& tbest, int& bslot,
-                                                int& bcap) {
-    for (uint32_t k = b; k < e; ++k)
-        test_prim(J.prims, J.meta, (int)ld1((const uint32_t*)items + k), o, d, tmin, tbest, bslot, bcap);
+                                                int& bcap, uint32_t& bk, uint32_t* wc) {
+    uint32_t* wk = (J.flags & ST_TRACE_COUNT_WORK) ? wc : nullptr;
+    for (uint32_t k = b; k < e; ++k) {
+        const int prev = bslot;
+        test_prim(J.prims, J.meta, (int)ld1((const uint32_t*)items + k), o, d, tmin, tbest, bslot, bcap, wk);
+        if (bslot != prev) bk = k;
+    }
 }
 
-// candidate list of a segment: lb > le means none (BVH traversal)
+// Candidate list of a segment (lb > le: none, BVH traversal) and the key that
+// locates the next segment's list: depth 0 (sun ray): sun-plane cell; depth 1:
+// the mirror it reflected off (key = element); depth 2: the pair (first mirror,
+// second mirror) (key = index of the second mirror in the first one's list);
+// deeper: none.
 struct SegList {
     const int32_t* items;
     uint32_t lb, le;
@@ -269,17 +295,23 @@ __device__ __forceinline__ SegList cell_list(const JobDev& J, uint32_t cell) {
     if (!J.grid_start) return no_list();
     return SegList{J.grid_items, ld1(J.grid_start + cell), ld1(J.grid_start + cell + 1)};
 }
-// first reflection off element `from` (sun-cone incidence); later ones: BVH
-__device__ __forceinline__ SegList bounce_list(const JobDev& J, int depth, int from) {
-    if (!J.bounce_range || depth != 1 || from < 0) return no_list();
-    const float2 r = ld2(J.bounce_range + from);
-    return SegList{J.bounce_items, __float_as_uint(r.x), __float_as_uint(r.y)};
+__device__ __forceinline__ SegList next_list(const JobDev& J, int depth, int key) {
+    if (key < 0) return no_list();
+    if (depth == 1 && J.bounce_range) {
+        const float2 r = ld2(J.bounce_range + key);
+        return SegList{J.bounce_items, __float_as_uint(r.x), __float_as_uint(r.y)};
+    }
+    if (depth == 2 && J.pair_range) {
+        const float2 r = ld2(J.pair_range + key);
+        return SegList{J.pair_items, __float_as_uint(r.x), __float_as_uint(r.y)};
+    }
+    return no_list();
 }
 
 template <int STRIDE>
 __device__ __forceinline__ void traverse(const float4* __restrict__ nodes, const float4* __restrict__ prims,
                                          const uint32_t* __restrict__ meta, V3 o, V3 d, float tmin, float& tbest,
-                                         int& bslot, int& bcap, int32_t* __restrict__ stk) {
+                                         int& bslot, int& bcap, int32_t* __restrict__ stk, uint32_t* wk) {
     const float ix = 1.f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x));
     const float iy = 1.f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y));
     const float iz = 1.f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z));
@@ -288,6 +320,7 @@ __device__ __forceinline__ void traverse(const float4* __restrict__ nodes, const
     int32_t node = 0;
     for (;;) {
         while (node >= 0) {
+            if (wk) count(wk, ST_STAT_NODE_VISITS);
             const float4 n0 = ld4(nodes + 4 * node + 0), n1 = ld4(nodes + 4 * node + 1),
                          n2 = ld4(nodes + 4 * node + 2);
             const float2 n3 = ld2(nodes + 4 * node + 3);
@@ -318,26 +351,13 @@ __device__ __forceinline__ void traverse(const float4* __restrict__ nodes, const
             }
         }
         const int32_t start = leaf_start(node), cnt = leaf_count(node);
-        for (int k = 0; k < cnt; ++k) test_prim(prims, meta, start + k, o, d, tmin, tbest, bslot, bcap);
+        for (int k = 0; k < cnt; ++k) test_prim(prims, meta, start + k, o, d, tmin, tbest, bslot, bcap, wk);
         if (sp == 0) return;
         node = stk[--sp * STRIDE];
     }
 }
 
 // ---------------------------------------------------------------------------
-// Event counters: ST_STAT_* 0..10, [11] = hits on the receiver kept aside
-// (agg_slot). Each wave owns kNumCounters u32 in LDS; every lane that reaches
-// a counting site contributes one and the wave adds the number of such lanes
-// with a single LDS atomic, so the counters cost no registers during traversal.
-constexpr int kNumCounters = 12;
-constexpr int kAggCounter = 11;
-
-__device__ __forceinline__ void count_if(uint32_t* wc, int k, bool pred) {
-    const unsigned long long m = __ballot(pred);
-    const unsigned long long live = __ballot(1);
-    if (m && (int)(threadIdx.x & 63) == __ffsll((long long)live) - 1) atomicAdd(wc + k, (uint32_t)__popcll(m));
-}
-__device__ __forceinline__ void count(uint32_t* wc, int k) { count_if(wc, k, true); }
 
 // base[idx] += 1 for every lane that calls it (lanes may diverge around the
 // call). One round of wave aggregation: the lanes sharing the first caller's
@@ -372,7 +392,14 @@ __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame
     const U4 u = philox4x32_10(U4{rlo, rhi, 0u, J.job_id}, J.seed_lo, J.seed_hi);
     float a, b;
     if (S.position_sampling == ST_POS_STRATIFIED) {
-        cell = stratified_cell(ray, J.seed_lo, J.seed_hi, J.job_id);
+        // the 64 rays of a group share a cell and a wave's rays normally form
+        // one group: its cell is computed once, from wave-uniform values (scalar
+        // unit); lanes of a second group (unaligned ray ranges) compute theirs
+        const uint64_t g = ray >> 6;
+        const uint64_t g0 = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(g >> 32)) << 32) |
+                            (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)g);
+        cell = stratified_cell_of_group(g0, J.seed_lo, J.seed_hi, J.job_id);
+        if (g != g0) cell = stratified_cell_of_group(g, J.seed_lo, J.seed_hi, J.job_id);
         a = ((float)(cell & 255u) + u01(u.x)) * (1.f / 256.f);
         b = ((float)(cell >> 8) + u01(u.y)) * (1.f / 256.f);
     } else {
@@ -410,13 +437,16 @@ __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame
 // the next segment), false when it ended.
 template <int STRIDE>
 __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int32_t* stk, uint32_t* wc, V3& o,
-                                              V3& d, float& tmin, int& depth, int& last, SegList L, int& from) {
+                                              V3& d, float& tmin, int& depth, int& last, SegList L, int& key) {
     const uint32_t rlo = (uint32_t)ray, rhi = (uint32_t)(ray >> 32);
     count(wc, ST_STAT_SEGMENTS);
     float t = kTmax;
     int slot = -1, cap = 0;
-    if (L.lb <= L.le) closest_in_list(J, L.items, L.lb, L.le, o, d, tmin, t, slot, cap);
-    else traverse<STRIDE>(J.nodes, J.prims, J.meta, o, d, tmin, t, slot, cap, stk);
+    uint32_t bk = 0xFFFFFFFFu;
+    const bool listed = L.lb <= L.le;
+    if (listed) closest_in_list(J, L.items, L.lb, L.le, o, d, tmin, t, slot, cap, bk, wc);
+    else traverse<STRIDE>(J.nodes, J.prims, J.meta, o, d, tmin, t, slot, cap, stk,
+                          (J.flags & ST_TRACE_COUNT_WORK) ? wc : nullptr);
     if (slot < 0) {  // __miss__ms, cuda/materials.cu:236-253
         count(wc, ST_STAT_ESCAPED);
         return false;
@@ -479,13 +509,16 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
     last = nd;
     if (J.hits) add_one_aggregated((unsigned long long*)J.hits, elem);
     count(wc, ST_STAT_MIRROR_HITS);
+    // draws of bounce nd: block 2nd-1 = (reflectivity, slope xr, slope xp,
+    // specular xr), block 2nd = (specular xp, -, -, -), each drawn only when
+    // one of its optics terms is active
     const float4 op = ld4(J.optics + elem);
-    if (op.x < 1.f) {
-        const U4 ur = philox4x32_10(U4{rlo, rhi, 2u * (uint32_t)nd, J.job_id}, J.seed_lo, J.seed_hi);
-        if (u01(ur.x) >= op.x) {
-            count(wc, ST_STAT_ABSORBED);
-            return false;
-        }
+    U4 ue = U4{0u, 0u, 0u, 0u};
+    if (op.x < 1.f || op.y > 0.f || op.z > 0.f)
+        ue = philox4x32_10(U4{rlo, rhi, 2u * (uint32_t)nd - 1u, J.job_id}, J.seed_lo, J.seed_hi);
+    if (op.x < 1.f && u01(ue.x) >= op.x) {
+        count(wc, ST_STAT_ABSORBED);
+        return false;
     }
     V3 n;
     if (kind == kKindFlat) {
@@ -503,20 +536,19 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
         n = normalize(fma3(nl.x, ax, fma3(nl.y, ay, nl.z * az)));
     }
     if (dot(d, n) > 0.f) n = -n;  // faceforward(n, -d, n)
-    V3 rd;
-    if (op.y > 0.f || op.z > 0.f) {
-        const U4 ue = philox4x32_10(U4{rlo, rhi, 2u * (uint32_t)nd - 1u, J.job_id}, J.seed_lo, J.seed_hi);
-        if (op.y > 0.f) n = perturb(n, op.y, u01(ue.x), u01(ue.y));
-        rd = normalize(fma3(-2.f * dot(n, d), n, d));
-        if (op.z > 0.f) rd = perturb(rd, op.z, u01(ue.z), u01(ue.w));
-    } else {
-        rd = normalize(fma3(-2.f * dot(n, d), n, d));
+    if (op.y > 0.f) n = perturb(n, op.y, u01(ue.y), u01(ue.z));
+    V3 rd = normalize(fma3(-2.f * dot(n, d), n, d));
+    if (op.z > 0.f) {
+        const U4 us = philox4x32_10(U4{rlo, rhi, 2u * (uint32_t)nd, J.job_id}, J.seed_lo, J.seed_hi);
+        rd = perturb(rd, op.z, u01(ue.w), u01(us.x));
     }
     o = p;
     d = rd;
     tmin = kTminBounce;
+    // key of the next segment's list: a first reflection's list depends on its
+    // mirror only; a second reflection's on where it sits in that list
+    key = nd == 1 ? (int)elem : (nd == 2 && listed) ? (int)bk : -1;
     depth = nd;
-    from = (int)elem;
     return true;
 }
 
@@ -558,13 +590,13 @@ __device__ __forceinline__ uint32_t queue_push(float* q, uint32_t qn, bool cont,
     return qn + (uint32_t)__popcll(m);
 }
 
-// tag of a parked ray: offset in the chunk (< 2^11) | depth << 11 | (from + 1)
-// << 15, from = element it reflected off (0: none, or an index >= 2^17 - 1,
-// whose continuation is then traced through the BVH). A parked ray's last
-// record slot is its depth.
+// tag of a parked ray: offset in the chunk (< 2^11) | depth << 11 | (key + 1)
+// << 15, key = the list key of its next segment (0: none, or a key >= 2^17 - 1,
+// whose segment is then traced through the BVH). A parked ray's last record
+// slot is its depth.
 constexpr uint32_t kMaxChunkRays = 2048;
-__device__ __forceinline__ uint32_t queue_tag(uint64_t offset, int depth, int from) {
-    const uint32_t f = (uint32_t)(from + 1) < (1u << 17) ? (uint32_t)(from + 1) : 0u;
+__device__ __forceinline__ uint32_t queue_tag(uint64_t offset, int depth, int key) {
+    const uint32_t f = (uint32_t)(key + 1) < (1u << 17) ? (uint32_t)(key + 1) : 0u;
     return (uint32_t)offset | ((uint32_t)depth << 11) | (f << 15);
 }
 
@@ -610,7 +642,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                 bool cont = false;
                 V3 o = v3(0.f, 0.f, 0.f), d = o;
                 uint32_t off = 0;
-                int depth = 0, from = -1;
+                int depth = 0, key = -1;
                 if ((uint32_t)lane < take) {
                     const uint32_t i = qn + (uint32_t)lane;
                     o = v3(q[0 * kQueueCap + i], q[1 * kQueueCap + i], q[2 * kQueueCap + i]);
@@ -618,30 +650,30 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                     const uint32_t tag = __float_as_uint(q[6 * kQueueCap + i]);
                     off = tag & 0x7FFu;
                     depth = (int)((tag >> 11) & 15u);
-                    from = (int)(tag >> 15) - 1;
+                    key = (int)(tag >> 15) - 1;
                     int last = depth;
                     float tmin = kTminBounce;
                     const uint64_t ray = base_ray + off;
-                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, bounce_list(J, depth, from),
-                                                from);
+                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, next_list(J, depth, key),
+                                                key);
                     if (!cont) finish_ray(J, ray, last);
                 }
-                qn = queue_push(q, qn, cont, o, d, queue_tag(off, depth, from));
+                qn = queue_push(q, qn, cont, o, d, queue_tag(off, depth, key));
             };
             for (uint64_t b = first; b < end; b += 64) {
                 const uint64_t i = b + (uint64_t)lane;
                 bool cont = false;
                 V3 o = v3(0.f, 0.f, 0.f), d = o;
-                int depth = 0, last = 0, from = -1;
+                int depth = 0, last = 0, key = -1;
                 if (i < end) {
                     const uint64_t ray = J.ray_begin + i;
                     uint32_t cell;
                     sun_ray(J, S, ray, wc, o, d, cell);
                     float tmin = kTminPrimary;
-                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, cell_list(J, cell), from);
+                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, cell_list(J, cell), key);
                     if (!cont) finish_ray(J, ray, last);
                 }
-                qn = queue_push(q, qn, cont, o, d, queue_tag(i - first, depth, from));
+                qn = queue_push(q, qn, cont, o, d, queue_tag(i - first, depth, key));
                 while (qn >= 64) trace_parked(64);
             }
             while (qn > 0) trace_parked(min(qn, 64u));
@@ -652,10 +684,10 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                 uint32_t cell;
                 sun_ray(J, S, ray, wc, o, d, cell);
                 float tmin = kTminPrimary;
-                int depth = 0, last = 0, from = -1;
-                if (trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, cell_list(J, cell), from))
-                    while (trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, bounce_list(J, depth, from),
-                                                from)) {
+                int depth = 0, last = 0, key = -1;
+                if (trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, cell_list(J, cell), key))
+                    while (trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, next_list(J, depth, key),
+                                                key)) {
                     }
                 finish_ray(J, ray, last);
             }
@@ -1046,32 +1078,12 @@ __device__ bool cone_meets_box(const double a[3], double tan_phi, const double l
     return false;
 }
 
-__global__ void k_bounce_lists(const st_element_f64* __restrict__ elems, int32_t n, const int32_t* __restrict__ elem_slot,
-                               const double* __restrict__ frames, const float* __restrict__ bounds_slot,
-                               const float4* __restrict__ nodes, const SunFrame* __restrict__ sun, double theta_sun,
-                               uint32_t* __restrict__ counts, const uint2* __restrict__ range,
-                               int32_t* __restrict__ items) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const st_element_f64 E = elems[e];
-    double phi = 0.0;
-    if (!bounce_cone(E, theta_sun, &phi)) {
-        if (counts) counts[e] = kNoBounceList;
-        return;
-    }
-    uint32_t out = 0;
-    if (range) {
-        if (range[e].x > range[e].y) return;
-        out = range[e].x;
-    }
-    const double nz[3] = {frames[9 * (size_t)e + 2], frames[9 * (size_t)e + 5], frames[9 * (size_t)e + 8]};
-    const double din[3] = {-sun->sun[0], -sun->sun[1], -sun->sun[2]};
-    const double dn = din[0] * nz[0] + din[1] * nz[1] + din[2] * nz[2];
-    double a[3] = {din[0] - 2.0 * dn * nz[0], din[1] - 2.0 * dn * nz[1], din[2] - 2.0 * dn * nz[2]};
-    const double la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
-    for (int k = 0; k < 3; ++k) a[k] /= la;
-    const double tp = tan(phi);
-    const float* hb = bounds_slot + 6 * (size_t)elem_slot[e];
+// Primitives whose boxes a ray leaving box hb in a direction within the cone
+// (unit axis a, tan of the half angle tp) can reach, found by a BVH walk;
+// written to items[out ...] when items is given. Returns the count.
+__device__ uint32_t cone_walk(const float4* __restrict__ nodes, const float* __restrict__ bounds_slot,
+                              const float* hb, const double a[3], double tp, int32_t* __restrict__ items,
+                              uint32_t out) {
     auto meets = [&](const float lo[3], const float hi[3]) {
         const double dlo[3] = {(double)lo[0] - hb[3], (double)lo[1] - hb[4], (double)lo[2] - hb[5]};
         const double dhi[3] = {(double)hi[0] - hb[0], (double)hi[1] - hb[1], (double)hi[2] - hb[2]};
@@ -1096,7 +1108,7 @@ __global__ void k_bounce_lists(const st_element_f64* __restrict__ elems, int32_t
                 for (int i = 0; i < cnt; ++i) {
                     const float* b = bounds_slot + 6 * (size_t)(st + i);
                     if (!meets(b, b + 3)) continue;
-                    if (range) items[out + count] = st + i;
+                    if (items) items[out + count] = st + i;
                     ++count;
                 }
             }
@@ -1104,7 +1116,71 @@ __global__ void k_bounce_lists(const st_element_f64* __restrict__ elems, int32_t
         if (sp == 0) break;
         node = stack[--sp];
     }
-    if (counts) counts[e] = count > kMaxBounceList ? kNoBounceList : count;
+    return count;
+}
+
+// reflection axis reflect(-sun, n_e) of a sun ray off element e (unit)
+__device__ void sun_reflection_axis(const double* __restrict__ frames, int e, const double sun[3], double a[3]) {
+    const double nz[3] = {frames[9 * (size_t)e + 2], frames[9 * (size_t)e + 5], frames[9 * (size_t)e + 8]};
+    const double din[3] = {-sun[0], -sun[1], -sun[2]};
+    const double dn = din[0] * nz[0] + din[1] * nz[1] + din[2] * nz[2];
+    for (int k = 0; k < 3; ++k) a[k] = din[k] - 2.0 * dn * nz[k];
+    const double la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    for (int k = 0; k < 3; ++k) a[k] /= la;
+}
+
+// Task t: first reflections off element t (pairs == null), or second
+// reflections of rays that left element owner[t] and hit the primitive in
+// bounce item t (pairs given: bounce_items, owner). Count pass (range == null)
+// writes counts[t] (kNoBounceList: none); fill pass writes items[range[t].x ...].
+__global__ void k_cone_lists(const st_element_f64* __restrict__ elems, int32_t tasks,
+                             const int32_t* __restrict__ elem_slot, const uint32_t* __restrict__ meta,
+                             const double* __restrict__ frames, const float* __restrict__ bounds_slot,
+                             const float4* __restrict__ nodes, const SunFrame* __restrict__ sun, double theta_sun,
+                             const int32_t* __restrict__ pair_items, const int32_t* __restrict__ pair_owner,
+                             uint32_t* __restrict__ counts, const uint2* __restrict__ range,
+                             int32_t* __restrict__ items) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tasks) return;
+    uint32_t out = 0;
+    if (range) {
+        if (range[t].x > range[t].y) return;
+        out = range[t].x;
+    }
+    double a[3], phi = 0.0;
+    int from = t;  // element the rays leave from
+    if (!pair_items) {
+        if (!bounce_cone(elems[t], theta_sun, &phi)) {
+            if (counts) counts[t] = kNoBounceList;
+            return;
+        }
+        sun_reflection_axis(frames, t, sun->sun, a);
+    } else {
+        const int ea = pair_owner[t];
+        from = (int)meta_elem(meta[pair_items[t]]);
+        double phi_a = 0.0, phi_b = 0.0;
+        if (!bounce_cone(elems[ea], theta_sun, &phi_a) || !bounce_cone(elems[from], 0.0, &phi_b)) {
+            if (counts) counts[t] = kNoBounceList;
+            return;
+        }
+        // incidence within phi_a of a_A; reflection off the second mirror adds
+        // its own error and normal-spread terms (phi_b with no sun term)
+        double aa[3];
+        sun_reflection_axis(frames, ea, sun->sun, aa);
+        const double nb[3] = {frames[9 * (size_t)from + 2], frames[9 * (size_t)from + 5], frames[9 * (size_t)from + 8]};
+        const double dn = aa[0] * nb[0] + aa[1] * nb[1] + aa[2] * nb[2];
+        for (int k = 0; k < 3; ++k) a[k] = aa[k] - 2.0 * dn * nb[k];
+        const double la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        for (int k = 0; k < 3; ++k) a[k] /= la;
+        phi = phi_a + phi_b;
+        if (phi >= 0.35) {
+            if (counts) counts[t] = kNoBounceList;
+            return;
+        }
+    }
+    const uint32_t count = cone_walk(nodes, bounds_slot, bounds_slot + 6 * (size_t)elem_slot[from], a, tan(phi),
+                                     range ? items : nullptr, out);
+    if (counts) counts[t] = count > kMaxBounceList ? kNoBounceList : count;
 }
 
 __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t value) {
@@ -1245,12 +1321,13 @@ hipError_t launch_grid_fill(hipStream_t s, const float* bounds, int32_t n, const
     return hipGetLastError();
 }
 
-hipError_t launch_bounce_lists(hipStream_t s, const st_element_f64* elems, int32_t n, const int32_t* elem_slot,
-                               const double* frames, const float* bounds_slot, const float4* nodes,
-                               const SunFrame* sun, double theta_sun, uint32_t* counts, const uint2* range,
-                               int32_t* items) {
-    hipLaunchKernelGGL(k_bounce_lists, dim3((n + 63) / 64), dim3(64), 0, s, elems, n, elem_slot, frames, bounds_slot,
-                       nodes, sun, theta_sun, counts, range, items);
+hipError_t launch_cone_lists(hipStream_t s, const st_element_f64* elems, int32_t tasks, const int32_t* elem_slot,
+                             const uint32_t* meta, const double* frames, const float* bounds_slot,
+                             const float4* nodes, const SunFrame* sun, double theta_sun, const int32_t* pair_items,
+                             const int32_t* pair_owner, uint32_t* counts, const uint2* range, int32_t* items) {
+    if (tasks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cone_lists, dim3((tasks + 63) / 64), dim3(64), 0, s, elems, tasks, elem_slot, meta, frames,
+                       bounds_slot, nodes, sun, theta_sun, pair_items, pair_owner, counts, range, items);
     return hipGetLastError();
 }
 

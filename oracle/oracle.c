@@ -820,30 +820,30 @@ static void trace_ray(or_worker* w, uint64_t local) {
         if (rid) rid[nd] = (uint32_t)h.elem;
         if (w->hits) w->hits[h.elem]++;
         w->stats[2]++;
-        /* reflectivity draw (new): absorbed when u >= rho; rho is rounded to
-         * FP32 as the device stores it, so both sides compare the same numbers */
+        /* Draws of bounce nd: block 2nd-1 = (reflectivity, slope xr, slope xp,
+         * specular xr), block 2nd = (specular xp, -, -, -), each drawn only when
+         * one of its optics terms is active. Absorbed when u >= rho; rho is
+         * rounded to FP32 as the device stores it, so both sides compare the
+         * same numbers. */
         const double rho = (double)(float)p->rho;
-        if (rho < 1.0) {
-            double ur[4];
-            draw4(ray, (uint32_t)(2 * nd), dsc->job_id, dsc->seed, ur);
-            if (ur[0] >= rho) { w->stats[5]++; return; }
-        }
+        double ue[4] = {0, 0, 0, 0};
+        if (rho < 1.0 || p->slope > 0 || p->spec > 0) draw4(ray, (uint32_t)(2 * nd - 1), dsc->job_id, dsc->seed, ue);
+        if (rho < 1.0 && ue[0] >= rho) { w->stats[5]++; return; }
         /* faceforward(n, -d, n) (:38): n facing the incoming ray */
         double nf[3] = {h.n[0], h.n[1], h.n[2]};
         if (dot3(d, nf) > 0) { nf[0] = -nf[0]; nf[1] = -nf[1]; nf[2] = -nf[2]; }
-        double ue[4] = {0, 0, 0, 0};
-        if (p->slope > 0 || p->spec > 0) draw4(ray, (uint32_t)(2 * nd - 1), dsc->job_id, dsc->seed, ue);
         if (p->slope > 0) {
             double tmp[3];
-            or_perturb(nf, p->slope, ue[0], ue[1], tmp);
+            or_perturb(nf, p->slope, ue[1], ue[2], tmp);
             memcpy(nf, tmp, sizeof(tmp));
         }
         double rd[3];
         or_reflect(d, nf, rd);
         normalize3(rd);
         if (p->spec > 0) {
-            double tmp[3];
-            or_perturb(rd, p->spec, ue[2], ue[3], tmp);
+            double us[4], tmp[3];
+            draw4(ray, (uint32_t)(2 * nd), dsc->job_id, dsc->seed, us);
+            or_perturb(rd, p->spec, ue[3], us[0], tmp);
             memcpy(rd, tmp, sizeof(tmp));
         }
         memcpy(o, h.p, sizeof(o));

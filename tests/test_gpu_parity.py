@@ -194,6 +194,31 @@ def test_candidate_lists_equal_bvh(gpu_ctx, case):
             assert np.array_equal(r[k], ref[k]), (case, kernel, k)
 
 
+def test_work_counters(gpu_ctx):
+    """ST_TRACE_COUNT_WORK leaves every result unchanged and counts the
+    closest-hit work: with the candidate lists a ray needs a few primitive
+    tests and almost no BVH nodes; through the BVH alone it visits many nodes
+    and the closest-hit work is more than twice as large."""
+    w = scenes.heliostat_field(10_000)
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param, position_sampling=_abi.POS_STRATIFIED)
+    n = 2_000_000
+    kw = dict(max_depth=5, flux_nx=w.flux_nx, flux_ny=w.flux_ny, seed=3)
+    plain = sc.trace(n, **kw).numpy()
+    counted = sc.trace(n, count_work=True, **kw).numpy()
+    bvh = sc.trace(n, count_work=True, bvh_only=True, **kw).numpy()
+    for k in ("flux", "element_hits"):
+        assert np.array_equal(plain[k], counted[k]) and np.array_equal(plain[k], bvh[k])
+    assert np.array_equal(plain["stats"][:11], counted["stats"][:11])
+    assert np.array_equal(plain["stats"][:11], bvh["stats"][:11])
+    assert not plain["stats"][11:15].any()
+    c, b = counted["stats"].astype(np.int64) / n, bvh["stats"].astype(np.int64) / n
+    assert c[14] > 0 and b[14] > 0                       # the cylinder receiver is tested
+    assert c[11] < 0.5 and b[11] > 10                    # node visits per ray
+    work = lambda x: 34 * x[11] + 30 * x[12] + 70 * x[14]   # lane operations per node / flat / cylinder test
+    assert work(c) < 0.5 * work(b)
+
+
 def test_full_size_field_properties(gpu_ctx):
     """1e8 rays on the 10k field: outcomes balance, flux = binned hits, element
     counts add up, reflectivity 0.93 within binomial noise, run-to-run identical."""
