@@ -228,6 +228,17 @@ int st_scene_get_aabbs(st_scene scene, float* lo_hi /* n*6 reference AABBs */);
 int st_scene_bvh_info(st_scene scene, int32_t* n_nodes, int32_t* depth, int32_t* max_leaf);
 int st_scene_synchronize(st_scene scene);
 
+/* Candidate lists (new). Sun rays test the primitives binned under their
+ * sun-plane cell; first and second reflections off mirrors test per-mirror
+ * and per-mirror-pair lists of everything their bounded reflection cone can
+ * reach. Hits are identical to BVH traversal. The sun-plane grid is rebuilt
+ * with the sun plane; the reflection lists lazily, once 2^24 rays have been
+ * traced on the current scene state (earlier traces use the BVH).
+ * st_scene_build_lists builds them now; st_scene_list_info reports their sizes
+ * (grid_items / reflection_items / pair_items, 0 when unused). */
+int st_scene_build_lists(st_scene scene);
+int st_scene_list_info(st_scene scene, uint64_t* grid_items, uint64_t* reflection_items, uint64_t* pair_items);
+
 /* Launch-shape autotuner (new): times every trace-kernel variant (block size,
  * traversal stack depth, waves per SIMD) at several persistent grid sizes on
  * sample_rays rays of this scene with its current sun, keeps the fastest as

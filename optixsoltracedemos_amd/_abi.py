@@ -219,6 +219,8 @@ SIGNATURES = {
     "st_scene_bvh_info": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "st_scene_synchronize": (C.c_int, [P]),
     "st_scene_autotune": (C.c_int, [P, C.c_uint64, C.c_int32, C.POINTER(TuneResult)]),
+    "st_scene_build_lists": (C.c_int, [P]),
+    "st_scene_list_info": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "st_trace": (C.c_int, [P, C.POINTER(TraceDesc), C.POINTER(Outputs)]),
     "st_flux_to_watts": (C.c_int, [P, C.POINTER(TraceDesc), P, P, C.POINTER(C.c_double)]),
     "st_trace_batch": (C.c_int, [P, C.POINTER(BatchDesc), P, P, P]),

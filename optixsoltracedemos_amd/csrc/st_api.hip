@@ -279,8 +279,23 @@ int scene_sun_plane(st_scene s) {
     s->plane.distance = F.distance;
     s->plane.area = F.area;
     s->plane.box_half_angle = F.box_half_angle;
-    ST_TRY(scene_grid(s));
-    return scene_bounce_lists(s);
+    // reflection lists cost more to build than the grid: rebuilt on demand
+    s->bounce_valid = s->pair_valid = false;
+    s->lists_stale = true;
+    s->rays_since_change = 0;
+    return scene_grid(s);
+}
+
+// Reflection candidate lists are (re)built once enough rays are traced on the
+// current scene state to repay the build (about a millisecond on a 10k-mirror
+// field); until then reflected rays traverse the BVH.
+int scene_ensure_lists(st_scene s, uint64_t rays) {
+    if (!s->lists_stale || !s->sun_set) return ST_OK;
+    s->rays_since_change += rays;
+    if (s->rays_since_change < kListMinRays) return ST_OK;
+    ST_TRY(scene_bounce_lists(s));
+    s->lists_stale = false;
+    return ST_OK;
 }
 
 // Largest angle between a sampled sun ray and the sun axis. Pillbox: the half
@@ -685,6 +700,8 @@ int validate_trace(st_scene s, const st_trace_desc* d, const st_outputs* out) {
 extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* out) {
     ST_TRY(validate_trace(s, d, out));
     HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    if (!(d->flags & ST_TRACE_BVH_ONLY))
+        ST_TRY(scene_ensure_lists(s, d->ray_count ? d->ray_count : d->num_rays - d->ray_begin));
     JobDev J;
     ST_TRY(fill_job(s, d, out, s->d_prims.ptr, s->d_nodes.ptr, s->d_sun.ptr, &J));
     hipStream_t st = d->stream ? (hipStream_t)d->stream : s->ctx->stream;
@@ -703,12 +720,31 @@ extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* ou
     return ST_OK;
 }
 
+extern "C" int st_scene_build_lists(st_scene s) {
+    if (!s) return fail(ST_ERR_INVALID, "null scene");
+    if (!s->sun_set) return fail(ST_ERR_STATE, "st_scene_set_sun must be called before building lists");
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    if (!s->lists_stale) return ST_OK;
+    ST_TRY(scene_bounce_lists(s));
+    s->lists_stale = false;
+    return ST_OK;
+}
+
+extern "C" int st_scene_list_info(st_scene s, uint64_t* grid_items, uint64_t* reflection_items, uint64_t* pair_items) {
+    if (!s) return fail(ST_ERR_INVALID, "null scene");
+    if (grid_items) *grid_items = s->grid_valid ? s->grid_items : 0;
+    if (reflection_items) *reflection_items = s->bounce_valid ? s->bounce_items : 0;
+    if (pair_items) *pair_items = s->pair_valid ? s->pair_items : 0;
+    return ST_OK;
+}
+
 extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_depth, st_tune_result* out) {
     if (!s || !out) return fail(ST_ERR_INVALID, "null argument");
     if (!s->sun_set) return fail(ST_ERR_STATE, "st_scene_set_sun must be called before tuning");
     if (sample_rays < 64 || sample_rays > (1ull << 32)) return fail(ST_ERR_INVALID, "sample_rays outside [64, 2^32]");
     if (max_depth < 1 || max_depth > kMaxDepthLimit) return fail(ST_ERR_INVALID, "bad max_depth %d", max_depth);
     HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    ST_TRY(scene_ensure_lists(s, sample_rays));  // tune the path production traces will take
     hipStream_t st = s->ctx->stream;
     DevBuf hits, stats, flux;
     ST_TRY(hits.ensure(sizeof(uint64_t) * s->n, "tune hits"));

@@ -107,6 +107,8 @@ struct st_scene_s {
     std::vector<uint2> h_bounce_range, h_pair_range;
     std::vector<int32_t> h_pair_owner;
     bool bounce_valid = false, pair_valid = false;
+    bool lists_stale = true;         // scene state changed since the lists were built
+    uint64_t rays_since_change = 0;  // rays traced on the current state without lists
     uint64_t bounce_items = 0, pair_items = 0;
     bool sun_set = false;
     st_sun_desc sun{};
@@ -124,6 +126,8 @@ int scene_refresh(st_scene s, bool use_overrides);
 int scene_sun_plane(st_scene s);  // also rebuilds the sun-plane candidate grid
 int scene_grid(st_scene s);
 int scene_bounce_lists(st_scene s);
+int scene_ensure_lists(st_scene s, uint64_t rays);
+constexpr uint64_t kListMinRays = 1ull << 24;  // traced rays that repay a list build
 int resolve_flux_element(st_scene s, int32_t requested, int32_t nx, int32_t ny, int32_t* slot);
 int fill_job(st_scene s, const st_trace_desc* d, const st_outputs* out, const void* prims, const void* nodes,
              const void* sun, JobDev* J);

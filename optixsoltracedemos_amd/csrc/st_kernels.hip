@@ -1051,29 +1051,47 @@ __device__ bool bounce_cone(const st_element_f64& E, double theta_sun, double* p
     return p < 0.35;
 }
 
-__device__ bool cone_meets_box(const double a[3], double tan_phi, const double lo[3], const double hi[3]) {
-    double far2 = 0.0;
+// Cone from the apex (origin) with unit axis a and half angle phi; pieces of
+// the cone are bounded by boxes over [0, first], then [t, t * ratio] up to the
+// far corner of the tested box (ratio fixed per launch from the scene size).
+struct Cone {
+    double a[3];
+    double tp, cp, sp;  // tan, cos, sin of phi
+    double ratio;
+};
+constexpr int kConePieces = 32;
+constexpr double kConeFirst = 0.25;
+
+__device__ bool cone_meets_box(const Cone& C, const double lo[3], const double hi[3]) {
+    // the cone misses the box if it misses the box's bounding sphere: the
+    // sphere spans asin(r / |c|) around its centre direction
+    double r2 = 0.0, d2 = 0.0, ac = 0.0, far2 = 0.0;
     for (int k = 0; k < 3; ++k) {
+        const double c = 0.5 * (lo[k] + hi[k]), h = 0.5 * (hi[k] - lo[k]);
+        r2 += h * h;
+        d2 += c * c;
+        ac += C.a[k] * c;
         const double m = fmax(fabs(lo[k]), fabs(hi[k]));
         far2 += m * m;
     }
+    if (d2 > r2) {
+        const double d = sqrt(d2), sb = sqrt(r2) / d, cb = sqrt(fmax(0.0, 1.0 - sb * sb));
+        if (ac / d < C.cp * cb - C.sp * sb - 1e-9) return false;
+    }
     const double T = sqrt(far2);
-    constexpr int kPieces = 32;
-    const double first = 0.25;
-    const double ratio = T > first ? pow(T / first, 1.0 / (kPieces - 1)) : 1.0;
-    double t0 = 0.0, t1 = fmin(T, first);
-    for (int i = 0; i < kPieces; ++i) {
-        const double r0 = t0 * tan_phi, r1 = t1 * tan_phi;
+    double t0 = 0.0, t1 = fmin(T, kConeFirst);
+    for (int i = 0; i < kConePieces; ++i) {
+        const double r0 = t0 * C.tp, r1 = t1 * C.tp;
         bool meet = true;
         for (int k = 0; k < 3 && meet; ++k) {
-            const double c0 = a[k] * t0, c1 = a[k] * t1;
+            const double c0 = C.a[k] * t0, c1 = C.a[k] * t1;
             const double slo = fmin(c0 - r0, c1 - r1), shi = fmax(c0 + r0, c1 + r1);
             meet = !(shi < lo[k] || slo > hi[k]);
         }
         if (meet) return true;
         if (t1 >= T) break;
         t0 = t1;
-        t1 = i == kPieces - 2 ? T : fmin(T, t1 * ratio);
+        t1 = i == kConePieces - 2 ? T : fmin(T, t1 * C.ratio);
     }
     return false;
 }
@@ -1082,12 +1100,11 @@ __device__ bool cone_meets_box(const double a[3], double tan_phi, const double l
 // (unit axis a, tan of the half angle tp) can reach, found by a BVH walk;
 // written to items[out ...] when items is given. Returns the count.
 __device__ uint32_t cone_walk(const float4* __restrict__ nodes, const float* __restrict__ bounds_slot,
-                              const float* hb, const double a[3], double tp, int32_t* __restrict__ items,
-                              uint32_t out) {
+                              const float* hb, const Cone& C, int32_t* __restrict__ items, uint32_t out) {
     auto meets = [&](const float lo[3], const float hi[3]) {
         const double dlo[3] = {(double)lo[0] - hb[3], (double)lo[1] - hb[4], (double)lo[2] - hb[5]};
         const double dhi[3] = {(double)hi[0] - hb[0], (double)hi[1] - hb[1], (double)hi[2] - hb[2]};
-        return cone_meets_box(a, tp, dlo, dhi);
+        return cone_meets_box(C, dlo, dhi);
     };
     uint32_t count = 0;
     int32_t stack[kStackDepth + 1];  // depth <= 31 levels, one pending sibling each
@@ -1178,7 +1195,18 @@ __global__ void k_cone_lists(const st_element_f64* __restrict__ elems, int32_t t
             return;
         }
     }
-    const uint32_t count = cone_walk(nodes, bounds_slot, bounds_slot + 6 * (size_t)elem_slot[from], a, tan(phi),
+    Cone C;
+    for (int k = 0; k < 3; ++k) C.a[k] = a[k];
+    C.tp = tan(phi);
+    C.cp = cos(phi);
+    C.sp = sin(phi);
+    // pieces reach past the scene's extent: the root's two child boxes bound it
+    const float4 r0 = nodes[0], r1 = nodes[1], r2 = nodes[2];
+    const double ex = fmax(r0.y, r1.y) - fmin(r0.x, r1.x), ey = fmax(r0.w, r1.w) - fmin(r0.z, r1.z);
+    const double ez = fmax(r2.y, r2.w) - fmin(r2.x, r2.z);
+    const double reach = 2.0 * sqrt(ex * ex + ey * ey + ez * ez) + 2.0 * kConeFirst;
+    C.ratio = pow(reach / kConeFirst, 1.0 / (kConePieces - 2));
+    const uint32_t count = cone_walk(nodes, bounds_slot, bounds_slot + 6 * (size_t)elem_slot[from], C,
                                      range ? items : nullptr, out);
     if (counts) counts[t] = count > kMaxBounceList ? kNoBounceList : count;
 }

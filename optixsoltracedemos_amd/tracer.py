@@ -120,6 +120,14 @@ class Scene:
         call("st_scene_autotune", self.h, int(sample_rays), int(max_depth), C.byref(r))
         return {k: getattr(r, k) for k, _ in _abi.TuneResult._fields_ if k != "reserved"}
 
+    def build_lists(self):
+        call("st_scene_build_lists", self.h)
+
+    def list_info(self) -> dict:
+        g, r, p = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        call("st_scene_list_info", self.h, C.byref(g), C.byref(r), C.byref(p))
+        return {"grid_items": g.value, "reflection_items": r.value, "pair_items": p.value}
+
     def synchronize(self):
         call("st_scene_synchronize", self.h)
 

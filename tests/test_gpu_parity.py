@@ -188,10 +188,33 @@ def test_candidate_lists_equal_bvh(gpu_ctx, case):
     sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=pos)
     kw = dict(max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny, records=True, seed=5)
     ref = sc.trace(n, bvh_only=True, **kw).numpy()
+    sc.build_lists()
+    info = sc.list_info()
+    if case.startswith("field") and case != "field_low_sun":
+        assert info["grid_items"] > 0 and info["reflection_items"] > 0 and info["pair_items"] > 0, info
     for kernel in (_abi.KERNEL_QUEUED, _abi.KERNEL_MEGA):
         r = sc.trace(n, kernel=kernel, **kw).numpy()
         for k in ("flux", "element_hits", "stats", "records", "record_elements"):
             assert np.array_equal(r[k], ref[k]), (case, kernel, k)
+
+
+def test_lists_build_lazily_and_follow_the_sun(gpu_ctx):
+    """Reflection lists appear only after 2^24 traced rays (or an explicit
+    build) and are dropped by a sun change; results never depend on them."""
+    w = scenes.heliostat_field(2_000)
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param, position_sampling=_abi.POS_STRATIFIED)
+    kw = dict(max_depth=5, flux_nx=w.flux_nx, flux_ny=w.flux_ny, seed=9)
+    assert sc.list_info()["reflection_items"] == 0 and sc.list_info()["grid_items"] > 0
+    a = sc.trace(1_000_000, **kw).numpy()
+    assert sc.list_info()["reflection_items"] == 0
+    sc.trace(1 << 24, **kw)
+    assert sc.list_info()["reflection_items"] > 0 and sc.list_info()["pair_items"] > 0
+    b = sc.trace(1_000_000, **kw).numpy()
+    for k in ("flux", "element_hits", "stats"):
+        assert np.array_equal(a[k], b[k])
+    sc.set_sun(scenes.sun_vector(40.0, 150.0), w.sun_param, position_sampling=_abi.POS_STRATIFIED)
+    assert sc.list_info()["reflection_items"] == 0
 
 
 def test_work_counters(gpu_ctx):
@@ -204,6 +227,7 @@ def test_work_counters(gpu_ctx):
     sc.set_sun(w.sun_vector, w.sun_param, position_sampling=_abi.POS_STRATIFIED)
     n = 2_000_000
     kw = dict(max_depth=5, flux_nx=w.flux_nx, flux_ny=w.flux_ny, seed=3)
+    sc.build_lists()
     plain = sc.trace(n, **kw).numpy()
     counted = sc.trace(n, count_work=True, **kw).numpy()
     bvh = sc.trace(n, count_work=True, bvh_only=True, **kw).numpy()
