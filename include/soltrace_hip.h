@@ -232,8 +232,10 @@ int st_scene_synchronize(st_scene scene);
  * sun-plane cell; first and second reflections off mirrors test per-mirror
  * and per-mirror-pair lists of everything their bounded reflection cone can
  * reach. Hits are identical to BVH traversal. The sun-plane grid is rebuilt
- * with the sun plane; the reflection lists lazily, once 2^24 rays have been
- * traced on the current scene state (earlier traces use the BVH).
+ * with the sun plane; the reflection lists lazily, once enough rays have been
+ * traced on the current scene state to repay their build (2^26 at first, then
+ * scaled by the measured build time; earlier traces use the BVH). The
+ * autotuner builds them.
  * st_scene_build_lists builds them now; st_scene_list_info reports their sizes
  * (grid_items / reflection_items / pair_items, 0 when unused). */
 int st_scene_build_lists(st_scene scene);

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -287,13 +288,17 @@ int scene_sun_plane(st_scene s) {
 }
 
 // Reflection candidate lists are (re)built once enough rays are traced on the
-// current scene state to repay the build (about a millisecond on a 10k-mirror
-// field); until then reflected rays traverse the BVH.
-int scene_ensure_lists(st_scene s, uint64_t rays) {
+// current scene state to repay the build; until then reflected rays traverse
+// the BVH. The threshold follows the scene's last measured build time at
+// kListSavingPerRay seconds saved per traced ray.
+int scene_ensure_lists(st_scene s, uint64_t rays, bool force) {
     if (!s->lists_stale || !s->sun_set) return ST_OK;
     s->rays_since_change += rays;
-    if (s->rays_since_change < kListMinRays) return ST_OK;
+    if (!force && s->rays_since_change < s->list_min_rays) return ST_OK;
+    const auto t0 = std::chrono::steady_clock::now();
     ST_TRY(scene_bounce_lists(s));
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    s->list_min_rays = (uint64_t)std::min(4.0e9, std::max((double)kListMinRays, dt / kListSavingPerRay));
     s->lists_stale = false;
     return ST_OK;
 }
@@ -307,8 +312,8 @@ static double sun_cone_angle(const st_sun_desc& sun) {
 }
 static double sun_cone_tan(const st_sun_desc& sun) { return tan(sun_cone_angle(sun)); }
 
-// count pass -> host prefix sum (lists over kMaxBounceList or marked none get
-// an empty-but-invalid range) -> fill pass. Returns the number of items.
+// walk pass (lists into per-task scratch) -> host prefix sum (lists over
+// kMaxBounceList or marked none get an empty-but-invalid range) -> compaction.
 template <class Launch>
 static int build_cone_lists(st_scene s, size_t tasks, DevBuf& counts, DevBuf& range, DevBuf& items,
                             std::vector<uint2>& h_range, uint64_t* total_out, Launch launch) {
@@ -317,7 +322,8 @@ static int build_cone_lists(st_scene s, size_t tasks, DevBuf& counts, DevBuf& ra
     if (tasks == 0) return ST_OK;
     ST_TRY(counts.ensure(sizeof(uint32_t) * tasks, "list counts"));
     ST_TRY(range.ensure(sizeof(uint2) * tasks, "list ranges"));
-    HIP_TRY(launch((uint32_t*)counts.ptr, nullptr, nullptr), "list count kernel");
+    ST_TRY(s->d_list_scratch.ensure(sizeof(int32_t) * kMaxBounceList * tasks, "list scratch"));
+    HIP_TRY(launch((uint32_t*)counts.ptr, (int32_t*)s->d_list_scratch.ptr), "list walk kernel");
     s->h_counts.resize(tasks);
     h_range.resize(tasks);
     HIP_TRY(hipMemcpyAsync(s->h_counts.data(), counts.ptr, sizeof(uint32_t) * tasks, hipMemcpyDeviceToHost, st),
@@ -338,8 +344,10 @@ static int build_cone_lists(st_scene s, size_t tasks, DevBuf& counts, DevBuf& ra
     ST_TRY(items.ensure(sizeof(int32_t) * total, "list items"));
     HIP_TRY(hipMemcpyAsync(range.ptr, h_range.data(), sizeof(uint2) * tasks, hipMemcpyHostToDevice, st),
             "list ranges upload");
-    HIP_TRY(launch(nullptr, (const uint2*)range.ptr, (int32_t*)items.ptr), "list fill kernel");
-    HIP_TRY(hipStreamSynchronize(st), "list sync");  // h_range is reused by the next rebuild
+    HIP_TRY(launch_list_compact(st, (int32_t)tasks, (const uint2*)range.ptr, (const int32_t*)s->d_list_scratch.ptr,
+                                (int32_t*)items.ptr),
+            "list compaction kernel");
+    HIP_TRY(hipStreamSynchronize(st), "list sync");  // h_range and the scratch are reused by the next pass
     return ST_OK;
 }
 
@@ -347,17 +355,16 @@ int scene_bounce_lists(st_scene s) {
     hipStream_t st = s->ctx->stream;
     s->bounce_valid = s->pair_valid = false;
     const double theta = sun_cone_angle(s->sun);
-    auto common = [&](int32_t tasks, const int32_t* pitems, const int32_t* powner, uint32_t* c, const uint2* r,
-                      int32_t* it) {
+    auto common = [&](int32_t tasks, const int32_t* pitems, const int32_t* powner, uint32_t* c, int32_t* scratch) {
         return launch_cone_lists(st, (const st_element_f64*)s->d_elems.ptr, tasks, (const int32_t*)s->d_elem_slot.ptr,
                                  (const uint32_t*)s->d_meta.ptr, (const double*)s->d_frames.ptr,
                                  (const float*)s->d_bounds.ptr, (const float4*)s->d_nodes.ptr,
-                                 (const SunFrame*)s->d_sun.ptr, theta, pitems, powner, c, r, it);
+                                 (const SunFrame*)s->d_sun.ptr, theta, pitems, powner, c, scratch);
     };
     // first reflections: one list per element
     ST_TRY(build_cone_lists(s, (size_t)s->n, s->d_bounce_counts, s->d_bounce_range, s->d_bounce_items,
                             s->h_bounce_range, &s->bounce_items,
-                            [&](uint32_t* c, const uint2* r, int32_t* it) { return common(s->n, nullptr, nullptr, c, r, it); }));
+                            [&](uint32_t* c, int32_t* scratch) { return common(s->n, nullptr, nullptr, c, scratch); }));
     if (s->bounce_items == 0) return ST_OK;
     s->bounce_valid = true;
     // second reflections: one list per first-reflection list entry
@@ -370,9 +377,9 @@ int scene_bounce_lists(st_scene s) {
             "pair owners upload");
     const int32_t tasks = (int32_t)std::min<uint64_t>(s->bounce_items, 0x7FFFFFFF);
     ST_TRY(build_cone_lists(s, (size_t)tasks, s->d_pair_counts, s->d_pair_range, s->d_pair_items, s->h_pair_range,
-                            &s->pair_items, [&](uint32_t* c, const uint2* r, int32_t* it) {
+                            &s->pair_items, [&](uint32_t* c, int32_t* scratch) {
                                 return common(tasks, (const int32_t*)s->d_bounce_items.ptr,
-                                              (const int32_t*)s->d_pair_owner.ptr, c, r, it);
+                                              (const int32_t*)s->d_pair_owner.ptr, c, scratch);
                             }));
     s->pair_valid = s->pair_items > 0;
     return ST_OK;
@@ -701,7 +708,7 @@ extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* ou
     ST_TRY(validate_trace(s, d, out));
     HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
     if (!(d->flags & ST_TRACE_BVH_ONLY))
-        ST_TRY(scene_ensure_lists(s, d->ray_count ? d->ray_count : d->num_rays - d->ray_begin));
+        ST_TRY(scene_ensure_lists(s, d->ray_count ? d->ray_count : d->num_rays - d->ray_begin, false));
     JobDev J;
     ST_TRY(fill_job(s, d, out, s->d_prims.ptr, s->d_nodes.ptr, s->d_sun.ptr, &J));
     hipStream_t st = d->stream ? (hipStream_t)d->stream : s->ctx->stream;
@@ -724,10 +731,7 @@ extern "C" int st_scene_build_lists(st_scene s) {
     if (!s) return fail(ST_ERR_INVALID, "null scene");
     if (!s->sun_set) return fail(ST_ERR_STATE, "st_scene_set_sun must be called before building lists");
     HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
-    if (!s->lists_stale) return ST_OK;
-    ST_TRY(scene_bounce_lists(s));
-    s->lists_stale = false;
-    return ST_OK;
+    return scene_ensure_lists(s, 0, true);
 }
 
 extern "C" int st_scene_list_info(st_scene s, uint64_t* grid_items, uint64_t* reflection_items, uint64_t* pair_items) {
@@ -744,7 +748,7 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
     if (sample_rays < 64 || sample_rays > (1ull << 32)) return fail(ST_ERR_INVALID, "sample_rays outside [64, 2^32]");
     if (max_depth < 1 || max_depth > kMaxDepthLimit) return fail(ST_ERR_INVALID, "bad max_depth %d", max_depth);
     HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
-    ST_TRY(scene_ensure_lists(s, sample_rays));  // tune the path production traces will take
+    ST_TRY(scene_ensure_lists(s, sample_rays, true));  // tune the path production traces will take
     hipStream_t st = s->ctx->stream;
     DevBuf hits, stats, flux;
     ST_TRY(hits.ensure(sizeof(uint64_t) * s->n, "tune hits"));

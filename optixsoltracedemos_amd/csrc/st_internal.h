@@ -91,15 +91,17 @@ hipError_t launch_grid_fill(hipStream_t s, const float* bounds, int32_t n, const
                             const uint32_t* start, uint32_t* cursor, int32_t* items);
 // candidate lists of reflected rays (k_cone_lists). Task t: first
 // reflections off element t (pair_items null), or second reflections of rays
-// that left element pair_owner[t] and hit primitive pair_items[t]. Count pass
-// (range null) writes counts[t] (kNoBounceList: none); fill pass writes
-// items[range[t].x ...].
+// that left element pair_owner[t] and hit primitive pair_items[t]. Writes
+// counts[t] (kNoBounceList: none) and scratch[t * kMaxBounceList ...];
+// launch_list_compact then packs the lists at items[range[t].x ...].
 constexpr uint32_t kNoBounceList = 0xFFFFFFFFu;
 constexpr uint32_t kMaxBounceList = 64;
 hipError_t launch_cone_lists(hipStream_t s, const st_element_f64* elems, int32_t tasks, const int32_t* elem_slot,
                              const uint32_t* meta, const double* frames, const float* bounds_slot,
                              const float4* nodes, const SunFrame* sun, double theta_sun, const int32_t* pair_items,
-                             const int32_t* pair_owner, uint32_t* counts, const uint2* range, int32_t* items);
+                             const int32_t* pair_owner, uint32_t* counts, int32_t* scratch);
+hipError_t launch_list_compact(hipStream_t s, int32_t tasks, const uint2* range, const int32_t* scratch,
+                               int32_t* items);
 hipError_t launch_fill_u64(hipStream_t s, void* p, uint64_t n_u64, uint64_t value);
 hipError_t launch_flux_watts(hipStream_t s, const int64_t* flux, int32_t nbins, const SunFrame* sun,
                              double num_rays, double bin_area, double* out, double* power_per_ray_dev);

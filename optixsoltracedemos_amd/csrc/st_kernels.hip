@@ -1096,11 +1096,11 @@ __device__ bool cone_meets_box(const Cone& C, const double lo[3], const double h
     return false;
 }
 
-// Primitives whose boxes a ray leaving box hb in a direction within the cone
-// (unit axis a, tan of the half angle tp) can reach, found by a BVH walk;
-// written to items[out ...] when items is given. Returns the count.
+// Primitives whose boxes a ray leaving box hb in a direction within the cone C
+// can reach, found by a BVH walk; the first `cap` are written to items[0 ...].
+// Returns the count, or cap + 1 as soon as it exceeds cap.
 __device__ uint32_t cone_walk(const float4* __restrict__ nodes, const float* __restrict__ bounds_slot,
-                              const float* hb, const Cone& C, int32_t* __restrict__ items, uint32_t out) {
+                              const float* hb, const Cone& C, int32_t* __restrict__ items, uint32_t cap) {
     auto meets = [&](const float lo[3], const float hi[3]) {
         const double dlo[3] = {(double)lo[0] - hb[3], (double)lo[1] - hb[4], (double)lo[2] - hb[5]};
         const double dhi[3] = {(double)hi[0] - hb[0], (double)hi[1] - hb[1], (double)hi[2] - hb[2]};
@@ -1125,8 +1125,8 @@ __device__ uint32_t cone_walk(const float4* __restrict__ nodes, const float* __r
                 for (int i = 0; i < cnt; ++i) {
                     const float* b = bounds_slot + 6 * (size_t)(st + i);
                     if (!meets(b, b + 3)) continue;
-                    if (items) items[out + count] = st + i;
-                    ++count;
+                    if (count == cap) return cap + 1;
+                    items[count++] = st + i;
                 }
             }
         }
@@ -1146,29 +1146,23 @@ __device__ void sun_reflection_axis(const double* __restrict__ frames, int e, co
     for (int k = 0; k < 3; ++k) a[k] /= la;
 }
 
-// Task t: first reflections off element t (pairs == null), or second
-// reflections of rays that left element owner[t] and hit the primitive in
-// bounce item t (pairs given: bounce_items, owner). Count pass (range == null)
-// writes counts[t] (kNoBounceList: none); fill pass writes items[range[t].x ...].
+// Task t: first reflections off element t (pair_items == null), or second
+// reflections of rays that left element pair_owner[t] and hit the primitive
+// pair_items[t]. Writes counts[t] (kNoBounceList: none or longer than
+// kMaxBounceList) and the list to scratch[t * kMaxBounceList ...].
 __global__ void k_cone_lists(const st_element_f64* __restrict__ elems, int32_t tasks,
                              const int32_t* __restrict__ elem_slot, const uint32_t* __restrict__ meta,
                              const double* __restrict__ frames, const float* __restrict__ bounds_slot,
                              const float4* __restrict__ nodes, const SunFrame* __restrict__ sun, double theta_sun,
                              const int32_t* __restrict__ pair_items, const int32_t* __restrict__ pair_owner,
-                             uint32_t* __restrict__ counts, const uint2* __restrict__ range,
-                             int32_t* __restrict__ items) {
+                             uint32_t* __restrict__ counts, int32_t* __restrict__ scratch) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= tasks) return;
-    uint32_t out = 0;
-    if (range) {
-        if (range[t].x > range[t].y) return;
-        out = range[t].x;
-    }
     double a[3], phi = 0.0;
     int from = t;  // element the rays leave from
     if (!pair_items) {
         if (!bounce_cone(elems[t], theta_sun, &phi)) {
-            if (counts) counts[t] = kNoBounceList;
+            counts[t] = kNoBounceList;
             return;
         }
         sun_reflection_axis(frames, t, sun->sun, a);
@@ -1177,7 +1171,7 @@ __global__ void k_cone_lists(const st_element_f64* __restrict__ elems, int32_t t
         from = (int)meta_elem(meta[pair_items[t]]);
         double phi_a = 0.0, phi_b = 0.0;
         if (!bounce_cone(elems[ea], theta_sun, &phi_a) || !bounce_cone(elems[from], 0.0, &phi_b)) {
-            if (counts) counts[t] = kNoBounceList;
+            counts[t] = kNoBounceList;
             return;
         }
         // incidence within phi_a of a_A; reflection off the second mirror adds
@@ -1191,7 +1185,7 @@ __global__ void k_cone_lists(const st_element_f64* __restrict__ elems, int32_t t
         for (int k = 0; k < 3; ++k) a[k] /= la;
         phi = phi_a + phi_b;
         if (phi >= 0.35) {
-            if (counts) counts[t] = kNoBounceList;
+            counts[t] = kNoBounceList;
             return;
         }
     }
@@ -1207,8 +1201,18 @@ __global__ void k_cone_lists(const st_element_f64* __restrict__ elems, int32_t t
     const double reach = 2.0 * sqrt(ex * ex + ey * ey + ez * ez) + 2.0 * kConeFirst;
     C.ratio = pow(reach / kConeFirst, 1.0 / (kConePieces - 2));
     const uint32_t count = cone_walk(nodes, bounds_slot, bounds_slot + 6 * (size_t)elem_slot[from], C,
-                                     range ? items : nullptr, out);
-    if (counts) counts[t] = count > kMaxBounceList ? kNoBounceList : count;
+                                     scratch + (size_t)t * kMaxBounceList, kMaxBounceList);
+    counts[t] = count > kMaxBounceList ? kNoBounceList : count;
+}
+
+// items[range[t].x + k] = scratch[t * kMaxBounceList + k] for every listed task
+__global__ void k_list_compact(int32_t tasks, const uint2* __restrict__ range, const int32_t* __restrict__ scratch,
+                               int32_t* __restrict__ items) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)tasks * kMaxBounceList) return;
+    const int t = (int)(g / kMaxBounceList), k = (int)(g % kMaxBounceList);
+    const uint2 r = range[t];
+    if (r.x <= r.y && (uint32_t)k < r.y - r.x) items[r.x + k] = scratch[g];
 }
 
 __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t value) {
@@ -1352,10 +1356,19 @@ hipError_t launch_grid_fill(hipStream_t s, const float* bounds, int32_t n, const
 hipError_t launch_cone_lists(hipStream_t s, const st_element_f64* elems, int32_t tasks, const int32_t* elem_slot,
                              const uint32_t* meta, const double* frames, const float* bounds_slot,
                              const float4* nodes, const SunFrame* sun, double theta_sun, const int32_t* pair_items,
-                             const int32_t* pair_owner, uint32_t* counts, const uint2* range, int32_t* items) {
+                             const int32_t* pair_owner, uint32_t* counts, int32_t* scratch) {
     if (tasks <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_cone_lists, dim3((tasks + 63) / 64), dim3(64), 0, s, elems, tasks, elem_slot, meta, frames,
-                       bounds_slot, nodes, sun, theta_sun, pair_items, pair_owner, counts, range, items);
+                       bounds_slot, nodes, sun, theta_sun, pair_items, pair_owner, counts, scratch);
+    return hipGetLastError();
+}
+
+hipError_t launch_list_compact(hipStream_t s, int32_t tasks, const uint2* range, const int32_t* scratch,
+                               int32_t* items) {
+    if (tasks <= 0) return hipSuccess;
+    const int64_t total = (int64_t)tasks * kMaxBounceList;
+    hipLaunchKernelGGL(k_list_compact, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, tasks, range, scratch,
+                       items);
     return hipGetLastError();
 }
 

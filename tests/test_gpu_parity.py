@@ -199,8 +199,9 @@ def test_candidate_lists_equal_bvh(gpu_ctx, case):
 
 
 def test_lists_build_lazily_and_follow_the_sun(gpu_ctx):
-    """Reflection lists appear only after 2^24 traced rays (or an explicit
-    build) and are dropped by a sun change; results never depend on them."""
+    """Reflection lists appear only after 2^26 traced rays on a scene state (or
+    an explicit build) and are dropped by a sun change; results never depend
+    on them."""
     w = scenes.heliostat_field(2_000)
     sc = tracer.Scene(gpu_ctx, w.elements)
     sc.set_sun(w.sun_vector, w.sun_param, position_sampling=_abi.POS_STRATIFIED)
@@ -208,7 +209,7 @@ def test_lists_build_lazily_and_follow_the_sun(gpu_ctx):
     assert sc.list_info()["reflection_items"] == 0 and sc.list_info()["grid_items"] > 0
     a = sc.trace(1_000_000, **kw).numpy()
     assert sc.list_info()["reflection_items"] == 0
-    sc.trace(1 << 24, **kw)
+    sc.trace(1 << 26, **kw)
     assert sc.list_info()["reflection_items"] > 0 and sc.list_info()["pair_items"] > 0
     b = sc.trace(1_000_000, **kw).numpy()
     for k in ("flux", "element_hits", "stats"):
