@@ -254,6 +254,11 @@ def load(path: str = LIB_PATH):
     if not os.path.exists(path):
         raise FileNotFoundError(
             f"{path} is missing: build the HIP extension first (python -c 'import __graft_entry__ as g; g.build()')")
+    # torch bundles its own HIP runtime under the same soname (libamdhip64.so.7)
+    # as the ROCm one this library links; the first one loaded serves the whole
+    # process. Loading torch first keeps torch on the runtime it was built for
+    # (the other order leaves torch reporting no GPU).
+    import torch  # noqa: F401
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -356,10 +356,11 @@ int scene_bounce_lists(st_scene s) {
     s->bounce_valid = s->pair_valid = false;
     const double theta = sun_cone_angle(s->sun);
     auto common = [&](int32_t tasks, const int32_t* pitems, const int32_t* powner, uint32_t* c, int32_t* scratch) {
-        return launch_cone_lists(st, (const st_element_f64*)s->d_elems.ptr, tasks, (const int32_t*)s->d_elem_slot.ptr,
-                                 (const uint32_t*)s->d_meta.ptr, (const double*)s->d_frames.ptr,
-                                 (const float*)s->d_bounds.ptr, (const float4*)s->d_nodes.ptr,
-                                 (const SunFrame*)s->d_sun.ptr, theta, pitems, powner, c, scratch);
+        return launch_cone_lists(st, (const st_element_f64*)s->d_elems.ptr, tasks, tasks, s->n, s->n_nodes,
+                                 (const int32_t*)s->d_elem_slot.ptr, (const uint32_t*)s->d_meta.ptr,
+                                 (const double*)s->d_frames.ptr, (const float*)s->d_bounds.ptr,
+                                 (const float4*)s->d_nodes.ptr, (const SunFrame*)s->d_sun.ptr, theta, pitems, powner,
+                                 c, nullptr, scratch);
     };
     // first reflections: one list per element
     ST_TRY(build_cone_lists(s, (size_t)s->n, s->d_bounce_counts, s->d_bounce_range, s->d_bounce_items,
@@ -386,31 +387,29 @@ int scene_bounce_lists(st_scene s) {
 }
 
 int scene_grid(st_scene s) {
-    constexpr uint32_t kCells = kGridRes * kGridRes;
-    constexpr uint32_t kMaxAverage = 6, kMaxLongest = 64;  // beyond these the BVH is cheaper
     hipStream_t st = s->ctx->stream;
     s->grid_valid = false;
-    ST_TRY(s->d_grid_counts.ensure(sizeof(uint32_t) * kCells, "grid counts"));
-    ST_TRY(s->d_grid_cursor.ensure(sizeof(uint32_t) * kCells, "grid cursor"));
-    ST_TRY(s->d_grid_start.ensure(sizeof(uint32_t) * (kCells + 1), "grid start"));
+    ST_TRY(s->d_grid_counts.ensure(sizeof(uint32_t) * kGridCells, "grid counts"));
+    ST_TRY(s->d_grid_cursor.ensure(sizeof(uint32_t) * kGridCells, "grid cursor"));
+    ST_TRY(s->d_grid_start.ensure(sizeof(uint32_t) * (kGridCells + 1), "grid start"));
     ST_TRY(s->d_grid_stats.ensure(sizeof(uint32_t) * 2, "grid stats"));
     const double tmax = sun_cone_tan(s->sun);
-    HIP_TRY(launch_fill_u64(st, s->d_grid_counts.ptr, kCells / 2, 0), "grid clear");
-    HIP_TRY(launch_grid_count(st, (const float*)s->d_bounds.ptr, s->n, (const SunFrame*)s->d_sun.ptr, tmax,
+    HIP_TRY(launch_fill_u64(st, s->d_grid_counts.ptr, kGridCells / 2, 0), "grid clear");
+    HIP_TRY(launch_grid_count(st, (const float*)s->d_bounds.ptr, s->n, 1, (const SunFrame*)s->d_sun.ptr, tmax,
                               (uint32_t*)s->d_grid_counts.ptr, (uint32_t*)s->d_grid_start.ptr,
-                              (uint32_t*)s->d_grid_stats.ptr),
+                              (uint32_t*)s->d_grid_stats.ptr, nullptr, nullptr, 0),
             "grid count kernels");
     uint32_t stats[2] = {0, 0};
     HIP_TRY(hipMemcpyAsync(stats, s->d_grid_stats.ptr, sizeof(stats), hipMemcpyDeviceToHost, st), "grid stats");
     HIP_TRY(hipStreamSynchronize(st), "grid sync");
     s->grid_items = stats[0];
     s->grid_longest = stats[1];
-    if (stats[0] == 0 || stats[0] > kMaxAverage * kCells || stats[1] > kMaxLongest) return ST_OK;
+    if (!grid_lists_pay(stats[0], stats[1])) return ST_OK;
     ST_TRY(s->d_grid_items.ensure(sizeof(int32_t) * stats[0], "grid items"));
-    HIP_TRY(launch_fill_u64(st, s->d_grid_cursor.ptr, kCells / 2, 0), "grid cursor clear");
-    HIP_TRY(launch_grid_fill(st, (const float*)s->d_bounds.ptr, s->n, (const SunFrame*)s->d_sun.ptr, tmax,
+    HIP_TRY(launch_fill_u64(st, s->d_grid_cursor.ptr, kGridCells / 2, 0), "grid cursor clear");
+    HIP_TRY(launch_grid_fill(st, (const float*)s->d_bounds.ptr, s->n, 1, (const SunFrame*)s->d_sun.ptr, tmax,
                              (const uint32_t*)s->d_grid_start.ptr, (uint32_t*)s->d_grid_cursor.ptr,
-                             (int32_t*)s->d_grid_items.ptr),
+                             (int32_t*)s->d_grid_items.ptr, stats[0]),
             "grid fill kernel");
     s->grid_valid = true;
     return ST_OK;
@@ -894,15 +893,14 @@ static bool same_batch(const BatchCache& c, const st_batch_desc* d) {
 }
 }  // namespace st
 
-static int batch_enqueue(st_scene s, hipStream_t st) {
+// per step: FP64 re-aim (+ frames for the reflection lists), refit, sun plane
+static int batch_scene_states(st_scene s, hipStream_t st) {
     BatchCache& b = s->batch;
     const int32_t K = b.steps, n = s->n;
-    if (b.flux_out) HIP_TRY(launch_fill_u64(st, b.flux_out, (uint64_t)K * b.nbins, 0), "zero flux");
-    if (b.hits_out) HIP_TRY(launch_fill_u64(st, b.hits_out, (uint64_t)K * n, 0), "zero hits");
-    if (b.stats_out) HIP_TRY(launch_fill_u64(st, b.stats_out, (uint64_t)K * ST_STAT_COUNT, 0), "zero stats");
     HIP_TRY(launch_frames_batch(st, (const st_element_f64*)s->d_elems.ptr, n, (const int32_t*)s->d_elem_slot.ptr, K,
                                 (const double*)b.d_sun_units.ptr, (const double*)b.d_target.ptr,
-                                (double*)b.d_ref_aabb.ptr, (float*)b.d_bounds.ptr, (float4*)b.d_prims.ptr),
+                                (double*)b.d_ref_aabb.ptr, (float*)b.d_bounds.ptr, (float4*)b.d_prims.ptr,
+                                b.lists ? (double*)b.d_frames.ptr : nullptr),
             "batched frames");
     HIP_TRY(launch_refit(st, (const int32_t*)s->d_child.ptr, (const int32_t*)s->d_level_nodes.ptr,
                          s->level_offsets.data(), (int32_t)s->level_offsets.size() - 1, s->n_nodes, n, K,
@@ -911,6 +909,81 @@ static int batch_enqueue(st_scene s, hipStream_t st) {
     HIP_TRY(launch_sun_plane(st, (const double*)b.d_ref_aabb.ptr, n, K, (const double*)b.d_sun_units.ptr, &b.sun,
                              (SunFrame*)b.d_sun.ptr),
             "batched sun plane");
+    return ST_OK;
+}
+
+// Sizes the per-step grids and reflection lists of a new batch. The grid
+// item capacity per step is the longest total over the steps whose grid pays,
+// found by one eager pass over the scene states (the runs recompute the same
+// states, so the same steps get grids); steps over it trace the BVH.
+static int batch_plan_lists(st_scene s, hipStream_t st) {
+    BatchCache& b = s->batch;
+    const int32_t K = b.steps, n = s->n;
+    b.grids = b.lists = false;
+    b.grid_cap = 0;
+    if (b.flags & ST_TRACE_BVH_ONLY) return ST_OK;
+    const uint64_t list_tasks = (uint64_t)K * n;
+    const uint64_t list_bytes = list_tasks * (sizeof(int32_t) * kMaxBounceList + sizeof(uint2) + 9 * sizeof(double));
+    b.lists = list_bytes <= kBatchListBytes / 2 && list_tasks * kMaxBounceList < (1ull << 32);
+    if (b.lists) {
+        ST_TRY(b.d_frames.ensure(sizeof(double) * 9 * list_tasks, "batch frames"));
+        ST_TRY(b.d_bounce_range.ensure(sizeof(uint2) * list_tasks, "batch reflection ranges"));
+        ST_TRY(b.d_bounce_scratch.ensure(sizeof(int32_t) * kMaxBounceList * list_tasks, "batch reflection lists"));
+    }
+    const uint64_t grid_bytes = (uint64_t)K * (3 * kGridCells + 1) * sizeof(uint32_t);
+    if (grid_bytes > kBatchListBytes / 4) return ST_OK;
+    ST_TRY(b.d_grid_counts.ensure(sizeof(uint32_t) * kGridCells * (size_t)K, "batch grid counts"));
+    ST_TRY(b.d_grid_cursor.ensure(sizeof(uint32_t) * kGridCells * (size_t)K, "batch grid cursor"));
+    ST_TRY(b.d_grid_start.ensure(sizeof(uint32_t) * (kGridCells + 1) * (size_t)K, "batch grid start"));
+    ST_TRY(b.d_grid_stats.ensure(sizeof(uint32_t) * 2 * (size_t)K, "batch grid stats"));
+    ST_TRY(batch_scene_states(s, st));
+    HIP_TRY(launch_fill_u64(st, b.d_grid_counts.ptr, (uint64_t)K * kGridCells / 2, 0), "batch grid clear");
+    HIP_TRY(launch_grid_count(st, (const float*)b.d_bounds.ptr, n, K, (const SunFrame*)b.d_sun.ptr, sun_cone_tan(b.sun),
+                              (uint32_t*)b.d_grid_counts.ptr, (uint32_t*)b.d_grid_start.ptr,
+                              (uint32_t*)b.d_grid_stats.ptr, nullptr, nullptr, 0),
+            "batch grid count");
+    std::vector<uint32_t> gs(2 * (size_t)K);
+    HIP_TRY(hipMemcpyAsync(gs.data(), b.d_grid_stats.ptr, sizeof(uint32_t) * gs.size(), hipMemcpyDeviceToHost, st),
+            "batch grid stats");
+    HIP_TRY(hipStreamSynchronize(st), "batch grid sync");
+    uint64_t cap = 0;
+    for (int32_t k = 0; k < K; ++k)
+        if (grid_lists_pay(gs[2 * k], gs[2 * k + 1])) cap = std::max<uint64_t>(cap, gs[2 * k]);
+    if (cap == 0 || cap * K * sizeof(int32_t) > kBatchListBytes / 4) return ST_OK;
+    ST_TRY(b.d_grid_items.ensure(sizeof(int32_t) * cap * (size_t)K, "batch grid items"));
+    b.grid_cap = (uint32_t)cap;
+    b.grids = true;
+    return ST_OK;
+}
+
+static int batch_enqueue(st_scene s, hipStream_t st) {
+    BatchCache& b = s->batch;
+    const int32_t K = b.steps, n = s->n;
+    if (b.flux_out) HIP_TRY(launch_fill_u64(st, b.flux_out, (uint64_t)K * b.nbins, 0), "zero flux");
+    if (b.hits_out) HIP_TRY(launch_fill_u64(st, b.hits_out, (uint64_t)K * n, 0), "zero hits");
+    if (b.stats_out) HIP_TRY(launch_fill_u64(st, b.stats_out, (uint64_t)K * ST_STAT_COUNT, 0), "zero stats");
+    ST_TRY(batch_scene_states(s, st));
+    if (b.grids) {
+        HIP_TRY(launch_fill_u64(st, b.d_grid_counts.ptr, (uint64_t)K * kGridCells / 2, 0), "batched grid clear");
+        HIP_TRY(launch_grid_count(st, (const float*)b.d_bounds.ptr, n, K, (const SunFrame*)b.d_sun.ptr,
+                                  sun_cone_tan(b.sun), (uint32_t*)b.d_grid_counts.ptr, (uint32_t*)b.d_grid_start.ptr,
+                                  (uint32_t*)b.d_grid_stats.ptr, (JobDev*)b.d_jobs.ptr,
+                                  (const int32_t*)b.d_grid_items.ptr, b.grid_cap),
+                "batched grid count");
+        HIP_TRY(launch_fill_u64(st, b.d_grid_cursor.ptr, (uint64_t)K * kGridCells / 2, 0), "batched grid cursor clear");
+        HIP_TRY(launch_grid_fill(st, (const float*)b.d_bounds.ptr, n, K, (const SunFrame*)b.d_sun.ptr,
+                                 sun_cone_tan(b.sun), (const uint32_t*)b.d_grid_start.ptr,
+                                 (uint32_t*)b.d_grid_cursor.ptr, (int32_t*)b.d_grid_items.ptr, b.grid_cap),
+                "batched grid fill");
+    }
+    if (b.lists)
+        HIP_TRY(launch_cone_lists(st, (const st_element_f64*)s->d_elems.ptr, K * n, n, n, s->n_nodes,
+                                  (const int32_t*)s->d_elem_slot.ptr, (const uint32_t*)s->d_meta.ptr,
+                                  (const double*)b.d_frames.ptr, (const float*)b.d_bounds.ptr,
+                                  (const float4*)b.d_nodes.ptr, (const SunFrame*)b.d_sun.ptr, sun_cone_angle(b.sun),
+                                  nullptr, nullptr, nullptr, (uint2*)b.d_bounce_range.ptr,
+                                  (int32_t*)b.d_bounce_scratch.ptr),
+                "batched reflection lists");
     JobDev dummy;
     memset(&dummy, 0, sizeof(dummy));
     HIP_TRY(launch_trace(st, (const JobDev*)b.d_jobs.ptr, (const uint64_t*)b.d_prefix.ptr, K, b.total_chunks,
@@ -987,9 +1060,12 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
         ST_TRY(b.d_jobs.ensure(sizeof(JobDev) * K, "batch jobs"));
         ST_TRY(b.d_prefix.ensure(sizeof(uint64_t) * (K + 1), "batch chunk prefix"));
         ST_TRY(b.d_counter.ensure(sizeof(uint32_t) * 2, "batch counter"));
+        HIP_TRY(hipMemcpyAsync(b.d_sun_units.ptr, units.data(), sizeof(double) * 3 * K, hipMemcpyHostToDevice, st), "upload");
+        HIP_TRY(hipMemcpyAsync(b.d_target.ptr, d->target, sizeof(double) * 3, hipMemcpyHostToDevice, st), "upload");
+        ST_TRY(batch_plan_lists(s, st));
         uint64_t total_rays = 0;
         for (int k = 0; k < K; ++k) total_rays += d->budgets[k];
-        b.shape = pick_shape(total_rays / K, d->block_size, d->grid_blocks, ST_KERNEL_AUTO, s->topo.depth,
+        b.shape = pick_shape(total_rays, d->block_size, d->grid_blocks, ST_KERNEL_AUTO, s->topo.depth,
                              s->tuned_valid ? &s->tuned : nullptr);
         std::vector<JobDev> jobs(K);
         std::vector<uint64_t> prefix(K + 1, 0);
@@ -1012,12 +1088,18 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
             ST_TRY(fill_job(s, &td, &o, (const float4*)b.d_prims.ptr + (size_t)k * n * 4,
                             (const float4*)b.d_nodes.ptr + (size_t)k * s->n_nodes * 4,
                             (const SunFrame*)b.d_sun.ptr + k, &jobs[k]));
+            // grid pointers are set by each run's grid scan; lists of step k
+            // are indexed by element within the step's range block
+            jobs[k].grid_start = nullptr;
+            jobs[k].grid_items = nullptr;
+            if (b.lists) {
+                jobs[k].bounce_range = (const uint2*)b.d_bounce_range.ptr + (size_t)k * n;
+                jobs[k].bounce_items = (const int32_t*)b.d_bounce_scratch.ptr;
+            }
             prefix[k + 1] = prefix[k] + (d->budgets[k] + b.shape.chunk - 1) / b.shape.chunk;
         }
         b.total_chunks = prefix[K];
         if (b.total_chunks >= (1ull << 32)) return fail(ST_ERR_INVALID, "too many rays for one sweep");
-        HIP_TRY(hipMemcpyAsync(b.d_sun_units.ptr, units.data(), sizeof(double) * 3 * K, hipMemcpyHostToDevice, st), "upload");
-        HIP_TRY(hipMemcpyAsync(b.d_target.ptr, d->target, sizeof(double) * 3, hipMemcpyHostToDevice, st), "upload");
         HIP_TRY(hipMemcpyAsync(b.d_jobs.ptr, jobs.data(), sizeof(JobDev) * K, hipMemcpyHostToDevice, st), "upload");
         HIP_TRY(hipMemcpyAsync(b.d_prefix.ptr, prefix.data(), sizeof(uint64_t) * (K + 1), hipMemcpyHostToDevice, st),
                 "upload");

@@ -50,7 +50,16 @@ struct BatchCache {
     uint64_t* stats_out = nullptr;
     LaunchShape shape;
     DevBuf d_sun_units, d_target, d_ref_aabb, d_bounds, d_prims, d_nodes, d_sun, d_jobs, d_prefix, d_counter;
+    // per-step sun-plane grids (items of step k at d_grid_items + k * grid_cap)
+    // and first-reflection lists (range [steps][n], items in the per-task
+    // scratch), rebuilt inside every run of the batch
+    bool grids = false, lists = false;
+    uint32_t grid_cap = 0;
+    DevBuf d_frames, d_grid_counts, d_grid_start, d_grid_cursor, d_grid_stats, d_grid_items, d_bounce_range,
+        d_bounce_scratch;
 };
+// device bytes a batch may spend on its per-step grids and reflection lists
+constexpr uint64_t kBatchListBytes = 3ull << 30;
 
 }  // namespace st
 

@@ -44,6 +44,13 @@ struct JobDev {
 };
 
 constexpr int kGridRes = 256;  // sun-plane candidate grid, the stratified-sampling cells
+constexpr int kGridCells = kGridRes * kGridRes;
+// a grid is used when its lists are short: beyond these averages or lengths
+// the BVH finds a sun ray's hit with fewer tests
+constexpr uint32_t kGridMaxAverage = 6, kGridMaxLongest = 64;
+__host__ __device__ inline bool grid_lists_pay(uint32_t total, uint32_t longest) {
+    return total > 0 && total <= kGridMaxAverage * (uint32_t)kGridCells && longest <= kGridMaxLongest;
+}
 
 struct LaunchShape {
     int block = 256;     // threads per block (multiple of 64)
@@ -69,7 +76,7 @@ hipError_t launch_aim(hipStream_t s, st_element_f64* elems, int32_t n, const dou
 hipError_t launch_frames_batch(hipStream_t s, const st_element_f64* elems, int32_t n, const int32_t* elem_slot,
                                int32_t steps, const double* sun_units /* steps*3 */, const double* target,
                                double* ref_aabb /* steps*n*6 */, float* bounds_slot /* steps*n*6 */,
-                               float4* prims /* steps*n*4 */);
+                               float4* prims /* steps*n*4 */, double* frames /* steps*n*9 or null */);
 // refit inner nodes level by level (deepest first); nodes per step are
 // contiguous blocks of n_nodes*4 float4, bounds per step n*6 floats
 hipError_t launch_refit(hipStream_t s, const int32_t* child, const int32_t* level_nodes,
@@ -82,24 +89,34 @@ hipError_t launch_sun_plane(hipStream_t s, const double* ref_aabb, int32_t n, in
 hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk_prefix, int32_t n_jobs,
                         uint64_t total_chunks, uint32_t* work_counter, const LaunchShape& shape,
                         const JobDev& single);
-// sun-plane candidate grid: counts (zeroed, kGridRes^2 u32) -> start
-// (kGridRes^2 + 1), stats[0] = total items, stats[1] = longest cell list
-hipError_t launch_grid_count(hipStream_t s, const float* bounds, int32_t n, const SunFrame* sun, double tan_max,
-                             uint32_t* counts, uint32_t* start, uint32_t* stats);
-// cursor zeroed (kGridRes^2 u32); items sized stats[0]
-hipError_t launch_grid_fill(hipStream_t s, const float* bounds, int32_t n, const SunFrame* sun, double tan_max,
-                            const uint32_t* start, uint32_t* cursor, int32_t* items);
-// candidate lists of reflected rays (k_cone_lists). Task t: first
-// reflections off element t (pair_items null), or second reflections of rays
-// that left element pair_owner[t] and hit primitive pair_items[t]. Writes
-// counts[t] (kNoBounceList: none) and scratch[t * kMaxBounceList ...];
-// launch_list_compact then packs the lists at items[range[t].x ...].
+// sun-plane candidate grids of `steps` scene states (bounds per step n*6
+// floats, one SunFrame per step): counts (zeroed, steps * kGridCells u32) ->
+// start (steps * (kGridCells + 1)), stats[2k] = total items, stats[2k+1] =
+// longest cell list. With jobs, job k's grid pointers are set (items at
+// items + k * item_cap) when grid_lists_pay and the total fits item_cap, else
+// cleared.
+hipError_t launch_grid_count(hipStream_t s, const float* bounds, int32_t n, int32_t steps, const SunFrame* sun,
+                             double tan_max, uint32_t* counts, uint32_t* start, uint32_t* stats, JobDev* jobs,
+                             const int32_t* items, uint32_t item_cap);
+// cursor zeroed (steps * kGridCells u32); steps whose total exceeds item_cap are skipped
+hipError_t launch_grid_fill(hipStream_t s, const float* bounds, int32_t n, int32_t steps, const SunFrame* sun,
+                            double tan_max, const uint32_t* start, uint32_t* cursor, int32_t* items, uint32_t item_cap);
+// candidate lists of reflected rays (k_cone_lists). Task t belongs to scene
+// state k = t / tasks_per_step (frames n*9, bounds n*6, nodes n_nodes*4 and one
+// SunFrame per state) and is, with i = t % tasks_per_step: first reflections
+// off element i (pair_items null), or second reflections of rays that left
+// element pair_owner[t] and hit primitive pair_items[t]. The list goes to
+// scratch[t * kMaxBounceList ...]; counts[t] gets its length (kNoBounceList:
+// none) and, when range is given, range[t] = (t * kMaxBounceList, + length) or
+// (1, 0) for none, so the scratch serves as the item array directly.
+// launch_list_compact packs counted lists at items[range[t].x ...].
 constexpr uint32_t kNoBounceList = 0xFFFFFFFFu;
 constexpr uint32_t kMaxBounceList = 64;
-hipError_t launch_cone_lists(hipStream_t s, const st_element_f64* elems, int32_t tasks, const int32_t* elem_slot,
-                             const uint32_t* meta, const double* frames, const float* bounds_slot,
-                             const float4* nodes, const SunFrame* sun, double theta_sun, const int32_t* pair_items,
-                             const int32_t* pair_owner, uint32_t* counts, int32_t* scratch);
+hipError_t launch_cone_lists(hipStream_t s, const st_element_f64* elems, int32_t tasks, int32_t tasks_per_step,
+                             int32_t n, int32_t n_nodes, const int32_t* elem_slot, const uint32_t* meta,
+                             const double* frames, const float* bounds_slot, const float4* nodes, const SunFrame* sun,
+                             double theta_sun, const int32_t* pair_items, const int32_t* pair_owner, uint32_t* counts,
+                             uint2* range, int32_t* scratch);
 hipError_t launch_list_compact(hipStream_t s, int32_t tasks, const uint2* range, const int32_t* scratch,
                                int32_t* items);
 hipError_t launch_fill_u64(hipStream_t s, void* p, uint64_t n_u64, uint64_t value);

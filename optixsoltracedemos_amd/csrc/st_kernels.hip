@@ -751,7 +751,7 @@ __global__ void k_frames_batch(const st_element_f64* __restrict__ elems, int32_t
                                const int32_t* __restrict__ elem_slot, int32_t steps,
                                const double* __restrict__ sun_units, const double* __restrict__ target,
                                double* __restrict__ ref_aabb, float* __restrict__ bounds_slot,
-                               float4* __restrict__ prims) {
+                               float4* __restrict__ prims, double* __restrict__ frames) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (int64_t)n * steps) return;
     const int k = (int)(g / n), e = (int)(g % n);
@@ -763,6 +763,8 @@ __global__ void k_frames_batch(const st_element_f64* __restrict__ elems, int32_t
     }
     double l2g[9];
     element_frame(E, aim, E.zrot_deg, l2g);
+    if (frames)
+        for (int j = 0; j < 9; ++j) frames[((size_t)k * n + e) * 9 + j] = l2g[j];
     double lo[3], hi[3];
     element_ref_aabb(E, l2g, lo, hi);
     double* ra = ref_aabb + ((size_t)k * n + e) * 6;
@@ -968,23 +970,31 @@ __device__ __forceinline__ bool grid_range(const float* __restrict__ bounds, int
     return r[0] <= r[1] && r[2] <= r[3];
 }
 
-__global__ void k_grid_count(const float* __restrict__ bounds, int32_t n, const SunFrame* __restrict__ sun,
-                             double tan_max, uint32_t* __restrict__ counts) {
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= n) return;
+__global__ void k_grid_count(const float* __restrict__ bounds, int32_t n, int32_t steps,
+                             const SunFrame* __restrict__ sun, double tan_max, uint32_t* __restrict__ counts) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)n * steps) return;
+    const int k = (int)(g / n), slot = (int)(g % n);
     int r[4];
-    if (!grid_range(bounds, slot, *sun, tan_max, r)) return;
+    if (!grid_range(bounds + (size_t)k * n * 6, slot, sun[k], tan_max, r)) return;
+    uint32_t* C = counts + (size_t)k * kGridCells;
     for (int b = r[2]; b <= r[3]; ++b)
-        for (int a = r[0]; a <= r[1]; ++a) atomicAdd(&counts[b * kGridRes + a], 1u);
+        for (int a = r[0]; a <= r[1]; ++a) atomicAdd(&C[b * kGridRes + a], 1u);
 }
 
-// exclusive prefix sum of kGridRes^2 counts (one block of 1024); start[cells]
-// = total, stats[0] = total, stats[1] = longest list
-__global__ __launch_bounds__(1024) void k_grid_scan(const uint32_t* __restrict__ counts, uint32_t* __restrict__ start,
-                                                    uint32_t* __restrict__ stats) {
-    constexpr int kCells = kGridRes * kGridRes, kPer = kCells / 1024;
+// exclusive prefix sum of the kGridRes^2 counts of step blockIdx.x (one block
+// of 1024 per step); start[cells] = total, stats[0] = total, stats[1] = longest
+// list. With jobs: the step's job gets the grid when its lists fit item_cap and
+// are short enough to beat the BVH, else none (its sun rays traverse the BVH).
+__global__ __launch_bounds__(1024) void k_grid_scan(const uint32_t* __restrict__ counts_all,
+                                                    uint32_t* __restrict__ start_all, uint32_t* __restrict__ stats_all,
+                                                    JobDev* __restrict__ jobs, const int32_t* __restrict__ items_all,
+                                                    uint32_t item_cap) {
+    constexpr int kPer = kGridCells / 1024;
     __shared__ uint32_t part[1024], top[1024];
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, k = blockIdx.x;
+    const uint32_t* counts = counts_all + (size_t)k * kGridCells;
+    uint32_t* start = start_all + (size_t)k * (kGridCells + 1);
     uint32_t sum = 0, mx = 0;
     for (int i = 0; i < kPer; ++i) {
         const uint32_t c = counts[t * kPer + i];
@@ -1008,19 +1018,32 @@ __global__ __launch_bounds__(1024) void k_grid_scan(const uint32_t* __restrict__
         run += counts[t * kPer + i];
     }
     if (t == 1023) {
-        start[kCells] = part[1023];
-        stats[0] = part[1023];
-        stats[1] = top[1023];
+        const uint32_t total = part[1023], longest = top[1023];
+        start[kGridCells] = total;
+        stats_all[2 * k] = total;
+        stats_all[2 * k + 1] = longest;
+        if (jobs) {
+            const bool ok = grid_lists_pay(total, longest) && total <= item_cap;
+            jobs[k].grid_start = ok ? start : nullptr;
+            jobs[k].grid_items = ok ? items_all + (size_t)k * item_cap : nullptr;
+        }
     }
 }
 
-__global__ void k_grid_fill(const float* __restrict__ bounds, int32_t n, const SunFrame* __restrict__ sun,
-                            double tan_max, const uint32_t* __restrict__ start, uint32_t* __restrict__ cursor,
-                            int32_t* __restrict__ items) {
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= n) return;
+// items of step k at items_all + k * item_cap; a step whose lists overflow
+// item_cap is skipped (its job gets no grid)
+__global__ void k_grid_fill(const float* __restrict__ bounds, int32_t n, int32_t steps,
+                            const SunFrame* __restrict__ sun, double tan_max, const uint32_t* __restrict__ start_all,
+                            uint32_t* __restrict__ cursor_all, int32_t* __restrict__ items_all, uint32_t item_cap) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)n * steps) return;
+    const int k = (int)(g / n), slot = (int)(g % n);
+    const uint32_t* start = start_all + (size_t)k * (kGridCells + 1);
+    if (start[kGridCells] > item_cap) return;
     int r[4];
-    if (!grid_range(bounds, slot, *sun, tan_max, r)) return;
+    if (!grid_range(bounds + (size_t)k * n * 6, slot, sun[k], tan_max, r)) return;
+    uint32_t* cursor = cursor_all + (size_t)k * kGridCells;
+    int32_t* items = items_all + (size_t)k * item_cap;
     for (int b = r[2]; b <= r[3]; ++b)
         for (int a = r[0]; a <= r[1]; ++a) {
             const int c = b * kGridRes + a;
@@ -1039,7 +1062,7 @@ __global__ void k_grid_fill(const float* __restrict__ bounds, int32_t n, const S
 // (geometric in t) and tested against the BVH, so the list holds every
 // primitive the ray can hit. Elements whose cone is wide (cylinders, large
 // errors) or whose list exceeds kMaxBounceList get none.
-__device__ bool bounce_cone(const st_element_f64& E, double theta_sun, double* phi) {
+__device__ __forceinline__ bool bounce_cone(const st_element_f64& E, double theta_sun, double* phi) {
     if (E.role != ST_ROLE_MIRROR || E.surface == ST_SURFACE_CYLINDER) return false;
     double spread = 0.0;
     if (E.surface == ST_SURFACE_PARABOLIC) {
@@ -1062,10 +1085,10 @@ struct Cone {
 constexpr int kConePieces = 32;
 constexpr double kConeFirst = 0.25;
 
-__device__ bool cone_meets_box(const Cone& C, const double lo[3], const double hi[3]) {
+__device__ __forceinline__ bool cone_meets_box(const Cone& C, const double lo[3], const double hi[3]) {
     // the cone misses the box if it misses the box's bounding sphere: the
     // sphere spans asin(r / |c|) around its centre direction
-    double r2 = 0.0, d2 = 0.0, ac = 0.0, far2 = 0.0;
+    double r2 = 0.0, d2 = 0.0, ac = 0.0, far2 = 0.0, smin = 0.0, smax = 0.0;
     for (int k = 0; k < 3; ++k) {
         const double c = 0.5 * (lo[k] + hi[k]), h = 0.5 * (hi[k] - lo[k]);
         r2 += h * h;
@@ -1073,22 +1096,35 @@ __device__ bool cone_meets_box(const Cone& C, const double lo[3], const double h
         ac += C.a[k] * c;
         const double m = fmax(fabs(lo[k]), fabs(hi[k]));
         far2 += m * m;
+        smin += fmin(C.a[k] * lo[k], C.a[k] * hi[k]);
+        smax += fmax(C.a[k] * lo[k], C.a[k] * hi[k]);
     }
     if (d2 > r2) {
         const double d = sqrt(d2), sb = sqrt(r2) / d, cb = sqrt(fmax(0.0, 1.0 - sb * sb));
         if (ac / d < C.cp * cb - C.sp * sb - 1e-9) return false;
     }
+    // piece [t0, t1] holds the cone points whose coordinate along the axis is
+    // in [t0, t1]; the box's points have theirs in [smin, smax] (widened for
+    // rounding), so only the pieces overlapping that range are tested
+    const double slack = 1e-9 * (fabs(smin) + fabs(smax) + 1.0);
+    smin -= slack;
+    smax += slack;
+    if (smax < 0.0) return false;
     const double T = sqrt(far2);
     double t0 = 0.0, t1 = fmin(T, kConeFirst);
     for (int i = 0; i < kConePieces; ++i) {
-        const double r0 = t0 * C.tp, r1 = t1 * C.tp;
-        bool meet = true;
-        for (int k = 0; k < 3 && meet; ++k) {
-            const double c0 = C.a[k] * t0, c1 = C.a[k] * t1;
-            const double slo = fmin(c0 - r0, c1 - r1), shi = fmax(c0 + r0, c1 + r1);
-            meet = !(shi < lo[k] || slo > hi[k]);
+        if (t0 > smax) break;
+        if (t1 >= smin) {
+            const double r0 = t0 * C.tp, r1 = t1 * C.tp;
+            // the piece's box overlaps the box along one axis (constant
+            // indices keep the cone in registers)
+            auto overlap = [&](double a, double blo, double bhi) __attribute__((always_inline)) {
+                const double c0 = a * t0, c1 = a * t1;
+                return !(fmax(c0 + r0, c1 + r1) < blo || fmin(c0 - r0, c1 - r1) > bhi);
+            };
+            if (overlap(C.a[0], lo[0], hi[0]) && overlap(C.a[1], lo[1], hi[1]) && overlap(C.a[2], lo[2], hi[2]))
+                return true;
         }
-        if (meet) return true;
         if (t1 >= T) break;
         t0 = t1;
         t1 = i == kConePieces - 2 ? T : fmin(T, t1 * C.ratio);
@@ -1098,46 +1134,51 @@ __device__ bool cone_meets_box(const Cone& C, const double lo[3], const double h
 
 // Primitives whose boxes a ray leaving box hb in a direction within the cone C
 // can reach, found by a BVH walk; the first `cap` are written to items[0 ...].
-// Returns the count, or cap + 1 as soon as it exceeds cap.
-__device__ uint32_t cone_walk(const float4* __restrict__ nodes, const float* __restrict__ bounds_slot,
-                              const float* hb, const Cone& C, int32_t* __restrict__ items, uint32_t cap) {
-    auto meets = [&](const float lo[3], const float hi[3]) {
+// Returns the count, or cap + 1 as soon as it exceeds cap. stack: this lane's
+// column of an LDS array, entry i at stack[i * STRIDE].
+template <int STRIDE>
+__device__ __forceinline__ uint32_t cone_walk(const float4* __restrict__ nodes, const float* __restrict__ bounds_slot,
+                                              const float* hb, const Cone& C, int32_t* __restrict__ items,
+                                              uint32_t cap, int32_t* __restrict__ stack) {
+    auto meets = [&](const float lo[3], const float hi[3]) __attribute__((always_inline)) {
         const double dlo[3] = {(double)lo[0] - hb[3], (double)lo[1] - hb[4], (double)lo[2] - hb[5]};
         const double dhi[3] = {(double)hi[0] - hb[0], (double)hi[1] - hb[1], (double)hi[2] - hb[2]};
         return cone_meets_box(C, dlo, dhi);
     };
-    uint32_t count = 0;
-    int32_t stack[kStackDepth + 1];  // depth <= 31 levels, one pending sibling each
+    uint32_t count = 0;  // the stack holds <= one pending sibling per level (depth <= 31)
     int sp = 0;
     int32_t node = 0;
     for (;;) {
         const float4 n0 = nodes[4 * node + 0], n1 = nodes[4 * node + 1], n2 = nodes[4 * node + 2], n3 = nodes[4 * node + 3];
         const float lo0[3] = {n0.x, n0.z, n2.x}, hi0[3] = {n0.y, n0.w, n2.y};
         const float lo1[3] = {n1.x, n1.z, n2.z}, hi1[3] = {n1.y, n1.w, n2.w};
-        const int32_t ch[2] = {__float_as_int(n3.x), __float_as_int(n3.y)};
-        const bool hit[2] = {meets(lo0, hi0), meets(lo1, hi1)};
-        for (int c = 0; c < 2; ++c) {
-            if (!hit[c] || (c == 1 && ch[1] == ch[0])) continue;  // a lone primitive sits under both children
-            if (ch[c] >= 0) {
-                stack[sp++] = ch[c];
-            } else {
-                const int32_t st = leaf_start(ch[c]), cnt = leaf_count(ch[c]);
-                for (int i = 0; i < cnt; ++i) {
-                    const float* b = bounds_slot + 6 * (size_t)(st + i);
-                    if (!meets(b, b + 3)) continue;
-                    if (count == cap) return cap + 1;
-                    items[count++] = st + i;
-                }
+        const int32_t ch0 = __float_as_int(n3.x), ch1 = __float_as_int(n3.y);
+        const bool hit0 = meets(lo0, hi0), hit1 = ch1 != ch0 && meets(lo1, hi1);  // a lone primitive sits under both
+        // an inner child is pushed; a leaf child's primitives are tested now
+        auto visit = [&](int32_t ch) __attribute__((always_inline)) -> bool {
+            if (ch >= 0) {
+                stack[(sp++) * STRIDE] = ch;
+                return true;
             }
-        }
+            const int32_t st = leaf_start(ch), cnt = leaf_count(ch);
+            for (int i = 0; i < cnt; ++i) {
+                const float* b = bounds_slot + 6 * (size_t)(st + i);
+                if (!meets(b, b + 3)) continue;
+                if (count == cap) return false;
+                items[count++] = st + i;
+            }
+            return true;
+        };
+        if (hit0 && !visit(ch0)) return cap + 1;
+        if (hit1 && !visit(ch1)) return cap + 1;
         if (sp == 0) break;
-        node = stack[--sp];
+        node = stack[(--sp) * STRIDE];
     }
     return count;
 }
 
 // reflection axis reflect(-sun, n_e) of a sun ray off element e (unit)
-__device__ void sun_reflection_axis(const double* __restrict__ frames, int e, const double sun[3], double a[3]) {
+__device__ __forceinline__ void sun_reflection_axis(const double* __restrict__ frames, int e, const double sun[3], double a[3]) {
     const double nz[3] = {frames[9 * (size_t)e + 2], frames[9 * (size_t)e + 5], frames[9 * (size_t)e + 8]};
     const double din[3] = {-sun[0], -sun[1], -sun[2]};
     const double dn = din[0] * nz[0] + din[1] * nz[1] + din[2] * nz[2];
@@ -1146,32 +1187,47 @@ __device__ void sun_reflection_axis(const double* __restrict__ frames, int e, co
     for (int k = 0; k < 3; ++k) a[k] /= la;
 }
 
-// Task t: first reflections off element t (pair_items == null), or second
-// reflections of rays that left element pair_owner[t] and hit the primitive
-// pair_items[t]. Writes counts[t] (kNoBounceList: none or longer than
-// kMaxBounceList) and the list to scratch[t * kMaxBounceList ...].
-__global__ void k_cone_lists(const st_element_f64* __restrict__ elems, int32_t tasks,
-                             const int32_t* __restrict__ elem_slot, const uint32_t* __restrict__ meta,
-                             const double* __restrict__ frames, const float* __restrict__ bounds_slot,
-                             const float4* __restrict__ nodes, const SunFrame* __restrict__ sun, double theta_sun,
+// Task t of scene state k = t / tasks_per_step, i = t % tasks_per_step: first
+// reflections off element i (pair_items == null), or second reflections of
+// rays that left element pair_owner[t] and hit the primitive pair_items[t].
+// The list goes to scratch[t * kMaxBounceList ...]; counts[t] (if given) gets
+// its length, kNoBounceList for none or longer than kMaxBounceList, and
+// range[t] (if given) its (begin, end) in the scratch, (1, 0) for none.
+constexpr int kConeBlock = 64;
+__global__ __launch_bounds__(kConeBlock) void k_cone_lists(const st_element_f64* __restrict__ elems, int32_t tasks,
+                                                           int32_t tasks_per_step,
+                             int32_t n, int32_t n_nodes, const int32_t* __restrict__ elem_slot,
+                             const uint32_t* __restrict__ meta, const double* __restrict__ frames_all,
+                             const float* __restrict__ bounds_all, const float4* __restrict__ nodes_all,
+                             const SunFrame* __restrict__ sun_all, double theta_sun,
                              const int32_t* __restrict__ pair_items, const int32_t* __restrict__ pair_owner,
-                             uint32_t* __restrict__ counts, int32_t* __restrict__ scratch) {
+                             uint32_t* __restrict__ counts, uint2* __restrict__ range, int32_t* __restrict__ scratch) {
+    __shared__ int32_t lds_stack[(kStackDepth + 1) * kConeBlock];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= tasks) return;
+    const int k = t / tasks_per_step;
+    const double* frames = frames_all + (size_t)k * n * 9;
+    const float* bounds_slot = bounds_all + (size_t)k * n * 6;
+    const float4* nodes = nodes_all + (size_t)k * n_nodes * 4;
+    const SunFrame* sun = sun_all + k;
+    auto none = [&]() {
+        if (counts) counts[t] = kNoBounceList;
+        if (range) range[t] = make_uint2(1u, 0u);
+    };
     double a[3], phi = 0.0;
-    int from = t;  // element the rays leave from
+    int from = t - k * tasks_per_step;  // element the rays leave from
     if (!pair_items) {
-        if (!bounce_cone(elems[t], theta_sun, &phi)) {
-            counts[t] = kNoBounceList;
+        if (!bounce_cone(elems[from], theta_sun, &phi)) {
+            none();
             return;
         }
-        sun_reflection_axis(frames, t, sun->sun, a);
+        sun_reflection_axis(frames, from, sun->sun, a);
     } else {
         const int ea = pair_owner[t];
         from = (int)meta_elem(meta[pair_items[t]]);
         double phi_a = 0.0, phi_b = 0.0;
         if (!bounce_cone(elems[ea], theta_sun, &phi_a) || !bounce_cone(elems[from], 0.0, &phi_b)) {
-            counts[t] = kNoBounceList;
+            none();
             return;
         }
         // incidence within phi_a of a_A; reflection off the second mirror adds
@@ -1180,17 +1236,17 @@ __global__ void k_cone_lists(const st_element_f64* __restrict__ elems, int32_t t
         sun_reflection_axis(frames, ea, sun->sun, aa);
         const double nb[3] = {frames[9 * (size_t)from + 2], frames[9 * (size_t)from + 5], frames[9 * (size_t)from + 8]};
         const double dn = aa[0] * nb[0] + aa[1] * nb[1] + aa[2] * nb[2];
-        for (int k = 0; k < 3; ++k) a[k] = aa[k] - 2.0 * dn * nb[k];
+        for (int j = 0; j < 3; ++j) a[j] = aa[j] - 2.0 * dn * nb[j];
         const double la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
-        for (int k = 0; k < 3; ++k) a[k] /= la;
+        for (int j = 0; j < 3; ++j) a[j] /= la;
         phi = phi_a + phi_b;
         if (phi >= 0.35) {
-            counts[t] = kNoBounceList;
+            none();
             return;
         }
     }
     Cone C;
-    for (int k = 0; k < 3; ++k) C.a[k] = a[k];
+    for (int j = 0; j < 3; ++j) C.a[j] = a[j];
     C.tp = tan(phi);
     C.cp = cos(phi);
     C.sp = sin(phi);
@@ -1200,9 +1256,15 @@ __global__ void k_cone_lists(const st_element_f64* __restrict__ elems, int32_t t
     const double ez = fmax(r2.y, r2.w) - fmin(r2.x, r2.z);
     const double reach = 2.0 * sqrt(ex * ex + ey * ey + ez * ez) + 2.0 * kConeFirst;
     C.ratio = pow(reach / kConeFirst, 1.0 / (kConePieces - 2));
-    const uint32_t count = cone_walk(nodes, bounds_slot, bounds_slot + 6 * (size_t)elem_slot[from], C,
-                                     scratch + (size_t)t * kMaxBounceList, kMaxBounceList);
-    counts[t] = count > kMaxBounceList ? kNoBounceList : count;
+    const uint32_t count = cone_walk<kConeBlock>(nodes, bounds_slot, bounds_slot + 6 * (size_t)elem_slot[from], C,
+                                                 scratch + (size_t)t * kMaxBounceList, kMaxBounceList,
+                                                 lds_stack + threadIdx.x);
+    if (count > kMaxBounceList) {
+        none();
+        return;
+    }
+    if (counts) counts[t] = count;
+    if (range) range[t] = make_uint2((uint32_t)t * kMaxBounceList, (uint32_t)t * kMaxBounceList + count);
 }
 
 // items[range[t].x + k] = scratch[t * kMaxBounceList + k] for every listed task
@@ -1251,10 +1313,10 @@ hipError_t launch_aim(hipStream_t s, st_element_f64* elems, int32_t n, const dou
 
 hipError_t launch_frames_batch(hipStream_t s, const st_element_f64* elems, int32_t n, const int32_t* elem_slot,
                                int32_t steps, const double* sun_units, const double* target, double* ref_aabb,
-                               float* bounds_slot, float4* prims) {
+                               float* bounds_slot, float4* prims, double* frames) {
     const int64_t total = (int64_t)n * steps;
     hipLaunchKernelGGL(k_frames_batch, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, elems, n, elem_slot,
-                       steps, sun_units, target, ref_aabb, bounds_slot, prims);
+                       steps, sun_units, target, ref_aabb, bounds_slot, prims, frames);
     return hipGetLastError();
 }
 
@@ -1337,29 +1399,37 @@ hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_grid_count(hipStream_t s, const float* bounds, int32_t n, const SunFrame* sun, double tan_max,
-                             uint32_t* counts, uint32_t* start, uint32_t* stats) {
-    hipLaunchKernelGGL(k_grid_count, dim3((n + 127) / 128), dim3(128), 0, s, bounds, n, sun, tan_max, counts);
+hipError_t launch_grid_count(hipStream_t s, const float* bounds, int32_t n, int32_t steps, const SunFrame* sun,
+                             double tan_max, uint32_t* counts, uint32_t* start, uint32_t* stats, JobDev* jobs,
+                             const int32_t* items, uint32_t item_cap) {
+    const int64_t total = (int64_t)n * steps;
+    hipLaunchKernelGGL(k_grid_count, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, bounds, n, steps, sun,
+                       tan_max, counts);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_grid_scan, dim3(1), dim3(1024), 0, s, counts, start, stats);
+    hipLaunchKernelGGL(k_grid_scan, dim3(steps), dim3(1024), 0, s, counts, start, stats, jobs, items, item_cap);
     return hipGetLastError();
 }
 
-hipError_t launch_grid_fill(hipStream_t s, const float* bounds, int32_t n, const SunFrame* sun, double tan_max,
-                            const uint32_t* start, uint32_t* cursor, int32_t* items) {
-    hipLaunchKernelGGL(k_grid_fill, dim3((n + 127) / 128), dim3(128), 0, s, bounds, n, sun, tan_max, start, cursor,
-                       items);
+hipError_t launch_grid_fill(hipStream_t s, const float* bounds, int32_t n, int32_t steps, const SunFrame* sun,
+                            double tan_max, const uint32_t* start, uint32_t* cursor, int32_t* items, uint32_t item_cap) {
+    const int64_t total = (int64_t)n * steps;
+    hipLaunchKernelGGL(k_grid_fill, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, bounds, n, steps, sun,
+                       tan_max, start, cursor, items, item_cap);
     return hipGetLastError();
 }
 
-hipError_t launch_cone_lists(hipStream_t s, const st_element_f64* elems, int32_t tasks, const int32_t* elem_slot,
-                             const uint32_t* meta, const double* frames, const float* bounds_slot,
-                             const float4* nodes, const SunFrame* sun, double theta_sun, const int32_t* pair_items,
-                             const int32_t* pair_owner, uint32_t* counts, int32_t* scratch) {
+hipError_t launch_cone_lists(hipStream_t s, const st_element_f64* elems, int32_t tasks, int32_t tasks_per_step,
+                             int32_t n, int32_t n_nodes, const int32_t* elem_slot, const uint32_t* meta,
+                             const double* frames, const float* bounds_slot, const float4* nodes, const SunFrame* sun,
+                             double theta_sun, const int32_t* pair_items, const int32_t* pair_owner, uint32_t* counts,
+                             uint2* range, int32_t* scratch) {
     if (tasks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cone_lists, dim3((tasks + 63) / 64), dim3(64), 0, s, elems, tasks, elem_slot, meta, frames,
-                       bounds_slot, nodes, sun, theta_sun, pair_items, pair_owner, counts, scratch);
+    if (tasks_per_step <= 0 || tasks % tasks_per_step != 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cone_lists, dim3((tasks + kConeBlock - 1) / kConeBlock), dim3(kConeBlock), 0, s, elems, tasks,
+                       tasks_per_step, n,
+                       n_nodes, elem_slot, meta, frames, bounds_slot, nodes, sun, theta_sun, pair_items, pair_owner,
+                       counts, range, scratch);
     return hipGetLastError();
 }
 

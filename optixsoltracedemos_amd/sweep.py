@@ -20,10 +20,12 @@ import numpy as np
 from . import _abi, scenes, tracer
 
 
-def run_batched(ctx, w, suns, budgets, seed=0, flux=(256, 128), use_graph=True, reps=2, position_sampling=_abi.POS_STRATIFIED):
+def run_batched(ctx, w, suns, budgets, seed=0, flux=(256, 128), use_graph=True, reps=2, position_sampling=_abi.POS_STRATIFIED,
+                bvh_only=False):
     import torch
     sc = tracer.Scene(ctx, w.elements)
-    kw = dict(seed=seed, max_depth=w.max_depth, flux_nx=flux[0], flux_ny=flux[1], use_graph=use_graph)
+    kw = dict(seed=seed, max_depth=w.max_depth, flux_nx=flux[0], flux_ny=flux[1], use_graph=use_graph,
+              bvh_only=bvh_only)
     sc.set_sun(suns[0], w.sun_param, position_sampling=position_sampling)
     out = sc.trace_batch(suns, budgets, w.extra["target"], w.sun_param, position_sampling=position_sampling, **kw)
     torch.cuda.synchronize()                                   # first call: set-up + graph capture
@@ -54,9 +56,11 @@ def compare(ctx, steps=96, seed=0):
     suns, budgets = scenes.annual_sweep(steps, seed=seed)
     total = int(budgets.sum())
     tb, out = run_batched(ctx, w, suns, budgets, seed=seed)
-    tp, plain = run_plain_loop(ctx, w, suns, budgets, seed=seed)
     fb = out["flux"].cpu().numpy()
-    identical = all(np.array_equal(fb[k], plain[k]) for k in range(steps))
-    return {"steps": steps, "total_rays": total, "batched_s": tb, "plain_loop_s": tp,
+    tw, out_bvh = run_batched(ctx, w, suns, budgets, seed=seed, bvh_only=True)
+    fw = out_bvh["flux"].cpu().numpy()
+    tp, plain = run_plain_loop(ctx, w, suns, budgets, seed=seed)
+    identical = all(np.array_equal(fb[k], plain[k]) and np.array_equal(fw[k], plain[k]) for k in range(steps))
+    return {"steps": steps, "total_rays": total, "batched_s": tb, "batched_bvh_only_s": tw, "plain_loop_s": tp,
             "batched_rays_per_s": total / tb, "plain_rays_per_s": total / tp, "speedup": tp / tb,
             "identical_flux": bool(identical)}

@@ -140,7 +140,8 @@ class Scene:
         d.num_rays, d.ray_begin, d.ray_count, d.seed = int(num_rays), int(ray_begin), int(ray_count), int(seed)
         d.job_id, d.max_depth = int(job_id), int(max_depth)
         d.flags = ((_abi.TRACE_RECORDS if records else 0) | (_abi.TRACE_FLUX_DIRECT_SUN if include_direct_sun else 0)
-                   | (_abi.TRACE_ZERO_OUTPUTS if zero else 0) | (_abi.TRACE_BVH_ONLY if bvh_only else 0)
+                   | (_abi.TRACE_ZERO_OUTPUTS if zero else 0) | (_abi.TRACE_BVH_ONLY if bvh_only else 0) \
+            | (_abi.TRACE_COUNT_WORK if count_work else 0)
                    | (_abi.TRACE_COUNT_WORK if count_work else 0))
         d.kernel, d.flux_element, d.flux_nx, d.flux_ny = int(kernel), int(flux_element), int(flux_nx), int(flux_ny)
         d.block_size, d.grid_blocks = int(block_size), int(grid_blocks)
@@ -190,7 +191,8 @@ class Scene:
 
     def trace_batch(self, sun_vectors: np.ndarray, budgets: np.ndarray, target, sun_param, sun_shape=_abi.SUN_PILLBOX,
                     seed=0, max_depth=5, flux_nx=0, flux_ny=0, flux_element=-1, use_graph=True, dni=1000.0,
-                    include_direct_sun=False, out=None, stream=None, position_sampling=_abi.POS_PHILOX):
+                    include_direct_sun=False, out=None, stream=None, position_sampling=_abi.POS_PHILOX,
+                    bvh_only=False, count_work=False):
         torch = _torch()
         K = len(budgets)
         suns = np.ascontiguousarray(sun_vectors, dtype=np.float64).reshape(K, 3)
@@ -204,7 +206,8 @@ class Scene:
         d.sun = _abi.sun_desc([0, 0, 1], sun_param, sun_shape, dni, position_sampling)
         d.seed = int(seed)
         d.flux_element, d.flux_nx, d.flux_ny = int(flux_element), int(flux_nx), int(flux_ny)
-        d.flags = _abi.TRACE_FLUX_DIRECT_SUN if include_direct_sun else 0
+        d.flags = (_abi.TRACE_FLUX_DIRECT_SUN if include_direct_sun else 0) | (_abi.TRACE_BVH_ONLY if bvh_only else 0) \
+            | (_abi.TRACE_COUNT_WORK if count_work else 0)
         d.use_graph = 1 if use_graph else 0
         d.stream = stream
         dev = f"cuda:{self.ctx.device}"

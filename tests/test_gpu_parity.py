@@ -334,6 +334,34 @@ def test_batch_equals_individual_steps(gpu_ctx):
         assert np.array_equal(again["flux"].cpu().numpy(), flux)
 
 
+@pytest.mark.parametrize("sampling", [_abi.POS_STRATIFIED, _abi.POS_PHILOX])
+def test_batch_lists_equal_bvh(gpu_ctx, sampling):
+    """Per-step sun-plane grids and first-reflection lists built inside the
+    batch give the BVH's results bitwise and replace most node visits."""
+    w = scenes.heliostat_field(10_000)
+    suns, _ = scenes.annual_sweep(6, seed=5)
+    suns[0] = scenes.sun_vector(4.0, 100.0)          # grazing sun: long grid lists
+    budgets = np.array([400_000, 1_500_000, 64, 900_001, 2_000_000, 333_333], dtype=np.uint64)
+    tgt = w.extra["target"]
+    kw = dict(seed=2, max_depth=5, flux_nx=64, flux_ny=32, position_sampling=sampling, count_work=True)
+    runs = {}
+    for bvh_only in (False, True):
+        sc = tracer.Scene(gpu_ctx, w.elements)
+        for graph in (False, True):
+            out = sc.trace_batch(suns, budgets, tgt, w.sun_param, use_graph=graph, bvh_only=bvh_only, **kw)
+            runs[(bvh_only, graph)] = {k: v.cpu().numpy() for k, v in out.items()}
+        sc.close()
+    ref = runs[(True, False)]
+    for key, r in runs.items():
+        assert np.array_equal(r["flux"], ref["flux"]), key
+        assert np.array_equal(r["element_hits"], ref["element_hits"]), key
+        assert np.array_equal(r["stats"][:, :11], ref["stats"][:, :11]), key
+    listed = runs[(False, True)]["stats"][:, 11].sum()
+    walked = ref["stats"][:, 11].sum()
+    print(f"node visits per batch: lists {listed}, BVH {walked}")
+    assert listed < 0.25 * walked
+
+
 def test_invalid_trace_requests_are_rejected(gpu_ctx):
     w = scenes.three_heliostats()
     sc = tracer.Scene(gpu_ctx, w.elements)
