@@ -77,15 +77,62 @@ __device__ __forceinline__ void cone_basis(V3 w, V3& u, V3& v) {
     v = cross(w, u);
 }
 
+// sin and cos of 2 pi x, x in [0, 1): the hardware sine and cosine take their
+// argument in revolutions (error ~1e-6 absolute, which moves a direction
+// sampled in a cone of half angle theta by ~1e-6 theta)
+__device__ __forceinline__ void sincos_2pi(float x, float& s, float& c) {
+    s = __builtin_amdgcn_sinf(x);
+    c = __builtin_amdgcn_cosf(x);
+}
+
+// sin and cos of an angle that is small in practice (cone and error angles of
+// a few mrad): Taylor series through x^9 / x^10 for |x| < 0.35, whose
+// truncation error (< 1e-9 relative) is below FP32 rounding; sincosf beyond
+__device__ __forceinline__ void sincos_small(float x, float& s, float& c) {
+    if (fabsf(x) < 0.35f) {
+        const float x2 = x * x;
+        s = fmaf(x * x2, fmaf(x2, fmaf(x2, fmaf(x2, 2.7557319e-6f, -1.9841270e-4f), 8.3333333e-3f), -0.16666667f), x);
+        c = fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, 2.4801587e-5f, -1.3888889e-3f), 4.1666667e-2f), -0.5f), 1.f);
+    } else {
+        sincosf(x, &s, &c);
+    }
+}
+
+// sqrt(-2 ln u) of a uniform u in (0, 1] (Box-Muller radius), from the
+// hardware log2 and square root
+__device__ __forceinline__ float box_muller_radius(float u) {
+    return __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u));
+}
+
 // isotropic 2-D Gaussian deviation (polar Box-Muller): theta = sigma sqrt(-2 ln xr)
 __device__ __forceinline__ V3 perturb(V3 a, float sigma, float xr, float xp) {
     V3 u, v;
     cone_basis(a, u, v);
-    float th = sigma * sqrtf(-2.f * logf(xr));
     float st, ct, sp, cp;
-    sincosf(th, &st, &ct);
-    sincospif(2.f * xp, &sp, &cp);
+    sincos_small(sigma * box_muller_radius(xr), st, ct);
+    sincos_2pi(xp, sp, cp);
     return normalize(fma3(ct, a, st * fma3(cp, u, sp * v)));
+}
+
+// atan2(y, x) in (-pi, pi]: octant reduction and a degree-15 odd polynomial
+// fitted to atan on [0, 1] (error ~1.3e-7 rad in FP32, at the level of the
+// rounding of the result); atan2(0, 0) = 0
+__device__ __forceinline__ float fast_atan2(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float a = mx > 0.f ? mn * __builtin_amdgcn_rcpf(mx) : 0.f;
+    const float s = a * a;
+    float r = fmaf(s, -0.0039626276f, 0.021518581f);
+    r = fmaf(s, r, -0.055396538f);
+    r = fmaf(s, r, 0.096028775f);
+    r = fmaf(s, r, -0.13892591f);
+    r = fmaf(s, r, 0.19943213f);
+    r = fmaf(s, r, -0.33329552f);
+    r = fmaf(s, r, 0.99999923f);
+    r *= a;
+    if (ay > ax) r = 1.5707963267948966f - r;
+    if (x < 0.f) r = 3.141592653589793f - r;
+    return copysignf(r, y);
 }
 
 __device__ __forceinline__ float halton2(uint32_t i) {
@@ -152,14 +199,14 @@ __device__ __forceinline__ bool isect_parabolic(const float4& q0, const float4& 
     float t = 0.f;
     bool valid = false;
     if (fabsf(A) < 1e-12f) {
-        t = -C / B;
+        t = -C * __builtin_amdgcn_rcpf(B);
         valid = t > 0.f;
     } else {
         float disc = B * B - 4.f * A * C;
         if (disc >= 0.f) {
             float sq = sqrtf(disc);
             float qq = B >= 0.f ? -0.5f * (B + sq) : -0.5f * (B - sq);
-            float ra = qq / A, rb = qq != 0.f ? C / qq : ra;
+            float ra = qq * __builtin_amdgcn_rcpf(A), rb = qq != 0.f ? C * __builtin_amdgcn_rcpf(qq) : ra;
             float t1 = B >= 0.f ? ra : rb;
             float t2 = B >= 0.f ? rb : ra;
             if (t1 > 0.f && t1 < t2) {
@@ -191,11 +238,12 @@ __device__ __forceinline__ bool isect_cylinder(const float4& q0, const float4& q
     float t_wall = tbig;
     float A = dx * dx + dz * dz;
     if (A > 0.f) {
-        float tc = -(ox * dx + oz * dz) / A;
+        const float rA = __builtin_amdgcn_rcpf(A);
+        float tc = -(ox * dx + oz * dz) * rA;
         float px = fmaf(tc, dx, ox), pz = fmaf(tc, dz, oz);
         float h2 = rad * rad - (px * px + pz * pz);
         if (h2 >= 0.f) {
-            float dt = sqrtf(h2 / A);
+            float dt = sqrtf(h2 * rA);
             float t1 = tc - dt, t2 = tc + dt;
             if (t1 > tmin && t1 < tmax && fabsf(fmaf(t1, dy, oy)) <= hh) t_wall = t1;
             else if (t2 > tmin && t2 < tmax && fabsf(fmaf(t2, dy, oy)) <= hh) t_wall = t2;
@@ -203,7 +251,7 @@ __device__ __forceinline__ bool isect_cylinder(const float4& q0, const float4& q
     }
     float t_cap = tbig;
     if (fabsf(dy) > 1e-6f) {
-        float inv = 1.f / dy;
+        float inv = __builtin_amdgcn_rcpf(dy);
         float tb = (-hh - oy) * inv, tt = (hh - oy) * inv;
         float bx = fmaf(tb, dx, ox), bz = fmaf(tb, dz, oz);
         if (tb > tmin && tb < tmax && bx * bx + bz * bz <= rad * rad) t_cap = tb;
@@ -234,9 +282,12 @@ __device__ __forceinline__ float next_up(float t) { return __int_as_float(__floa
 // Event counters: ST_STAT_* 0..14, [15] = hits on the receiver kept aside
 // (agg_slot). Each wave owns kNumCounters u32 in LDS; every lane that reaches
 // a counting site contributes one and the wave adds the number of such lanes
-// with a single LDS atomic, so the counters cost no registers during traversal.
+// with a single LDS atomic. The vote and the atomic are scalar and LDS work,
+// which issue beside the vector work of other waves; counters kept per lane
+// in vector registers measured slower.
 constexpr int kNumCounters = 16;
 constexpr int kAggCounter = 15;
+constexpr uint32_t kMaxChunkRays = 2048;
 
 __device__ __forceinline__ void count_if(uint32_t* wc, int k, bool pred) {
     const unsigned long long m = __ballot(pred);
@@ -418,13 +469,13 @@ __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame
     {
         const V3 w = v3(S.w[0], S.w[1], S.w[2]), su = v3(S.u[0], S.u[1], S.u[2]), sv = v3(S.v[0], S.v[1], S.v[2]);
         float sp, cp, st, ct;
-        sincospif(2.f * u01(u.z), &sp, &cp);
+        sincos_2pi(u01(u.z), sp, cp);
         if (S.shape == ST_SUN_GAUSSIAN) {
-            sincosf(S.sigma * sqrtf(-2.f * logf(u01(u.w))), &st, &ct);
+            sincos_small(S.sigma * box_muller_radius(u01(u.w)), st, ct);
         } else {
             const float s = S.vers * (1.f - u01(u.w));
             ct = 1.f - s;
-            st = sqrtf(s * (2.f - s));
+            st = __builtin_amdgcn_sqrtf(s * (2.f - s));
         }
         d = normalize(fma3(ct, w, st * fma3(cp, su, sp * sv)));
     }
@@ -479,16 +530,17 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
                 const float lx = fmaf(t, dot(d, v3(q1)), dot(r, v3(q1)));
                 const float ly = fmaf(t, dot(d, v3(q2)), dot(r, v3(q2)));
                 float fx, fy;
+                // bins per unit length of the receiver's local axes
+                const float by = (float)J.flux_ny * 0.5f * __builtin_amdgcn_rcpf(q1.w);
                 if (kind == kKindCylinder) {
                     const float lz = fmaf(t, dot(d, v3(q3)), dot(r, v3(q3)));
-                    float th = atan2f(-lz, lx);
+                    float th = fast_atan2(-lz, lx);
                     if (th < 0.f) th += 6.283185307179586f;
                     fx = th * (float)J.flux_nx * 0.15915494309189535f;
-                    fy = (ly + q1.w) * (float)J.flux_ny / (2.f * q1.w);
                 } else {
-                    fx = (lx + q0.w) * (float)J.flux_nx / (2.f * q0.w);
-                    fy = (ly + q1.w) * (float)J.flux_ny / (2.f * q1.w);
+                    fx = (lx + q0.w) * ((float)J.flux_nx * 0.5f * __builtin_amdgcn_rcpf(q0.w));
                 }
+                fy = (ly + q1.w) * by;
                 int ix = (int)floorf(fx), iy = (int)floorf(fy);
                 ix = min(max(ix, 0), J.flux_nx - 1);
                 iy = min(max(iy, 0), J.flux_ny - 1);
@@ -594,7 +646,6 @@ __device__ __forceinline__ uint32_t queue_push(float* q, uint32_t qn, bool cont,
 // << 15, key = the list key of its next segment (0: none, or a key >= 2^17 - 1,
 // whose segment is then traced through the BVH). A parked ray's last record
 // slot is its depth.
-constexpr uint32_t kMaxChunkRays = 2048;
 __device__ __forceinline__ uint32_t queue_tag(uint64_t offset, int depth, int key) {
     const uint32_t f = (uint32_t)(key + 1) < (1u << 17) ? (uint32_t)(key + 1) : 0u;
     return (uint32_t)offset | ((uint32_t)depth << 11) | (f << 15);
