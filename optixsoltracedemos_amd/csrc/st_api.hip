@@ -370,8 +370,9 @@ int scene_bounce_lists(st_scene s) {
     s->bounce_valid = true;
     // second reflections: one list per first-reflection list entry
     s->h_pair_owner.assign(s->bounce_items, 0);
-    for (int32_t e = 0; e < s->n; ++e)
-        for (uint32_t j = s->h_bounce_range[e].x; j < s->h_bounce_range[e].y; ++j) s->h_pair_owner[j] = e;
+    for (int32_t e = 0; e < s->n; ++e)  // a list's first entry is its header: no second-reflection task
+        for (uint32_t j = s->h_bounce_range[e].x; j < s->h_bounce_range[e].y; ++j)
+            s->h_pair_owner[j] = j == s->h_bounce_range[e].x ? -1 : e;
     ST_TRY(s->d_pair_owner.ensure(sizeof(int32_t) * s->bounce_items, "pair owners"));
     HIP_TRY(hipMemcpyAsync(s->d_pair_owner.ptr, s->h_pair_owner.data(), sizeof(int32_t) * s->bounce_items,
                            hipMemcpyHostToDevice, st),

@@ -316,16 +316,47 @@ __device__ __forceinline__ void test_prim(const float4* __restrict__ prims, cons
     }
 }
 
-// Closest hit over a candidate list (items[b, e)): a sun ray's sun-plane cell
-// list, a first reflection's list of its mirror or a second reflection's list
-// of its (first, second) mirror pair. Every primitive such a ray can reach is
-// listed (conservative binning, k_grid_* / k_cone_lists), so the result equals
-// a full BVH traversal. bk = list index of the winner.
+// A flat candidate of the closest-hit search: test_prim without the kind
+// dispatch (the primitive is known to be flat).
+__device__ __forceinline__ void test_flat(const float4* __restrict__ prims, const uint32_t* __restrict__ meta, int slot,
+                                          V3 o, V3 d, float tmin, float& tbest, int& bslot, int& bcap, uint32_t* wc) {
+    if (wc) count(wc, ST_STAT_FLAT_TESTS);
+    const float lim = bslot >= 0 ? next_up(tbest) : tbest;
+    const float4 q0 = ld4(prims + 4 * slot + 0), q1 = ld4(prims + 4 * slot + 1), q2 = ld4(prims + 4 * slot + 2),
+                 q3 = ld4(prims + 4 * slot + 3);
+    float t;
+    if (isect_flat(q0, q1, q2, q3, o, d, tmin, lim, t)) {
+        if (bslot < 0 || t < tbest || (t == tbest && meta_elem(ld1(meta + slot)) < meta_elem(ld1(meta + bslot)))) {
+            tbest = t;
+            bslot = slot;
+            bcap = 0;
+        }
+    }
+}
+
+// Closest hit over a candidate list: a sun ray's sun-plane cell list
+// (items[b, e)), or a reflected ray's list of its mirror (first reflections)
+// or of its (first, second) mirror pair (second reflections), stored as
+// items[b] = number of flat candidates, then those, then the curved ones: the
+// lanes of a wave hold lists of different mirrors, and with the kinds apart
+// the wave runs one flat loop and one curved loop instead of both tests in
+// every step. Every primitive such a ray can reach is listed (conservative
+// binning, k_grid_* / k_cone_lists), so the result equals a full BVH
+// traversal. bk = list index of the winner.
 __device__ __forceinline__ void closest_in_list(const JobDev& J, const int32_t* __restrict__ items, uint32_t b,
-                                                uint32_t e, V3 o, V3 d, float tmin, float& tbest, int& bslot,
-                                                int& bcap, uint32_t& bk, uint32_t* wc) {
+                                                uint32_t e, bool headed, V3 o, V3 d, float tmin, float& tbest,
+                                                int& bslot, int& bcap, uint32_t& bk, uint32_t* wc) {
     uint32_t* wk = (J.flags & ST_TRACE_COUNT_WORK) ? wc : nullptr;
-    for (uint32_t k = b; k < e; ++k) {
+    uint32_t k = b;
+    if (headed) {
+        const uint32_t fe = b + 1 + ld1((const uint32_t*)items + b);
+        for (k = b + 1; k < fe; ++k) {
+            const int prev = bslot;
+            test_flat(J.prims, J.meta, (int)ld1((const uint32_t*)items + k), o, d, tmin, tbest, bslot, bcap, wk);
+            if (bslot != prev) bk = k;
+        }
+    }
+    for (; k < e; ++k) {
         const int prev = bslot;
         test_prim(J.prims, J.meta, (int)ld1((const uint32_t*)items + k), o, d, tmin, tbest, bslot, bcap, wk);
         if (bslot != prev) bk = k;
@@ -336,25 +367,26 @@ __device__ __forceinline__ void closest_in_list(const JobDev& J, const int32_t* 
 // locates the next segment's list: depth 0 (sun ray): sun-plane cell; depth 1:
 // the mirror it reflected off (key = element); depth 2: the pair (first mirror,
 // second mirror) (key = index of the second mirror in the first one's list);
-// deeper: none.
+// deeper: none. Reflection lists start with their flat-candidate count.
 struct SegList {
     const int32_t* items;
     uint32_t lb, le;
+    bool headed;
 };
-__device__ __forceinline__ SegList no_list() { return SegList{nullptr, 1u, 0u}; }
+__device__ __forceinline__ SegList no_list() { return SegList{nullptr, 1u, 0u, false}; }
 __device__ __forceinline__ SegList cell_list(const JobDev& J, uint32_t cell) {
     if (!J.grid_start) return no_list();
-    return SegList{J.grid_items, ld1(J.grid_start + cell), ld1(J.grid_start + cell + 1)};
+    return SegList{J.grid_items, ld1(J.grid_start + cell), ld1(J.grid_start + cell + 1), false};
 }
 __device__ __forceinline__ SegList next_list(const JobDev& J, int depth, int key) {
     if (key < 0) return no_list();
     if (depth == 1 && J.bounce_range) {
         const float2 r = ld2(J.bounce_range + key);
-        return SegList{J.bounce_items, __float_as_uint(r.x), __float_as_uint(r.y)};
+        return SegList{J.bounce_items, __float_as_uint(r.x), __float_as_uint(r.y), true};
     }
     if (depth == 2 && J.pair_range) {
         const float2 r = ld2(J.pair_range + key);
-        return SegList{J.pair_items, __float_as_uint(r.x), __float_as_uint(r.y)};
+        return SegList{J.pair_items, __float_as_uint(r.x), __float_as_uint(r.y), true};
     }
     return no_list();
 }
@@ -495,7 +527,7 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
     int slot = -1, cap = 0;
     uint32_t bk = 0xFFFFFFFFu;
     const bool listed = L.lb <= L.le;
-    if (listed) closest_in_list(J, L.items, L.lb, L.le, o, d, tmin, t, slot, cap, bk, wc);
+    if (listed) closest_in_list(J, L.items, L.lb, L.le, L.headed, o, d, tmin, t, slot, cap, bk, wc);
     else traverse<STRIDE>(J.nodes, J.prims, J.meta, o, d, tmin, t, slot, cap, stk,
                           (J.flags & ST_TRACE_COUNT_WORK) ? wc : nullptr);
     if (slot < 0) {  // __miss__ms, cuda/materials.cu:236-253
@@ -1240,10 +1272,12 @@ __device__ __forceinline__ void sun_reflection_axis(const double* __restrict__ f
 
 // Task t of scene state k = t / tasks_per_step, i = t % tasks_per_step: first
 // reflections off element i (pair_items == null), or second reflections of
-// rays that left element pair_owner[t] and hit the primitive pair_items[t].
-// The list goes to scratch[t * kMaxBounceList ...]; counts[t] (if given) gets
-// its length, kNoBounceList for none or longer than kMaxBounceList, and
-// range[t] (if given) its (begin, end) in the scratch, (1, 0) for none.
+// rays that left element pair_owner[t] (< 0: none) and hit the primitive
+// pair_items[t]. The list goes to scratch[t * kMaxBounceList ...] as its
+// number of flat candidates followed by the flat, then the curved candidates;
+// counts[t] (if given) gets its length with that header, kNoBounceList for
+// none or longer than kMaxBounceList, and range[t] (if given) its (begin, end)
+// in the scratch, (1, 0) for none.
 constexpr int kConeBlock = 64;
 __global__ __launch_bounds__(kConeBlock) void k_cone_lists(const st_element_f64* __restrict__ elems, int32_t tasks,
                                                            int32_t tasks_per_step,
@@ -1275,6 +1309,10 @@ __global__ __launch_bounds__(kConeBlock) void k_cone_lists(const st_element_f64*
         sun_reflection_axis(frames, from, sun->sun, a);
     } else {
         const int ea = pair_owner[t];
+        if (ea < 0) {  // a list header, not a candidate
+            none();
+            return;
+        }
         from = (int)meta_elem(meta[pair_items[t]]);
         double phi_a = 0.0, phi_b = 0.0;
         if (!bounce_cone(elems[ea], theta_sun, &phi_a) || !bounce_cone(elems[from], 0.0, &phi_b)) {
@@ -1307,15 +1345,25 @@ __global__ __launch_bounds__(kConeBlock) void k_cone_lists(const st_element_f64*
     const double ez = fmax(r2.y, r2.w) - fmin(r2.x, r2.z);
     const double reach = 2.0 * sqrt(ex * ex + ey * ey + ez * ez) + 2.0 * kConeFirst;
     C.ratio = pow(reach / kConeFirst, 1.0 / (kConePieces - 2));
-    const uint32_t count = cone_walk<kConeBlock>(nodes, bounds_slot, bounds_slot + 6 * (size_t)elem_slot[from], C,
-                                                 scratch + (size_t)t * kMaxBounceList, kMaxBounceList,
-                                                 lds_stack + threadIdx.x);
-    if (count > kMaxBounceList) {
+    int32_t* list = scratch + (size_t)t * kMaxBounceList;
+    const uint32_t len = cone_walk<kConeBlock>(nodes, bounds_slot, bounds_slot + 6 * (size_t)elem_slot[from], C,
+                                               list + 1, kMaxBounceList - 1, lds_stack + threadIdx.x);
+    if (len > kMaxBounceList - 1) {
         none();
         return;
     }
-    if (counts) counts[t] = count;
-    if (range) range[t] = make_uint2((uint32_t)t * kMaxBounceList, (uint32_t)t * kMaxBounceList + count);
+    uint32_t nf = 0;  // flat candidates first
+    for (uint32_t i = 0; i < len; ++i) {
+        const int32_t v = list[1 + i];
+        if (meta_kind(meta[v]) == kKindFlat) {
+            list[1 + i] = list[1 + nf];
+            list[1 + nf] = v;
+            ++nf;
+        }
+    }
+    list[0] = (int32_t)nf;
+    if (counts) counts[t] = len + 1;
+    if (range) range[t] = make_uint2((uint32_t)t * kMaxBounceList, (uint32_t)t * kMaxBounceList + len + 1);
 }
 
 // items[range[t].x + k] = scratch[t * kMaxBounceList + k] for every listed task
