@@ -99,8 +99,13 @@ LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kerne
         sh.waves = sh.queued ? 3 : 4;
         if (!trace_variant_exists(sh.block, sh.stack, sh.waves, sh.queued)) sh.block = 128;
     }
+    // Chunks of 64 to 2048 rays. Every fetch from the shared chunk counter (a
+    // single address, ~20 ns each in series) costs launch time, so a launch
+    // that fits one chunk per resident wave gets exactly that (launch_trace
+    // then skips the counter); larger ones get about four chunks per wave.
+    const uint64_t resident = (uint64_t)trace_resident_waves(sh);
     const uint64_t waves = (uint64_t)sh.grid * (uint64_t)(sh.block / 64);
-    uint64_t chunk = rays / (4 * waves);
+    uint64_t chunk = resident > 0 && rays <= resident * 2048 ? (rays + resident - 1) / resident : rays / (4 * waves);
     chunk = (chunk + 63) / 64 * 64;
     sh.chunk = (int)std::min<uint64_t>(std::max<uint64_t>(chunk, 64), 2048);
     sh.kernel = sh.queued ? ST_KERNEL_QUEUED : ST_KERNEL_MEGA;

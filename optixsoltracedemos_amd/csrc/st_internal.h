@@ -62,6 +62,8 @@ struct LaunchShape {
     bool queued = false; // continuing rays parked in LDS, traced in full waves
 };
 bool trace_variant_exists(int block, int stack, int waves, bool queued);
+// waves of this trace variant the device holds at once (0 if unknown)
+int trace_resident_waves(const LaunchShape& shape);
 constexpr bool kDefaultQueued = true;  // form used when the caller and the tuner leave it open
 
 // ---- launchers (all asynchronous on `stream`) -----------------------------

@@ -17,6 +17,8 @@
 // flushed once per chunk with one atomic per wave.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "st_internal.h"
 
 namespace st {
@@ -443,19 +445,27 @@ __device__ __forceinline__ void traverse(const float4* __restrict__ nodes, const
 // ---------------------------------------------------------------------------
 
 // base[idx] += 1 for every lane that calls it (lanes may diverge around the
-// call). One round of wave aggregation: the lanes sharing the first caller's
-// index add their count with a single atomic, the rest add one each. With
-// wave-coherent rays most callers share the index.
-__device__ __forceinline__ void add_one_aggregated(unsigned long long* base, uint32_t idx) {
-    const unsigned long long live = __ballot(1);
-    const int leader = __ffsll((long long)live) - 1;
-    const uint32_t lidx = __shfl(idx, leader, 64);
-    const unsigned long long same = __ballot(idx == lidx);
-    if (idx == lidx) {
-        if ((int)(threadIdx.x & 63) == leader) gadd(base + lidx, (unsigned long long)__popcll(same));
-    } else {
-        gadd(base + idx, 1ull);
+// call). The calling lanes are grouped by index, one device-scope atomic per
+// group, for up to ROUNDS groups; lanes left after that add one each. Coherent
+// waves share few indices (the mirrors a wave's rays hit), so most calls end
+// after a round or two.
+template <int ROUNDS>
+__device__ __forceinline__ void add_one_grouped(unsigned long long* base, uint32_t idx) {
+    const int lane = (int)(threadIdx.x & 63);
+    bool done = false;
+    for (int r = 0; r < ROUNDS; ++r) {
+        if (!__ballot(!done)) return;
+        if (!done) {
+            // the first lane still to add leads; readfirstlane reads it
+            // without a cross-lane shuffle through LDS
+            const uint32_t lidx = __builtin_amdgcn_readfirstlane(idx);
+            const bool mine = idx == lidx;
+            const unsigned long long same = __ballot(mine);
+            if (lane == __ffsll((long long)__ballot(1)) - 1) gadd(base + lidx, (unsigned long long)__popcll(same));
+            done = mine;
+        }
     }
+    if (!done) gadd(base + idx, 1ull);
 }
 
 __device__ __forceinline__ void record(const JobDev& J, uint64_t ray, int slot, float tag, V3 p, uint32_t elem) {
@@ -576,7 +586,7 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
                 int ix = (int)floorf(fx), iy = (int)floorf(fy);
                 ix = min(max(ix, 0), J.flux_nx - 1);
                 iy = min(max(iy, 0), J.flux_ny - 1);
-                add_one_aggregated((unsigned long long*)J.flux, (uint32_t)(iy * J.flux_nx + ix));
+                add_one_grouped<2>((unsigned long long*)J.flux, (uint32_t)(iy * J.flux_nx + ix));
                 count(wc, ST_STAT_FLUX_BINNED);
             }
         } else {
@@ -591,7 +601,7 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
     }
     record(J, ray, nd, 1.f, p, elem);
     last = nd;
-    if (J.hits) add_one_aggregated((unsigned long long*)J.hits, elem);
+    if (J.hits) add_one_grouped<64>((unsigned long long*)J.hits, elem);
     count(wc, ST_STAT_MIRROR_HITS);
     // draws of bounce nd: block 2nd-1 = (reflectivity, slope xr, slope xp,
     // specular xr), block 2nd = (specular xp, -, -, -), each drawn only when
@@ -687,7 +697,7 @@ template <int BLOCK, int STACK, int MINW, bool QUEUED>
 __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict__ jobs,
                                                        const uint64_t* __restrict__ chunk_prefix, int32_t n_jobs,
                                                        uint64_t total_chunks, uint32_t* __restrict__ work_counter,
-                                                       int32_t chunk_rays, JobDev single) {
+                                                       int32_t chunk_rays, int32_t static_chunks, JobDev single) {
     __shared__ int32_t lds_stack[STACK * BLOCK];
     __shared__ uint32_t lds_counters[BLOCK / 64][kNumCounters];
     __shared__ float lds_queue[QUEUED ? BLOCK / 64 : 1][QUEUED ? 7 * kQueueCap : 1];
@@ -695,25 +705,34 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
     const int lane = threadIdx.x & 63;
     uint32_t* wc = lds_counters[threadIdx.x >> 6];
     if (lane < kNumCounters) wc[lane] = 0;
-    for (;;) {
-        uint32_t c = 0;
-        if (lane == 0) c = gadd32(&work_counter[0], 1u);
-        c = __builtin_amdgcn_readfirstlane(c);
-        if ((uint64_t)c >= total_chunks) break;
+    const uint32_t waves = gridDim.x * (BLOCK / 64);
+    const uint32_t wave_id = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    for (bool first_pass = true;; first_pass = false) {
+        // static_chunks: every wave of the (fully resident) grid traces chunk
+        // wave_id only, and the shared counter is not touched
+        uint64_t c = wave_id;
+        if (static_chunks) {
+            if (!first_pass) break;
+        } else {
+            uint32_t f = 0;
+            if (lane == 0) f = gadd32(&work_counter[0], 1u);
+            c = __builtin_amdgcn_readfirstlane(f);
+        }
+        if (c >= total_chunks) break;
         int lo = 0;
         uint64_t base = 0;
         if (jobs) {
             int hi = n_jobs;  // chunk_prefix[lo] <= c < chunk_prefix[hi]
             while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
-                if (chunk_prefix[mid] <= (uint64_t)c) lo = mid;
+                if (chunk_prefix[mid] <= c) lo = mid;
                 else hi = mid;
             }
             base = chunk_prefix[lo];
         }
         const JobDev& J = jobs ? jobs[lo] : single;
         const ST_CONST SunFrame& S = *(const ST_CONST SunFrame*)J.sun;
-        const uint64_t first = ((uint64_t)c - base) * (uint64_t)chunk_rays;
+        const uint64_t first = (c - base) * (uint64_t)chunk_rays;
         const uint64_t end = min(first + (uint64_t)chunk_rays, J.ray_count);
         if (QUEUED) {
             float* q = lds_queue[threadIdx.x >> 6];
@@ -786,9 +805,8 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
             }
         }
     }
-    if (lane == 0) {
+    if (!static_chunks && lane == 0) {
         __threadfence();
-        const uint32_t waves = gridDim.x * (BLOCK / 64);
         if (gadd32(&work_counter[1], 1u) == waves - 1u) {
             atomicExch(&work_counter[0], 0u);
             atomicExch(&work_counter[1], 0u);
@@ -1442,9 +1460,25 @@ hipError_t launch_sun_plane(hipStream_t s, const double* ref_aabb, int32_t n, in
 
 template <int BLOCK, int STACK, int MINW, bool QUEUED>
 static void launch_variant(hipStream_t s, int grid, const JobDev* jobs, const uint64_t* chunk_prefix, int32_t n_jobs,
-                           uint64_t total_chunks, uint32_t* work_counter, int chunk, const JobDev& single) {
+                           uint64_t total_chunks, uint32_t* work_counter, int chunk, int static_chunks,
+                           const JobDev& single) {
     hipLaunchKernelGGL((k_trace<BLOCK, STACK, MINW, QUEUED>), dim3(grid), dim3(BLOCK), 0, s, jobs, chunk_prefix,
-                       n_jobs, total_chunks, work_counter, chunk, single);
+                       n_jobs, total_chunks, work_counter, chunk, static_chunks, single);
+}
+
+template <int BLOCK, int STACK, int MINW, bool QUEUED>
+static int resident_blocks_per_cu() {
+    static std::atomic<int> cached{-1};  // per variant; the library runs on one kind of device
+    int nb = cached.load();
+    if (nb < 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_trace<BLOCK, STACK, MINW, QUEUED>, BLOCK,
+                                                         0) != hipSuccess) {
+            (void)hipGetLastError();
+            nb = 0;
+        }
+        cached.store(nb);
+    }
+    return nb;
 }
 
 // Compiled shapes. The queued form keeps 3.5 KB of parked rays per wave in
@@ -1461,6 +1495,35 @@ bool trace_variant_exists(int block, int stack, int waves, bool queued) {
     return false;
 }
 
+#define ST_VARIANTS                \
+    ST_VARIANT(64, 32, 4, false)   \
+    ST_VARIANT(64, 32, 5, false)   \
+    ST_VARIANT(128, 32, 4, false)  \
+    ST_VARIANT(128, 32, 5, false)  \
+    ST_VARIANT(256, 32, 4, false)  \
+    ST_VARIANT(128, 16, 4, false)  \
+    ST_VARIANT(128, 16, 6, false)  \
+    ST_VARIANT(128, 16, 8, false)  \
+    ST_VARIANT(256, 16, 4, false)  \
+    ST_VARIANT(256, 16, 6, false)  \
+    ST_VARIANT(256, 16, 8, false)  \
+    ST_VARIANT(128, 16, 4, true)   \
+    ST_VARIANT(128, 16, 5, true)   \
+    ST_VARIANT(256, 16, 4, true)   \
+    ST_VARIANT(256, 16, 5, true)   \
+    ST_VARIANT(64, 32, 3, true)    \
+    ST_VARIANT(128, 32, 3, true)
+
+int trace_resident_waves(const LaunchShape& shape) {
+    const int st = shape.stack, w = shape.waves, b = shape.block;
+    const bool r = shape.queued;
+#define ST_VARIANT(B, S, W, Q) \
+    if (b == B && st == S && w == W && r == Q) return resident_blocks_per_cu<B, S, W, Q>() * device_cu_count() * (B / 64);
+    ST_VARIANTS
+#undef ST_VARIANT
+    return 0;
+}
+
 hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk_prefix, int32_t n_jobs,
                         uint64_t total_chunks, uint32_t* work_counter, const LaunchShape& shape,
                         const JobDev& single) {
@@ -1472,28 +1535,18 @@ hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk
     const int st = shape.stack, w = shape.waves, b = shape.block, c = shape.chunk;
     const bool r = shape.queued;
     if (r && (c <= 0 || c > (int)kMaxChunkRays)) return hipErrorInvalidValue;  // parked rays keep an 11-bit offset
-#define ST_VARIANT(B, S, W, Q)                                                                             \
-    if (b == B && st == S && w == W && r == Q) {                                                          \
-        launch_variant<B, S, W, Q>(s, grid, jobs, chunk_prefix, n_jobs, total_chunks, work_counter, c, single); \
-        return hipGetLastError();                                                                         \
+    // one chunk per wave without the shared counter, when every wave fits on
+    // the device at once (a wave not yet resident would hold its chunk back
+    // until others finish)
+    const int stat = total_chunks <= (uint64_t)trace_resident_waves(shape) ? 1 : 0;
+    if (stat) grid = (int)max_blocks;
+#define ST_VARIANT(B, S, W, Q)                                                                                 \
+    if (b == B && st == S && w == W && r == Q) {                                                              \
+        launch_variant<B, S, W, Q>(s, grid, jobs, chunk_prefix, n_jobs, total_chunks, work_counter, c, stat, \
+                                   single);                                                                   \
+        return hipGetLastError();                                                                             \
     }
-    ST_VARIANT(64, 32, 4, false)
-    ST_VARIANT(64, 32, 5, false)
-    ST_VARIANT(128, 32, 4, false)
-    ST_VARIANT(128, 32, 5, false)
-    ST_VARIANT(256, 32, 4, false)
-    ST_VARIANT(128, 16, 4, false)
-    ST_VARIANT(128, 16, 6, false)
-    ST_VARIANT(128, 16, 8, false)
-    ST_VARIANT(256, 16, 4, false)
-    ST_VARIANT(256, 16, 6, false)
-    ST_VARIANT(256, 16, 8, false)
-    ST_VARIANT(128, 16, 4, true)
-    ST_VARIANT(128, 16, 5, true)
-    ST_VARIANT(256, 16, 4, true)
-    ST_VARIANT(256, 16, 5, true)
-    ST_VARIANT(64, 32, 3, true)
-    ST_VARIANT(128, 32, 3, true)
+    ST_VARIANTS
 #undef ST_VARIANT
     return hipErrorInvalidValue;
 }

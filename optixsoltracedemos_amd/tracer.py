@@ -140,8 +140,7 @@ class Scene:
         d.num_rays, d.ray_begin, d.ray_count, d.seed = int(num_rays), int(ray_begin), int(ray_count), int(seed)
         d.job_id, d.max_depth = int(job_id), int(max_depth)
         d.flags = ((_abi.TRACE_RECORDS if records else 0) | (_abi.TRACE_FLUX_DIRECT_SUN if include_direct_sun else 0)
-                   | (_abi.TRACE_ZERO_OUTPUTS if zero else 0) | (_abi.TRACE_BVH_ONLY if bvh_only else 0) \
-            | (_abi.TRACE_COUNT_WORK if count_work else 0)
+                   | (_abi.TRACE_ZERO_OUTPUTS if zero else 0) | (_abi.TRACE_BVH_ONLY if bvh_only else 0)
                    | (_abi.TRACE_COUNT_WORK if count_work else 0))
         d.kernel, d.flux_element, d.flux_nx, d.flux_ny = int(kernel), int(flux_element), int(flux_nx), int(flux_ny)
         d.block_size, d.grid_blocks = int(block_size), int(grid_blocks)
