@@ -287,8 +287,10 @@ int st_trace_batch(st_scene scene, const st_batch_desc* desc, int64_t* flux, uin
 typedef struct st_server_config {
     uint64_t hbm_budget_bytes;  /* admission limit over resident job footprints */
     int32_t max_active_jobs;    /* slots in the continuous batch                */
-    uint64_t chunk_rays;        /* rays per job per scheduling round            */
+    uint64_t chunk_rays;        /* most rays of one job per scheduling round    */
     int32_t block_size, grid_blocks;
+    uint64_t round_rays;        /* most rays of one round over all jobs, shortest
+                                   remaining job first (0: 4 x chunk_rays)      */
 } st_server_config;
 typedef struct st_job_desc {
     int32_t scene_slot;          /* index returned by st_server_add_scene        */

@@ -154,6 +154,7 @@ class ServerConfig(C.Structure):
         ("chunk_rays", C.c_uint64),
         ("block_size", C.c_int32),
         ("grid_blocks", C.c_int32),
+        ("round_rays", C.c_uint64),
     ]
 
 

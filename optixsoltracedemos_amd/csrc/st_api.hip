@@ -77,7 +77,7 @@ int DevBuf::ensure(size_t n, const char* what) {
 }
 
 LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel, int32_t bvh_depth,
-                       const LaunchShape* tuned) {
+                       const LaunchShape* tuned, int32_t max_elements) {
     LaunchShape sh;
     const int cus = device_cu_count();
     // ST_KERNEL_MEGA: whole ray per lane; ST_KERNEL_QUEUED: continuing rays
@@ -98,6 +98,12 @@ LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kerne
         sh.stack = 32;
         sh.waves = sh.queued ? 3 : 4;
         if (!trace_variant_exists(sh.block, sh.stack, sh.waves, sh.queued)) sh.block = 128;
+    }
+    if (sh.queued && max_elements <= kLdsHitElements && bvh_depth <= 8 && block <= 0) {
+        sh.block = 256;
+        sh.stack = 8;
+        sh.waves = 5;
+        sh.lds_hits = true;
     }
     // Chunks of 64 to 2048 rays. Every fetch from the shared chunk counter (a
     // single address, ~20 ns each in series) costs launch time, so a launch
@@ -308,13 +314,6 @@ int scene_ensure_lists(st_scene s, uint64_t rays, bool force) {
     return ST_OK;
 }
 
-// Largest angle between a sampled sun ray and the sun axis. Pillbox: the half
-// angle. Gaussian: sigma * sqrt(-2 ln u) with u >= 2^-24 (the smallest
-// uniform) = 5.768 sigma. Widened for FP32 direction rounding.
-static double sun_cone_angle(const st_sun_desc& sun) {
-    const double th = sun.shape == ST_SUN_GAUSSIAN ? 5.7686 * sun.param : sun.param;
-    return th * 1.001 + 1e-6;
-}
 static double sun_cone_tan(const st_sun_desc& sun) { return tan(sun_cone_angle(sun)); }
 
 // walk pass (lists into per-task scratch) -> host prefix sum (lists over
@@ -723,7 +722,7 @@ extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* ou
         if (out->stats) HIP_TRY(launch_fill_u64(st, out->stats, ST_STAT_COUNT, 0), "zero stats");
     }
     const LaunchShape sh = pick_shape(J.ray_count, d->block_size, d->grid_blocks, d->kernel, s->topo.depth,
-                                      s->tuned_valid ? &s->tuned : nullptr);
+                                      s->tuned_valid ? &s->tuned : nullptr, s->n);
     const uint64_t chunks = (J.ray_count + sh.chunk - 1) / sh.chunk;
     if (chunks >= (1ull << 32)) return fail(ST_ERR_INVALID, "too many rays for one call");
     if (launch_injected()) return fail(ST_ERR_INJECTED, "injected launch failure (trace)");
@@ -795,7 +794,7 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
         return ST_OK;
     };
     const int cus = device_cu_count();
-    const LaunchShape def = pick_shape(sample_rays, 0, 0, ST_KERNEL_AUTO, s->topo.depth, nullptr);
+    const LaunchShape def = pick_shape(sample_rays, 0, 0, ST_KERNEL_AUTO, s->topo.depth, nullptr, s->n);
     double def_ms = 0;
     int rc = time_shape(def, &def_ms);
     LaunchShape best = def;
@@ -1072,7 +1071,7 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
         uint64_t total_rays = 0;
         for (int k = 0; k < K; ++k) total_rays += d->budgets[k];
         b.shape = pick_shape(total_rays, d->block_size, d->grid_blocks, ST_KERNEL_AUTO, s->topo.depth,
-                             s->tuned_valid ? &s->tuned : nullptr);
+                             s->tuned_valid ? &s->tuned : nullptr, s->n);
         std::vector<JobDev> jobs(K);
         std::vector<uint64_t> prefix(K + 1, 0);
         for (int k = 0; k < K; ++k) {

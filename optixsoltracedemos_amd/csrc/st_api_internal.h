@@ -31,9 +31,11 @@ struct DevBuf {
 };
 
 // launch shape for `rays` rays; block/grid > 0 override, else the scene's tuned
-// shape when there is one, else defaults (16-entry stack when the BVH allows)
+// shape when there is one, else defaults (16-entry stack when the BVH allows);
+// max_elements: the largest scene in the launch (small ones count element
+// hits in LDS)
 LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel, int32_t bvh_depth,
-                       const LaunchShape* tuned);
+                       const LaunchShape* tuned, int32_t max_elements);
 
 struct BatchCache {
     bool valid = false;
@@ -135,6 +137,13 @@ struct st_scene_s {
 };
 
 namespace st {
+// Largest angle between a sampled sun ray and the sun axis. Pillbox: the half
+// angle. Gaussian: sigma * sqrt(-2 ln u) with u >= 2^-24 (the smallest
+// uniform) = 5.768 sigma. Widened for FP32 direction rounding.
+inline double sun_cone_angle(const st_sun_desc& sun) {
+    const double th = sun.shape == ST_SUN_GAUSSIAN ? 5.7686 * sun.param : sun.param;
+    return th * 1.001 + 1e-6;
+}
 int scene_refresh(st_scene s, bool use_overrides);
 int scene_sun_plane(st_scene s);  // also rebuilds the sun-plane candidate grid
 int scene_grid(st_scene s);

@@ -60,7 +60,12 @@ struct LaunchShape {
     int stack = 32;      // traversal stack entries per lane (16 or 32, >= BVH depth)
     int waves = 4;       // waves per SIMD the register allocation is built for
     bool queued = false; // continuing rays parked in LDS, traced in full waves
+    bool lds_hits = false;  // per-element hit counts kept per wave in LDS (scenes of
+                            // <= kLdsHitElements elements; queued, block 256, stack 8)
 };
+// Scenes this small count element hits in LDS: their few counters would
+// otherwise take a device-scope atomic from nearly every wave of the launch.
+constexpr int kLdsHitElements = 128;
 bool trace_variant_exists(int block, int stack, int waves, bool queued);
 // waves of this trace variant the device holds at once (0 if unknown)
 int trace_resident_waves(const LaunchShape& shape);

@@ -529,8 +529,9 @@ __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame
 // program. Returns true when the ray continues (o, d, tmin, depth updated for
 // the next segment), false when it ended.
 template <int STRIDE>
-__device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int32_t* stk, uint32_t* wc, V3& o,
-                                              V3& d, float& tmin, int& depth, int& last, SegList L, int& key) {
+__device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int32_t* stk, uint32_t* wc,
+                                              uint32_t* wh, V3& o, V3& d, float& tmin, int& depth, int& last,
+                                              SegList L, int& key) {
     const uint32_t rlo = (uint32_t)ray, rhi = (uint32_t)(ray >> 32);
     count(wc, ST_STAT_SEGMENTS);
     float t = kTmax;
@@ -562,7 +563,10 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
             last = nd;
             const bool fe = slot == J.flux_slot;
             count_if(wc, kAggCounter, slot == J.agg_slot);
-            if (slot != J.agg_slot && J.hits) gadd(J.hits + elem, 1ull);
+            if (slot != J.agg_slot && J.hits) {
+                if (wh && elem < (uint32_t)kLdsHitElements) atomicAdd(wh + elem, 1u);
+                else gadd(J.hits + elem, 1ull);
+            }
             count(wc, ST_STAT_RECEIVER_HITS);
             const bool direct = depth == 0;
             count_if(wc, ST_STAT_DIRECT_SUN, direct);
@@ -601,7 +605,10 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
     }
     record(J, ray, nd, 1.f, p, elem);
     last = nd;
-    if (J.hits) add_one_grouped<64>((unsigned long long*)J.hits, elem);
+    if (J.hits) {
+        if (wh && elem < (uint32_t)kLdsHitElements) atomicAdd(wh + elem, 1u);
+        else add_one_grouped<64>((unsigned long long*)J.hits, elem);
+    }
     count(wc, ST_STAT_MIRROR_HITS);
     // draws of bounce nd: block 2nd-1 = (reflectivity, slope xr, slope xp,
     // specular xr), block 2nd = (specular xp, -, -, -), each drawn only when
@@ -693,7 +700,9 @@ __device__ __forceinline__ uint32_t queue_tag(uint64_t offset, int depth, int ke
     return (uint32_t)offset | ((uint32_t)depth << 11) | (f << 15);
 }
 
-template <int BLOCK, int STACK, int MINW, bool QUEUED>
+// LHITS > 0: element hit counts of scenes with at most LHITS elements are
+// kept per wave in LDS and added to the output once per chunk.
+template <int BLOCK, int STACK, int MINW, bool QUEUED, int LHITS>
 __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict__ jobs,
                                                        const uint64_t* __restrict__ chunk_prefix, int32_t n_jobs,
                                                        uint64_t total_chunks, uint32_t* __restrict__ work_counter,
@@ -701,10 +710,14 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
     __shared__ int32_t lds_stack[STACK * BLOCK];
     __shared__ uint32_t lds_counters[BLOCK / 64][kNumCounters];
     __shared__ float lds_queue[QUEUED ? BLOCK / 64 : 1][QUEUED ? 7 * kQueueCap : 1];
+    __shared__ uint32_t lds_hits[LHITS > 0 ? BLOCK / 64 : 1][LHITS > 0 ? LHITS : 1];
     int32_t* stk = lds_stack + threadIdx.x;
     const int lane = threadIdx.x & 63;
     uint32_t* wc = lds_counters[threadIdx.x >> 6];
     if (lane < kNumCounters) wc[lane] = 0;
+    uint32_t* wh = LHITS > 0 ? lds_hits[threadIdx.x >> 6] : nullptr;
+    if (LHITS > 0)
+        for (int i = lane; i < LHITS; i += 64) wh[i] = 0;
     const uint32_t waves = gridDim.x * (BLOCK / 64);
     const uint32_t wave_id = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
     for (bool first_pass = true;; first_pass = false) {
@@ -756,7 +769,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                     int last = depth;
                     float tmin = kTminBounce;
                     const uint64_t ray = base_ray + off;
-                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, next_list(J, depth, key),
+                    cont = trace_segment<BLOCK>(J, ray, stk, wc, wh, o, d, tmin, depth, last, next_list(J, depth, key),
                                                 key);
                     if (!cont) finish_ray(J, ray, last);
                 }
@@ -772,7 +785,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                     uint32_t cell;
                     sun_ray(J, S, ray, wc, o, d, cell);
                     float tmin = kTminPrimary;
-                    cont = trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, cell_list(J, cell), key);
+                    cont = trace_segment<BLOCK>(J, ray, stk, wc, wh, o, d, tmin, depth, last, cell_list(J, cell), key);
                     if (!cont) finish_ray(J, ray, last);
                 }
                 qn = queue_push(q, qn, cont, o, d, queue_tag(i - first, depth, key));
@@ -787,14 +800,22 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                 sun_ray(J, S, ray, wc, o, d, cell);
                 float tmin = kTminPrimary;
                 int depth = 0, last = 0, key = -1;
-                if (trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, cell_list(J, cell), key))
-                    while (trace_segment<BLOCK>(J, ray, stk, wc, o, d, tmin, depth, last, next_list(J, depth, key),
+                if (trace_segment<BLOCK>(J, ray, stk, wc, wh, o, d, tmin, depth, last, cell_list(J, cell), key))
+                    while (trace_segment<BLOCK>(J, ray, stk, wc, wh, o, d, tmin, depth, last, next_list(J, depth, key),
                                                 key)) {
                     }
                 finish_ray(J, ray, last);
             }
         }
         // flush this chunk's counters (same wave wrote them: LDS order suffices)
+        if (LHITS > 0)
+            for (int i = lane; i < LHITS; i += 64) {
+                const uint32_t h = wh[i];
+                if (h) {
+                    wh[i] = 0;
+                    if (J.hits) gadd(J.hits + i, (unsigned long long)h);
+                }
+            }
         const uint32_t v = lane < kNumCounters ? wc[lane] : 0u;
         if (lane < kNumCounters) wc[lane] = 0;
         if (v) {
@@ -1458,21 +1479,21 @@ hipError_t launch_sun_plane(hipStream_t s, const double* ref_aabb, int32_t n, in
     return hipGetLastError();
 }
 
-template <int BLOCK, int STACK, int MINW, bool QUEUED>
+template <int BLOCK, int STACK, int MINW, bool QUEUED, int LHITS>
 static void launch_variant(hipStream_t s, int grid, const JobDev* jobs, const uint64_t* chunk_prefix, int32_t n_jobs,
                            uint64_t total_chunks, uint32_t* work_counter, int chunk, int static_chunks,
                            const JobDev& single) {
-    hipLaunchKernelGGL((k_trace<BLOCK, STACK, MINW, QUEUED>), dim3(grid), dim3(BLOCK), 0, s, jobs, chunk_prefix,
+    hipLaunchKernelGGL((k_trace<BLOCK, STACK, MINW, QUEUED, LHITS>), dim3(grid), dim3(BLOCK), 0, s, jobs, chunk_prefix,
                        n_jobs, total_chunks, work_counter, chunk, static_chunks, single);
 }
 
-template <int BLOCK, int STACK, int MINW, bool QUEUED>
+template <int BLOCK, int STACK, int MINW, bool QUEUED, int LHITS>
 static int resident_blocks_per_cu() {
     static std::atomic<int> cached{-1};  // per variant; the library runs on one kind of device
     int nb = cached.load();
     if (nb < 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_trace<BLOCK, STACK, MINW, QUEUED>, BLOCK,
-                                                         0) != hipSuccess) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_trace<BLOCK, STACK, MINW, QUEUED, LHITS>,
+                                                         BLOCK, 0) != hipSuccess) {
             (void)hipGetLastError();
             nb = 0;
         }
@@ -1495,30 +1516,32 @@ bool trace_variant_exists(int block, int stack, int waves, bool queued) {
     return false;
 }
 
-#define ST_VARIANTS                \
-    ST_VARIANT(64, 32, 4, false)   \
-    ST_VARIANT(64, 32, 5, false)   \
-    ST_VARIANT(128, 32, 4, false)  \
-    ST_VARIANT(128, 32, 5, false)  \
-    ST_VARIANT(256, 32, 4, false)  \
-    ST_VARIANT(128, 16, 4, false)  \
-    ST_VARIANT(128, 16, 6, false)  \
-    ST_VARIANT(128, 16, 8, false)  \
-    ST_VARIANT(256, 16, 4, false)  \
-    ST_VARIANT(256, 16, 6, false)  \
-    ST_VARIANT(256, 16, 8, false)  \
-    ST_VARIANT(128, 16, 4, true)   \
-    ST_VARIANT(128, 16, 5, true)   \
-    ST_VARIANT(256, 16, 4, true)   \
-    ST_VARIANT(256, 16, 5, true)   \
-    ST_VARIANT(64, 32, 3, true)    \
-    ST_VARIANT(128, 32, 3, true)
+#define ST_VARIANTS                    \
+    ST_VARIANT(64, 32, 4, false, 0)    \
+    ST_VARIANT(64, 32, 5, false, 0)    \
+    ST_VARIANT(128, 32, 4, false, 0)   \
+    ST_VARIANT(128, 32, 5, false, 0)   \
+    ST_VARIANT(256, 32, 4, false, 0)   \
+    ST_VARIANT(128, 16, 4, false, 0)   \
+    ST_VARIANT(128, 16, 6, false, 0)   \
+    ST_VARIANT(128, 16, 8, false, 0)   \
+    ST_VARIANT(256, 16, 4, false, 0)   \
+    ST_VARIANT(256, 16, 6, false, 0)   \
+    ST_VARIANT(256, 16, 8, false, 0)   \
+    ST_VARIANT(128, 16, 4, true, 0)    \
+    ST_VARIANT(128, 16, 5, true, 0)    \
+    ST_VARIANT(256, 16, 4, true, 0)    \
+    ST_VARIANT(256, 16, 5, true, 0)    \
+    ST_VARIANT(64, 32, 3, true, 0)     \
+    ST_VARIANT(128, 32, 3, true, 0)    \
+    ST_VARIANT(256, 8, 5, true, kLdsHitElements)
 
 int trace_resident_waves(const LaunchShape& shape) {
-    const int st = shape.stack, w = shape.waves, b = shape.block;
+    const int st = shape.stack, w = shape.waves, b = shape.block, h = shape.lds_hits ? kLdsHitElements : 0;
     const bool r = shape.queued;
-#define ST_VARIANT(B, S, W, Q) \
-    if (b == B && st == S && w == W && r == Q) return resident_blocks_per_cu<B, S, W, Q>() * device_cu_count() * (B / 64);
+#define ST_VARIANT(B, S, W, Q, H)                                    \
+    if (b == B && st == S && w == W && r == Q && h == H)             \
+        return resident_blocks_per_cu<B, S, W, Q, H>() * device_cu_count() * (B / 64);
     ST_VARIANTS
 #undef ST_VARIANT
     return 0;
@@ -1533,6 +1556,7 @@ hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk
     const uint64_t max_blocks = (waves_needed + waves_per_block - 1) / waves_per_block;
     if ((uint64_t)grid > max_blocks) grid = (int)(max_blocks > 0 ? max_blocks : 1);
     const int st = shape.stack, w = shape.waves, b = shape.block, c = shape.chunk;
+    const int h = shape.lds_hits ? kLdsHitElements : 0;
     const bool r = shape.queued;
     if (r && (c <= 0 || c > (int)kMaxChunkRays)) return hipErrorInvalidValue;  // parked rays keep an 11-bit offset
     // one chunk per wave without the shared counter, when every wave fits on
@@ -1540,11 +1564,11 @@ hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk
     // until others finish)
     const int stat = total_chunks <= (uint64_t)trace_resident_waves(shape) ? 1 : 0;
     if (stat) grid = (int)max_blocks;
-#define ST_VARIANT(B, S, W, Q)                                                                                 \
-    if (b == B && st == S && w == W && r == Q) {                                                              \
-        launch_variant<B, S, W, Q>(s, grid, jobs, chunk_prefix, n_jobs, total_chunks, work_counter, c, stat, \
-                                   single);                                                                   \
-        return hipGetLastError();                                                                             \
+#define ST_VARIANT(B, S, W, Q, H)                                                                                 \
+    if (b == B && st == S && w == W && r == Q && h == H) {                                                       \
+        launch_variant<B, S, W, Q, H>(s, grid, jobs, chunk_prefix, n_jobs, total_chunks, work_counter, c, stat,  \
+                                      single);                                                                   \
+        return hipGetLastError();                                                                                \
     }
     ST_VARIANTS
 #undef ST_VARIANT

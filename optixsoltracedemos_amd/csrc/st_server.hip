@@ -8,13 +8,18 @@
 //      the whole budget is rejected;
 //   2. set up every newly admitted job: copy the scene's elements, re-aim its
 //      mirrors for the job's sun (FP64 bisector at the scene's last receiver),
-//      pack frames, refit the BVH and build the sun plane - all on the device;
-//   3. launch ONE trace kernel over a slice of every active job (jobs are
-//      chunk lists of a single launch, so work of different jobs overlaps);
+//      pack frames, refit the BVH, build the sun plane and its candidate grid
+//      and, for jobs of kServerListRays rays or more, the first-reflection
+//      candidate lists - all on the device, in buffers pooled per scene;
+//   3. launch ONE trace kernel over a slice of the active jobs, shortest
+//      remaining job first, at most chunk_rays per job and round_rays in all
+//      (jobs are chunk lists of a single launch, so work of different jobs
+//      overlaps and a short job does not wait behind long ones);
 //   4. retire jobs whose rays are all traced (statistics copied to the host,
-//      per-job scene copy freed, outputs kept until released).
-// Ray draws are keyed by (seed, job_id, ray index), so a job's result does not
-// depend on the slices or on which other jobs shared its launches.
+//      per-job scene copy back to the pool, outputs kept until released).
+// Ray draws are keyed by (seed, job_id, ray index) and the candidate lists
+// give the BVH's hits, so a job's result does not depend on the slices, on the
+// lists or on which other jobs shared its launches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,6 +44,18 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// device copy of a scene for one job (re-aimed), its sun plane, candidate
+// grid and reflection lists; pooled per scene
+struct WorkBufs {
+    DevBuf elems, aim_in, ref_aabb, bounds, prims, nodes, sun, frames, grid_start, grid_items, list_range, list_items;
+};
+// a job's outputs, kept until st_server_release; pooled
+struct OutBufs {
+    DevBuf flux, hits, stats;
+};
+
+constexpr uint64_t kServerListRays = 1ull << 24;  // jobs this long get first-reflection lists
+
 struct ServerJob {
     int64_t handle = 0;
     st_job_desc desc{};
@@ -50,7 +67,9 @@ struct ServerJob {
     uint64_t stats[ST_STAT_COUNT] = {0};
     double aim_host[6] = {0, 0, 0, 0, 0, 0};  // sun unit vector + aim target (source of an async upload)
     size_t footprint = 0, scene_bytes = 0;
-    DevBuf elems, aim_in, ref_aabb, bounds, prims, nodes, sun, flux, hits, stats_dev;
+    bool lists = false;
+    std::unique_ptr<WorkBufs> work;
+    std::unique_ptr<OutBufs> out;
     JobDev dev{};
 };
 
@@ -73,45 +92,83 @@ struct st_server_s {
     DevBuf d_jobs, d_prefix, d_counter;
     JobDev* h_jobs = nullptr;     // pinned
     uint64_t* h_prefix = nullptr; // pinned
+    // grid set-up scratch shared by the jobs set up in a round (worker stream)
+    DevBuf d_grid_counts, d_grid_cursor, d_grid_stats;
+    uint32_t* h_grid_stats = nullptr;  // pinned, 2 per active slot
+    // buffer pools: scene copies per scene slot (worker thread only), outputs (under mu)
+    std::unordered_map<int32_t, std::vector<std::unique_ptr<WorkBufs>>> work_pool;
+    std::vector<std::unique_ptr<OutBufs>> out_pool;
     st_server_stats stats{};
     std::string worker_error;
 };
 
 namespace {
 
+// device bytes of a job: the re-aimed scene copy with its sun plane, grid
+// (at most kGridMaxAverage entries per cell) and, for long jobs, frames and
+// reflection lists; plus the outputs
+size_t job_scene_bytes(st_scene sc, const st_job_desc& d) {
+    const size_t n = (size_t)sc->n, nn = (size_t)sc->n_nodes;
+    size_t b = n * (sizeof(st_element_f64) + 6 * sizeof(double) + 6 * sizeof(float) + 4 * sizeof(float4)) +
+               nn * 4 * sizeof(float4) + sizeof(SunFrame) + 6 * sizeof(double) +
+               (kGridCells + 1 + (size_t)kGridMaxAverage * kGridCells) * sizeof(uint32_t);
+    if (d.num_rays >= kServerListRays)
+        b += n * (9 * sizeof(double) + sizeof(uint2) + kMaxBounceList * sizeof(int32_t));
+    return b;
+}
+
 size_t job_footprint(st_scene sc, const st_job_desc& d) {
-    const size_t n = (size_t)sc->n, nn = (size_t)sc->n_nodes;
-    size_t scene_copy = n * (sizeof(st_element_f64) + 6 * sizeof(double) + 6 * sizeof(float) + 4 * sizeof(float4)) +
-                        nn * 4 * sizeof(float4) + sizeof(SunFrame) + 6 * sizeof(double);
-    size_t outputs = (size_t)std::max(d.flux_nx, 0) * (size_t)std::max(d.flux_ny, 0) * sizeof(int64_t) +
-                     n * sizeof(uint64_t) + ST_STAT_COUNT * sizeof(uint64_t);
-    return scene_copy + outputs;
+    const size_t n = (size_t)sc->n;
+    const size_t outputs = (size_t)std::max(d.flux_nx, 0) * (size_t)std::max(d.flux_ny, 0) * sizeof(int64_t) +
+                           n * sizeof(uint64_t) + ST_STAT_COUNT * sizeof(uint64_t);
+    return job_scene_bytes(sc, d) + outputs;
 }
 
-size_t job_scene_bytes(st_scene sc) {
-    const size_t n = (size_t)sc->n, nn = (size_t)sc->n_nodes;
-    return n * (sizeof(st_element_f64) + 6 * sizeof(double) + 6 * sizeof(float) + 4 * sizeof(float4)) +
-           nn * 4 * sizeof(float4) + sizeof(SunFrame) + 6 * sizeof(double);
-}
-
-// device set-up of one admitted job on the server stream
-int setup_job(st_server srv, ServerJob* j) {
+// First half of a job's device set-up on the server stream: scene copy, aim,
+// frames, refit, sun plane, zeroed outputs, and the count pass of its
+// candidate grid (totals to d_grid_stats[2 * slot]).
+int setup_job(st_server srv, ServerJob* j, int slot) {
     st_scene sc = j->scene;
     const size_t n = (size_t)sc->n, nn = (size_t)sc->n_nodes;
     hipStream_t st = srv->stream;
     const st_job_desc& d = j->desc;
+    auto& pool = srv->work_pool[d.scene_slot];
+    if (!pool.empty()) {
+        j->work = std::move(pool.back());
+        pool.pop_back();
+    } else {
+        j->work = std::make_unique<WorkBufs>();
+    }
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        if (!srv->out_pool.empty()) {
+            j->out = std::move(srv->out_pool.back());
+            srv->out_pool.pop_back();
+        }
+    }
+    if (!j->out) j->out = std::make_unique<OutBufs>();
+    WorkBufs& w = *j->work;
+    OutBufs& o = *j->out;
+    j->lists = d.num_rays >= kServerListRays;
     int rc;
-    if ((rc = j->elems.ensure(sizeof(st_element_f64) * n, "job elements")) != ST_OK) return rc;
-    if ((rc = j->aim_in.ensure(sizeof(double) * 6, "job sun")) != ST_OK) return rc;
-    if ((rc = j->ref_aabb.ensure(sizeof(double) * 6 * n, "job AABBs")) != ST_OK) return rc;
-    if ((rc = j->bounds.ensure(sizeof(float) * 6 * n, "job bounds")) != ST_OK) return rc;
-    if ((rc = j->prims.ensure(sizeof(float4) * 4 * n, "job primitives")) != ST_OK) return rc;
-    if ((rc = j->nodes.ensure(sizeof(float4) * 4 * nn, "job nodes")) != ST_OK) return rc;
-    if ((rc = j->sun.ensure(sizeof(SunFrame), "job sun frame")) != ST_OK) return rc;
+    if ((rc = w.elems.ensure(sizeof(st_element_f64) * n, "job elements")) != ST_OK) return rc;
+    if ((rc = w.aim_in.ensure(sizeof(double) * 6, "job sun")) != ST_OK) return rc;
+    if ((rc = w.ref_aabb.ensure(sizeof(double) * 6 * n, "job AABBs")) != ST_OK) return rc;
+    if ((rc = w.bounds.ensure(sizeof(float) * 6 * n, "job bounds")) != ST_OK) return rc;
+    if ((rc = w.prims.ensure(sizeof(float4) * 4 * n, "job primitives")) != ST_OK) return rc;
+    if ((rc = w.nodes.ensure(sizeof(float4) * 4 * nn, "job nodes")) != ST_OK) return rc;
+    if ((rc = w.sun.ensure(sizeof(SunFrame), "job sun frame")) != ST_OK) return rc;
+    if ((rc = w.grid_start.ensure(sizeof(uint32_t) * (kGridCells + 1), "job grid")) != ST_OK) return rc;
+    if (j->lists) {
+        if ((rc = w.frames.ensure(sizeof(double) * 9 * n, "job frames")) != ST_OK) return rc;
+        if ((rc = w.list_range.ensure(sizeof(uint2) * n, "job reflection ranges")) != ST_OK) return rc;
+        if ((rc = w.list_items.ensure(sizeof(int32_t) * kMaxBounceList * n, "job reflection lists")) != ST_OK)
+            return rc;
+    }
     const size_t nbins = (size_t)std::max(d.flux_nx, 0) * (size_t)std::max(d.flux_ny, 0);
-    if (nbins && (rc = j->flux.ensure(sizeof(int64_t) * nbins, "job flux")) != ST_OK) return rc;
-    if ((rc = j->hits.ensure(sizeof(uint64_t) * n, "job hits")) != ST_OK) return rc;
-    if ((rc = j->stats_dev.ensure(sizeof(uint64_t) * ST_STAT_COUNT, "job stats")) != ST_OK) return rc;
+    if (nbins && (rc = o.flux.ensure(sizeof(int64_t) * nbins, "job flux")) != ST_OK) return rc;
+    if ((rc = o.hits.ensure(sizeof(uint64_t) * n, "job hits")) != ST_OK) return rc;
+    if ((rc = o.stats.ensure(sizeof(uint64_t) * ST_STAT_COUNT, "job stats")) != ST_OK) return rc;
     const double l = std::sqrt(d.sun.vector[0] * d.sun.vector[0] + d.sun.vector[1] * d.sun.vector[1] +
                                d.sun.vector[2] * d.sun.vector[2]);
     double* in = j->aim_host;
@@ -122,31 +179,71 @@ int setup_job(st_server srv, ServerJob* j) {
             break;
         }
     hipError_t e;
-    if ((e = hipMemcpyAsync(j->aim_in.ptr, in, sizeof(j->aim_host), hipMemcpyHostToDevice, st)) != hipSuccess)
+    if ((e = hipMemcpyAsync(w.aim_in.ptr, in, sizeof(j->aim_host), hipMemcpyHostToDevice, st)) != hipSuccess)
         return hip_fail(e, "job sun upload");
-    if ((e = hipMemcpyAsync(j->elems.ptr, sc->d_elems.ptr, sizeof(st_element_f64) * n, hipMemcpyDeviceToDevice, st)) !=
+    if ((e = hipMemcpyAsync(w.elems.ptr, sc->d_elems.ptr, sizeof(st_element_f64) * n, hipMemcpyDeviceToDevice, st)) !=
         hipSuccess)
         return hip_fail(e, "job elements copy");
     if (!(d.flags & ST_JOB_KEEP_AIM) &&
-        (e = launch_aim(st, (st_element_f64*)j->elems.ptr, sc->n, (const double*)j->aim_in.ptr,
-                        (const double*)j->aim_in.ptr + 3)) != hipSuccess)
+        (e = launch_aim(st, (st_element_f64*)w.elems.ptr, sc->n, (const double*)w.aim_in.ptr,
+                        (const double*)w.aim_in.ptr + 3)) != hipSuccess)
         return hip_fail(e, "job aim");
-    if ((e = launch_frames(st, (const st_element_f64*)j->elems.ptr, sc->n, (const int32_t*)sc->d_elem_slot.ptr, nullptr,
-                           nullptr, nullptr, (double*)j->ref_aabb.ptr, (float*)j->bounds.ptr, (float4*)j->prims.ptr,
-                           nullptr, nullptr)) != hipSuccess)
+    if ((e = launch_frames(st, (const st_element_f64*)w.elems.ptr, sc->n, (const int32_t*)sc->d_elem_slot.ptr, nullptr,
+                           nullptr, j->lists ? (double*)w.frames.ptr : nullptr, (double*)w.ref_aabb.ptr,
+                           (float*)w.bounds.ptr, (float4*)w.prims.ptr, nullptr, nullptr)) != hipSuccess)
         return hip_fail(e, "job frames");
     if ((e = launch_refit(st, (const int32_t*)sc->d_child.ptr, (const int32_t*)sc->d_level_nodes.ptr,
                           sc->level_offsets.data(), (int32_t)sc->level_offsets.size() - 1, sc->n_nodes, sc->n, 1,
-                          (const float*)j->bounds.ptr, (float4*)j->nodes.ptr)) != hipSuccess)
+                          (const float*)w.bounds.ptr, (float4*)w.nodes.ptr)) != hipSuccess)
         return hip_fail(e, "job refit");
     st_sun_desc sun = d.sun;
     if (!(sun.dni > 0)) sun.dni = 1000.0;
-    if ((e = launch_sun_plane(st, (const double*)j->ref_aabb.ptr, sc->n, 1, (const double*)j->aim_in.ptr, &sun,
-                              (SunFrame*)j->sun.ptr)) != hipSuccess)
+    if ((e = launch_sun_plane(st, (const double*)w.ref_aabb.ptr, sc->n, 1, (const double*)w.aim_in.ptr, &sun,
+                              (SunFrame*)w.sun.ptr)) != hipSuccess)
         return hip_fail(e, "job sun plane");
-    if (nbins && (e = launch_fill_u64(st, j->flux.ptr, nbins, 0)) != hipSuccess) return hip_fail(e, "job zero");
-    if ((e = launch_fill_u64(st, j->hits.ptr, n, 0)) != hipSuccess) return hip_fail(e, "job zero");
-    if ((e = launch_fill_u64(st, j->stats_dev.ptr, ST_STAT_COUNT, 0)) != hipSuccess) return hip_fail(e, "job zero");
+    if (nbins && (e = launch_fill_u64(st, o.flux.ptr, nbins, 0)) != hipSuccess) return hip_fail(e, "job zero");
+    if ((e = launch_fill_u64(st, o.hits.ptr, n, 0)) != hipSuccess) return hip_fail(e, "job zero");
+    if ((e = launch_fill_u64(st, o.stats.ptr, ST_STAT_COUNT, 0)) != hipSuccess) return hip_fail(e, "job zero");
+    if ((e = launch_fill_u64(st, srv->d_grid_counts.ptr, kGridCells / 2, 0)) != hipSuccess)
+        return hip_fail(e, "job grid clear");
+    if ((e = launch_grid_count(st, (const float*)w.bounds.ptr, sc->n, 1, (const SunFrame*)w.sun.ptr,
+                               std::tan(sun_cone_angle(sun)), (uint32_t*)srv->d_grid_counts.ptr,
+                               (uint32_t*)w.grid_start.ptr, (uint32_t*)srv->d_grid_stats.ptr + 2 * slot, nullptr,
+                               nullptr, 0)) != hipSuccess)
+        return hip_fail(e, "job grid count");
+    return ST_OK;
+}
+
+// Second half, after the grid totals are on the host: the grid's fill pass
+// (when its lists pay), the reflection lists of long jobs, the job record.
+int finish_setup(st_server srv, ServerJob* j, uint32_t grid_total, uint32_t grid_longest) {
+    st_scene sc = j->scene;
+    hipStream_t st = srv->stream;
+    const st_job_desc& d = j->desc;
+    WorkBufs& w = *j->work;
+    OutBufs& o = *j->out;
+    st_sun_desc sun = d.sun;
+    if (!(sun.dni > 0)) sun.dni = 1000.0;
+    hipError_t e;
+    int rc;
+    const bool grid = grid_lists_pay(grid_total, grid_longest);
+    if (grid) {
+        if ((rc = w.grid_items.ensure(sizeof(int32_t) * grid_total, "job grid items")) != ST_OK) return rc;
+        if ((e = launch_fill_u64(st, srv->d_grid_cursor.ptr, kGridCells / 2, 0)) != hipSuccess)
+            return hip_fail(e, "job grid cursor clear");
+        if ((e = launch_grid_fill(st, (const float*)w.bounds.ptr, sc->n, 1, (const SunFrame*)w.sun.ptr,
+                                  std::tan(sun_cone_angle(sun)), (const uint32_t*)w.grid_start.ptr,
+                                  (uint32_t*)srv->d_grid_cursor.ptr, (int32_t*)w.grid_items.ptr, grid_total)) !=
+            hipSuccess)
+            return hip_fail(e, "job grid fill");
+    }
+    if (j->lists &&
+        (e = launch_cone_lists(st, (const st_element_f64*)w.elems.ptr, sc->n, sc->n, sc->n, sc->n_nodes,
+                               (const int32_t*)sc->d_elem_slot.ptr, (const uint32_t*)sc->d_meta.ptr,
+                               (const double*)w.frames.ptr, (const float*)w.bounds.ptr, (const float4*)w.nodes.ptr,
+                               (const SunFrame*)w.sun.ptr, sun_cone_angle(sun), nullptr, nullptr, nullptr,
+                               (uint2*)w.list_range.ptr, (int32_t*)w.list_items.ptr)) != hipSuccess)
+        return hip_fail(e, "job reflection lists");
     st_trace_desc td;
     memset(&td, 0, sizeof(td));
     td.num_rays = d.num_rays;
@@ -157,22 +254,32 @@ int setup_job(st_server srv, ServerJob* j) {
     td.flux_element = -1;
     td.flux_nx = d.flux_nx;
     td.flux_ny = d.flux_ny;
-    st_outputs o;
-    memset(&o, 0, sizeof(o));
-    o.flux = nbins ? (int64_t*)j->flux.ptr : nullptr;
-    o.element_hits = (uint64_t*)j->hits.ptr;
-    o.stats = (uint64_t*)j->stats_dev.ptr;
-    return fill_job(sc, &td, &o, j->prims.ptr, j->nodes.ptr, j->sun.ptr, &j->dev);
+    st_outputs out;
+    memset(&out, 0, sizeof(out));
+    const size_t nbins = (size_t)std::max(d.flux_nx, 0) * (size_t)std::max(d.flux_ny, 0);
+    out.flux = nbins ? (int64_t*)o.flux.ptr : nullptr;
+    out.element_hits = (uint64_t*)o.hits.ptr;
+    out.stats = (uint64_t*)o.stats.ptr;
+    if ((rc = fill_job(sc, &td, &out, w.prims.ptr, w.nodes.ptr, w.sun.ptr, &j->dev)) != ST_OK) return rc;
+    if (grid) {
+        j->dev.grid_start = (const uint32_t*)w.grid_start.ptr;
+        j->dev.grid_items = (const int32_t*)w.grid_items.ptr;
+    }
+    if (j->lists) {
+        j->dev.bounce_range = (const uint2*)w.list_range.ptr;
+        j->dev.bounce_items = (const int32_t*)w.list_items.ptr;
+    }
+    return ST_OK;
 }
 
-void release_scene_copy(ServerJob* j) {
-    j->elems.reset();
-    j->aim_in.reset();
-    j->ref_aabb.reset();
-    j->bounds.reset();
-    j->prims.reset();
-    j->nodes.reset();
-    j->sun.reset();
+// scene copy back to its scene's pool (worker thread)
+void release_scene_copy(st_server srv, ServerJob* j) {
+    if (j->work) srv->work_pool[j->desc.scene_slot].push_back(std::move(j->work));
+}
+
+// outputs back to the pool (caller holds srv->mu)
+void release_outputs(st_server srv, ServerJob* j) {
+    if (j->out) srv->out_pool.push_back(std::move(j->out));
 }
 
 void finish_job(st_server srv, ServerJob* j, int32_t status) {
@@ -202,14 +309,12 @@ void worker_main(st_server srv) {
                        srv->stats.resident_bytes + h->footprint <= srv->cfg.hbm_budget_bytes;
             });
             if (srv->stop) break;
-            // drop cancelled active jobs
+            // drop cancelled active jobs (nothing of theirs is in flight between rounds)
             for (size_t i = 0; i < srv->active.size();) {
                 ServerJob* j = srv->active[i];
                 if (j->cancel) {
-                    release_scene_copy(j);
-                    j->flux.reset();
-                    j->hits.reset();
-                    j->stats_dev.reset();
+                    release_scene_copy(srv, j);
+                    release_outputs(srv, j);
                     finish_job(srv, j, ST_ERR_CANCELLED);
                     srv->active.erase(srv->active.begin() + i);
                     srv->cv_done.notify_all();
@@ -250,36 +355,66 @@ void worker_main(st_server srv) {
             }
         }
         const double t0 = now_s();
-        for (ServerJob* j : fresh) {
-            int rc = setup_job(srv, j);
-            if (rc != ST_OK) {
-                std::lock_guard<std::mutex> lk(srv->mu);
-                release_scene_copy(j);
-                finish_job(srv, j, rc);
-                srv->active.erase(std::find(srv->active.begin(), srv->active.end(), j));
-                srv->cv_done.notify_all();
-            } else {
-                j->ready = true;
+        hipStream_t st = srv->stream;
+        // set-up of the admitted jobs: device work, one synchronisation for
+        // their grid totals, then grids, lists and job records
+        auto drop = [&](ServerJob* j, int rc) {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            release_scene_copy(srv, j);
+            release_outputs(srv, j);
+            finish_job(srv, j, rc);
+            srv->active.erase(std::find(srv->active.begin(), srv->active.end(), j));
+            srv->cv_done.notify_all();
+        };
+        std::vector<ServerJob*> staged;
+        for (size_t i = 0; i < fresh.size(); ++i) {
+            const int rc = setup_job(srv, fresh[i], (int)staged.size());
+            if (rc != ST_OK) drop(fresh[i], rc);
+            else staged.push_back(fresh[i]);
+        }
+        if (!staged.empty()) {
+            hipError_t e = hipMemcpyAsync(srv->h_grid_stats, srv->d_grid_stats.ptr, sizeof(uint32_t) * 2 * staged.size(),
+                                          hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            for (size_t i = 0; i < staged.size(); ++i) {
+                const int rc = e == hipSuccess ? finish_setup(srv, staged[i], srv->h_grid_stats[2 * i],
+                                                              srv->h_grid_stats[2 * i + 1])
+                                               : hip_fail(e, "job set-up");
+                if (rc != ST_OK) drop(staged[i], rc);
+                else staged[i]->ready = true;
             }
         }
-        // one launch over a slice of every active job
-        std::vector<ServerJob*> round;
-        uint64_t round_rays = 0;
+        // one launch over slices of the active jobs: shortest remaining first,
+        // at most chunk_rays per job and round_rays in all
+        std::vector<ServerJob*> cand;
         {
             std::lock_guard<std::mutex> lk(srv->mu);
             for (ServerJob* j : srv->active)
-                if (j->ready && j->dispatched < j->desc.num_rays) round.push_back(j);
+                if (j->ready && j->dispatched < j->desc.num_rays) cand.push_back(j);
         }
-        if (round.empty()) continue;
-        std::vector<uint64_t> slice(round.size());
-        for (size_t i = 0; i < round.size(); ++i) {
-            slice[i] = std::min<uint64_t>(srv->cfg.chunk_rays, round[i]->desc.num_rays - round[i]->dispatched);
-            round_rays += slice[i];
+        if (cand.empty()) continue;
+        std::stable_sort(cand.begin(), cand.end(), [](const ServerJob* x, const ServerJob* y) {
+            return x->desc.num_rays - x->dispatched < y->desc.num_rays - y->dispatched;
+        });
+        std::vector<ServerJob*> round;
+        std::vector<uint64_t> slice;
+        uint64_t round_rays = 0;
+        const uint64_t budget = srv->cfg.round_rays;
+        for (ServerJob* j : cand) {
+            if (round_rays >= budget) break;
+            const uint64_t sl = std::min<uint64_t>({srv->cfg.chunk_rays, j->desc.num_rays - j->dispatched,
+                                                    budget - round_rays});
+            round.push_back(j);
+            slice.push_back(sl);
+            round_rays += sl;
         }
-        int32_t depth = 0;
-        for (ServerJob* j : round) depth = std::max(depth, j->scene->topo.depth);
-        const LaunchShape sh =
-            pick_shape(round_rays, srv->cfg.block_size, srv->cfg.grid_blocks, ST_KERNEL_AUTO, depth, nullptr);
+        int32_t depth = 0, max_n = 0;
+        for (ServerJob* j : round) {
+            depth = std::max(depth, j->scene->topo.depth);
+            max_n = std::max(max_n, j->scene->n);
+        }
+        const LaunchShape sh = pick_shape(round_rays, srv->cfg.block_size, srv->cfg.grid_blocks, ST_KERNEL_AUTO, depth,
+                                          nullptr, max_n);
         srv->h_prefix[0] = 0;
         for (size_t i = 0; i < round.size(); ++i) {
             JobDev J = round[i]->dev;
@@ -289,7 +424,6 @@ void worker_main(st_server srv) {
             srv->h_jobs[i] = J;
             srv->h_prefix[i + 1] = srv->h_prefix[i] + (slice[i] + sh.chunk - 1) / sh.chunk;
         }
-        hipStream_t st = srv->stream;
         hipError_t e = hipMemcpyAsync(srv->d_jobs.ptr, srv->h_jobs, sizeof(JobDev) * round.size(),
                                       hipMemcpyHostToDevice, st);
         if (e == hipSuccess)
@@ -301,21 +435,22 @@ void worker_main(st_server srv) {
             e = launch_trace(st, (const JobDev*)srv->d_jobs.ptr, (const uint64_t*)srv->d_prefix.ptr,
                              (int32_t)round.size(), srv->h_prefix[round.size()], (uint32_t*)srv->d_counter.ptr, sh,
                              dummy);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        // retire finished jobs
+        // statistics and sun frames of the jobs this round completes
         std::vector<ServerJob*> finished;
+        std::vector<SunFrame> frames;
         for (size_t i = 0; i < round.size(); ++i) {
             round[i]->dispatched += slice[i];
             if (round[i]->dispatched >= round[i]->desc.num_rays) finished.push_back(round[i]);
         }
-        for (ServerJob* j : finished) {
-            if (e != hipSuccess) break;
-            SunFrame F;
-            e = hipMemcpyAsync(j->stats, j->stats_dev.ptr, sizeof(j->stats), hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipMemcpyAsync(&F, j->sun.ptr, sizeof(F), hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
-            if (e == hipSuccess) j->power_per_ray = F.dni * F.area / (double)j->desc.num_rays;
+        frames.resize(finished.size());
+        for (size_t i = 0; i < finished.size() && e == hipSuccess; ++i) {
+            ServerJob* j = finished[i];
+            e = hipMemcpyAsync(j->stats, j->out->stats.ptr, sizeof(j->stats), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(&frames[i], j->work->sun.ptr, sizeof(SunFrame), hipMemcpyDeviceToHost, st);
         }
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        for (size_t i = 0; i < finished.size() && e == hipSuccess; ++i)
+            finished[i]->power_per_ray = frames[i].dni * frames[i].area / (double)finished[i]->desc.num_rays;
         std::lock_guard<std::mutex> lk(srv->mu);
         srv->stats.launches++;
         srv->stats.rays_traced += round_rays;
@@ -323,7 +458,8 @@ void worker_main(st_server srv) {
         if (e != hipSuccess) {
             srv->worker_error = hipGetErrorString(e);
             for (ServerJob* j : srv->active) {
-                release_scene_copy(j);
+                release_scene_copy(srv, j);
+                release_outputs(srv, j);
                 finish_job(srv, j, ST_ERR_DEVICE);
             }
             srv->active.clear();
@@ -331,7 +467,7 @@ void worker_main(st_server srv) {
             continue;
         }
         for (ServerJob* j : finished) {
-            release_scene_copy(j);
+            release_scene_copy(srv, j);
             finish_job(srv, j, ST_OK);
             srv->active.erase(std::find(srv->active.begin(), srv->active.end(), j));
         }
@@ -372,6 +508,7 @@ extern "C" int st_server_create(st_context ctx, const st_server_config* cfg, st_
     srv->ctx_ref.hold(ctx);
     srv->cfg = *cfg;
     if (srv->cfg.chunk_rays == 0) srv->cfg.chunk_rays = 8u << 20;
+    if (srv->cfg.round_rays == 0) srv->cfg.round_rays = 4 * srv->cfg.chunk_rays;
     HIP_TRY(hipStreamCreateWithFlags(&srv->stream, hipStreamNonBlocking), "server stream");
     HIP_TRY(hipStreamCreateWithFlags(&srv->fetch_stream, hipStreamNonBlocking), "server fetch stream");
     const size_t J = (size_t)srv->cfg.max_active_jobs;
@@ -382,6 +519,10 @@ extern "C" int st_server_create(st_context ctx, const st_server_config* cfg, st_
     HIP_TRY(hipMemset(srv->d_counter.ptr, 0, sizeof(uint32_t) * 2), "counter clear");
     HIP_TRY(hipHostMalloc((void**)&srv->h_jobs, sizeof(JobDev) * J, hipHostMallocDefault), "pinned jobs");
     HIP_TRY(hipHostMalloc((void**)&srv->h_prefix, sizeof(uint64_t) * (J + 1), hipHostMallocDefault), "pinned prefix");
+    HIP_TRY(hipHostMalloc((void**)&srv->h_grid_stats, sizeof(uint32_t) * 2 * J, hipHostMallocDefault), "pinned grid stats");
+    if ((rc = srv->d_grid_counts.ensure(sizeof(uint32_t) * kGridCells, "server grid counts")) != ST_OK) return rc;
+    if ((rc = srv->d_grid_cursor.ensure(sizeof(uint32_t) * kGridCells, "server grid cursor")) != ST_OK) return rc;
+    if ((rc = srv->d_grid_stats.ensure(sizeof(uint32_t) * 2 * J, "server grid stats")) != ST_OK) return rc;
     HIP_TRY(hipDeviceSynchronize(), "server init sync");
     st_server s = srv.release();
     s->worker = std::thread(worker_main, s);
@@ -402,6 +543,7 @@ extern "C" int st_server_destroy(st_server srv) {
     srv->jobs.clear();
     if (srv->h_jobs) (void)hipHostFree(srv->h_jobs);
     if (srv->h_prefix) (void)hipHostFree(srv->h_prefix);
+    if (srv->h_grid_stats) (void)hipHostFree(srv->h_grid_stats);
     (void)hipStreamDestroy(srv->stream);
     (void)hipStreamDestroy(srv->fetch_stream);
     delete srv;
@@ -441,7 +583,7 @@ extern "C" int st_server_submit(st_server srv, const st_job_desc* job, int64_t* 
     j->scene = sc;
     j->submit_s = now_s();
     j->footprint = job_footprint(sc, *job);
-    j->scene_bytes = job_scene_bytes(sc);
+    j->scene_bytes = job_scene_bytes(sc, *job);
     ServerJob* raw = j.get();
     srv->jobs[j->handle] = std::move(j);
     srv->pending.push_back(raw);
@@ -485,10 +627,10 @@ extern "C" int st_server_fetch(st_server srv, int64_t h, int64_t* flux_host, uin
     HIP_TRY(hipSetDevice(srv->ctx->device), "hipSetDevice");
     const size_t nbins = (size_t)std::max(j->desc.flux_nx, 0) * (size_t)std::max(j->desc.flux_ny, 0);
     if (flux_host && nbins)
-        HIP_TRY(hipMemcpyAsync(flux_host, j->flux.ptr, nbins * sizeof(int64_t), hipMemcpyDeviceToHost, srv->fetch_stream),
+        HIP_TRY(hipMemcpyAsync(flux_host, j->out->flux.ptr, nbins * sizeof(int64_t), hipMemcpyDeviceToHost, srv->fetch_stream),
                 "fetch flux");
     if (hits_host)
-        HIP_TRY(hipMemcpyAsync(hits_host, j->hits.ptr, (size_t)j->scene->n * sizeof(uint64_t), hipMemcpyDeviceToHost,
+        HIP_TRY(hipMemcpyAsync(hits_host, j->out->hits.ptr, (size_t)j->scene->n * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                srv->fetch_stream),
                 "fetch hits");
     HIP_TRY(hipStreamSynchronize(srv->fetch_stream), "fetch sync");
@@ -516,9 +658,10 @@ extern "C" int st_server_release(st_server srv, int64_t h) {
         if (it->second->status == ST_OK) srv->stats.resident_bytes -= (it->second->footprint - it->second->scene_bytes);
         j = std::move(it->second);
         srv->jobs.erase(it);
-        srv->cv_work.notify_all();  // memory freed: admission may proceed
+        release_outputs(srv, j.get());  // the output buffers serve later jobs
+        srv->cv_work.notify_all();      // memory freed: admission may proceed
     }
-    return ST_OK;  // device buffers freed by ~ServerJob
+    return ST_OK;
 }
 
 extern "C" int st_server_get_stats(st_server srv, st_server_stats* out) {

@@ -18,10 +18,10 @@ ST_JOB_KEEP_AIM = 0x100
 
 class Server:
     def __init__(self, ctx: tracer.Context, hbm_budget_bytes: int = 64 << 30, max_active_jobs: int = 64,
-                 chunk_rays: int = 8 << 20, block_size: int = 0, grid_blocks: int = 0):
+                 chunk_rays: int = 8 << 20, block_size: int = 0, grid_blocks: int = 0, round_rays: int = 0):
         cfg = ServerConfig()
         cfg.hbm_budget_bytes, cfg.max_active_jobs, cfg.chunk_rays = int(hbm_budget_bytes), int(max_active_jobs), int(chunk_rays)
-        cfg.block_size, cfg.grid_blocks = int(block_size), int(grid_blocks)
+        cfg.block_size, cfg.grid_blocks, cfg.round_rays = int(block_size), int(grid_blocks), int(round_rays)
         h = C.c_void_p()
         call("st_server_create", ctx.h, C.byref(cfg), C.byref(h))
         self.h, self.ctx, self.scenes = h, ctx, []

@@ -50,8 +50,32 @@ def test_results_identical_alone_and_in_the_mix(gpu_ctx, mix):
     srv.close()
 
 
+def test_long_job_with_lists_identical_alone(gpu_ctx, mix):
+    """A job of 2^24 rays or more gets its own sun-plane grid and first-reflection
+    lists on the server; served among short jobs it equals the job traced alone."""
+    _, w, sc = mix[3]
+    srv = serving.Server(gpu_ctx, max_active_jobs=8, chunk_rays=1 << 22, round_rays=6 << 20)
+    slot = srv.add_scene(sc)
+    sun = scenes.sun_vector(38.0, 150.0)
+    rays = (1 << 24) + 12_345
+    long_job = srv.submit(slot, sun, rays, seed=9, job_id=77, flux_nx=64, flux_ny=32)
+    short = [srv.submit(slot, scenes.sun_vector(50.0 + k, 200.0), 300_000 + k, seed=9, job_id=k, flux_nx=64,
+                        flux_ny=32) for k in range(5)]
+    r = srv.wait(long_job)
+    assert r["status"] == 0 and all(srv.wait(h)["status"] == 0 for h in short)
+    flux, hits = srv.fetch(long_job, len(w.elements), (32, 64))
+    ref, _ = alone(gpu_ctx, w, sun, rays, 9, 77, (64, 32))
+    assert np.array_equal(r["stats"], ref["stats"])
+    assert np.array_equal(flux, ref["flux"])
+    assert np.array_equal(hits, ref["element_hits"])
+    assert srv.stats()["launches"] >= 3
+    for h in short + [long_job]:
+        srv.release(h)
+    srv.close()
+
+
 def test_admission_by_hbm_footprint(gpu_ctx, mix):
-    _, w, sc = mix[3]                                         # 10k field: ~4 MB per job
+    _, w, sc = mix[3]                                         # 10k field: ~5 MB per job
     srv = serving.Server(gpu_ctx, hbm_budget_bytes=6 << 20, max_active_jobs=8)
     slot = srv.add_scene(sc)
     hs = [srv.submit(slot, w.sun_vector, 200_000, job_id=k, flux_nx=64, flux_ny=32) for k in range(3)]
