@@ -45,9 +45,12 @@ struct JobDev {
 
 constexpr int kGridRes = 256;  // sun-plane candidate grid, the stratified-sampling cells
 constexpr int kGridCells = kGridRes * kGridRes;
-// a grid is used when its lists are short: beyond these averages or lengths
-// the BVH finds a sun ray's hit with fewer tests
-constexpr uint32_t kGridMaxAverage = 6, kGridMaxLongest = 64;
+// a grid is used unless its lists grow long: a wave whose rays share a cell
+// tests its list in lockstep at ~40 vector instructions per entry, while a
+// BVH walk costs each lane ~40 per node over ~20 nodes of the 10k field, so
+// even lists of a few hundred entries beat the walk; the average bounds the
+// grid's memory (kGridMaxAverage * kGridCells items)
+constexpr uint32_t kGridMaxAverage = 16, kGridMaxLongest = 512;
 __host__ __device__ inline bool grid_lists_pay(uint32_t total, uint32_t longest) {
     return total > 0 && total <= kGridMaxAverage * (uint32_t)kGridCells && longest <= kGridMaxLongest;
 }

@@ -1092,16 +1092,21 @@ __device__ __forceinline__ bool grid_range(const float* __restrict__ bounds, int
     return r[0] <= r[1] && r[2] <= r[3];
 }
 
+// kGridParts threads per (step, primitive): part p visits every kGridParts-th
+// cell of the primitive's footprint, so large footprints do not serialize
+constexpr int kGridParts = 16;
 __global__ void k_grid_count(const float* __restrict__ bounds, int32_t n, int32_t steps,
                              const SunFrame* __restrict__ sun, double tan_max, uint32_t* __restrict__ counts) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (int64_t)n * steps) return;
-    const int k = (int)(g / n), slot = (int)(g % n);
+    if (g >= (int64_t)n * steps * kGridParts) return;
+    const int part = (int)(g % kGridParts);
+    const int64_t e = g / kGridParts;
+    const int k = (int)(e / n), slot = (int)(e % n);
     int r[4];
     if (!grid_range(bounds + (size_t)k * n * 6, slot, sun[k], tan_max, r)) return;
     uint32_t* C = counts + (size_t)k * kGridCells;
-    for (int b = r[2]; b <= r[3]; ++b)
-        for (int a = r[0]; a <= r[1]; ++a) atomicAdd(&C[b * kGridRes + a], 1u);
+    const int w = r[1] - r[0] + 1, cells = w * (r[3] - r[2] + 1);
+    for (int i = part; i < cells; i += kGridParts) atomicAdd(&C[(r[2] + i / w) * kGridRes + r[0] + i % w], 1u);
 }
 
 // exclusive prefix sum of the kGridRes^2 counts of step blockIdx.x (one block
@@ -1158,19 +1163,21 @@ __global__ void k_grid_fill(const float* __restrict__ bounds, int32_t n, int32_t
                             const SunFrame* __restrict__ sun, double tan_max, const uint32_t* __restrict__ start_all,
                             uint32_t* __restrict__ cursor_all, int32_t* __restrict__ items_all, uint32_t item_cap) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (int64_t)n * steps) return;
-    const int k = (int)(g / n), slot = (int)(g % n);
+    if (g >= (int64_t)n * steps * kGridParts) return;
+    const int part = (int)(g % kGridParts);
+    const int64_t e = g / kGridParts;
+    const int k = (int)(e / n), slot = (int)(e % n);
     const uint32_t* start = start_all + (size_t)k * (kGridCells + 1);
     if (start[kGridCells] > item_cap) return;
     int r[4];
     if (!grid_range(bounds + (size_t)k * n * 6, slot, sun[k], tan_max, r)) return;
     uint32_t* cursor = cursor_all + (size_t)k * kGridCells;
     int32_t* items = items_all + (size_t)k * item_cap;
-    for (int b = r[2]; b <= r[3]; ++b)
-        for (int a = r[0]; a <= r[1]; ++a) {
-            const int c = b * kGridRes + a;
-            items[start[c] + atomicAdd(&cursor[c], 1u)] = slot;
-        }
+    const int w = r[1] - r[0] + 1, cells = w * (r[3] - r[2] + 1);
+    for (int i = part; i < cells; i += kGridParts) {
+        const int c = (r[2] + i / w) * kGridRes + r[0] + i % w;
+        items[start[c] + atomicAdd(&cursor[c], 1u)] = slot;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1578,8 +1585,8 @@ hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk
 hipError_t launch_grid_count(hipStream_t s, const float* bounds, int32_t n, int32_t steps, const SunFrame* sun,
                              double tan_max, uint32_t* counts, uint32_t* start, uint32_t* stats, JobDev* jobs,
                              const int32_t* items, uint32_t item_cap) {
-    const int64_t total = (int64_t)n * steps;
-    hipLaunchKernelGGL(k_grid_count, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, bounds, n, steps, sun,
+    const int64_t total = (int64_t)n * steps * kGridParts;
+    hipLaunchKernelGGL(k_grid_count, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, bounds, n, steps, sun,
                        tan_max, counts);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -1589,8 +1596,8 @@ hipError_t launch_grid_count(hipStream_t s, const float* bounds, int32_t n, int3
 
 hipError_t launch_grid_fill(hipStream_t s, const float* bounds, int32_t n, int32_t steps, const SunFrame* sun,
                             double tan_max, const uint32_t* start, uint32_t* cursor, int32_t* items, uint32_t item_cap) {
-    const int64_t total = (int64_t)n * steps;
-    hipLaunchKernelGGL(k_grid_fill, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, bounds, n, steps, sun,
+    const int64_t total = (int64_t)n * steps * kGridParts;
+    hipLaunchKernelGGL(k_grid_fill, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, bounds, n, steps, sun,
                        tan_max, start, cursor, items, item_cap);
     return hipGetLastError();
 }
