@@ -21,9 +21,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "traced rays/sec at 10k-heliostat field + p50/p99 job latency, 1 MI355X"
-# VALU issue peak of MI355X: 256 CU x 128 lane-ops per clock x 2.4 GHz = the
-# published FP32 vector peak 157.3 TFLOP/s with an FMA counted as one op (a
-# plain 32-bit add/xor/fma chain issues at ~71e12 lane-ops/s on the card)
+# MI355X published FP32 vector peak, 157.3 TFLOP/s, in lane operations (an FMA
+# counts one): 256 CU x 128 lane-ops per clock x 2.4 GHz = 78.6e12. That
+# figure counts packed FP32 (two FMAs per lane and cycle); this build issues
+# plain FP32 (packed FP32 is disabled), whose rate - half of it, ~39e12 - is
+# measured live by st_calibrate_valu and reported beside it.
 VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
 # Algorithmic lane operations per event (add/mul/fma/min/max/cmp/select/rcp/
 # rsqrt/sqrt = 1, a 32x32->64 multiply = 1, sincos/sincospi = 20, atan2 = 30),
@@ -161,6 +163,7 @@ def main():
     assert np.array_equal(work[:11], stats[:11] // calls), "work-count launch must trace the same rays"
     w_ops = ops_per_ray(work)
     achieved = w_ops * rays / (kernel_ms * 1e-3) / 1e12
+    plain_peak = ctx.calibrate_valu() / 1e12
     line = {
         "metric": METRIC,
         "value": value,
@@ -180,6 +183,7 @@ def main():
                    "seed": args.seed, "sampling": args.sampling, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s",
                      "frac": achieved / VALU_PEAK_TOPS, "traffic": read_traffic(),
+                     "peak_plain_fp32_measured": plain_peak, "frac_of_plain_fp32": achieved / plain_peak,
                      "ops_per_ray": w_ops, "kernel_ms": kernel_ms,
                      "work_per_ray": {"segments": float(work[1]) / rays, "node_visits": float(work[11]) / rays,
                                       "flat_tests": float(work[12]) / rays,

@@ -329,6 +329,13 @@ typedef struct st_server_stats {
 } st_server_stats;
 int st_server_get_stats(st_server srv, st_server_stats* stats);
 
+/* ---- measurement --------------------------------------------------------- */
+/* No counterpart in the reference. Plain (non-packed) FP32 vector-instruction
+ * issue rate of the context's device, from independent FMA chains: the roofline
+ * the trace kernel is measured against (it runs in plain FP32; the published
+ * FP32 peak counts packed two-wide FMAs). Lane operations per second. */
+int st_calibrate_valu(st_context ctx, double* lane_ops_per_s);
+
 /* ---- testing hooks ------------------------------------------------------ */
 /* Make the next `count` device allocations (or launches, what=1) fail with
  * ST_ERR_INJECTED; host-side only, no kernel ever faults. */

@@ -235,6 +235,7 @@ SIGNATURES = {
     "st_server_cancel": (C.c_int, [P, C.c_int64]),
     "st_server_release": (C.c_int, [P, C.c_int64]),
     "st_server_get_stats": (C.c_int, [P, C.POINTER(ServerStats)]),
+    "st_calibrate_valu": (C.c_int, [P, C.POINTER(C.c_double)]),
     "st_debug_inject_fault": (C.c_int, [C.c_int32, C.c_int32]),
 }
 

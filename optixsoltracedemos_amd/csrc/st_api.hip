@@ -1138,6 +1138,35 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
     return ST_OK;
 }
 
+extern "C" int st_calibrate_valu(st_context ctx, double* lane_ops_per_s) {
+    if (!ctx || !lane_ops_per_s) return fail(ST_ERR_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+    const int blocks = device_cu_count() * 16;
+    DevBuf out;
+    ST_TRY(out.ensure(sizeof(float) * 256 * (size_t)blocks, "calibration output"));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "event");
+    HIP_TRY(hipEventCreate(&e1), "event");
+    double ops = 0.0;
+    float best = 1e30f;
+    int rc = ST_OK;
+    for (int rep = 0; rep < 5 && rc == ST_OK; ++rep) {  // the first launch warms up
+        hipError_t e = hipEventRecord(e0, ctx->stream);
+        if (e == hipSuccess) e = launch_valu_calibrate(ctx->stream, (float*)out.ptr, blocks, &ops);
+        if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) rc = hip_fail(e, "VALU calibration");
+        else if (rep > 0) best = std::min(best, ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != ST_OK) return rc;
+    *lane_ops_per_s = ops / (best * 1e-3);
+    return ST_OK;
+}
+
 extern "C" int st_debug_inject_fault(int32_t what, int32_t count) {
     if (count < 0) return fail(ST_ERR_INVALID, "negative count");
     if (what == 0) g_fail_alloc.store(count);

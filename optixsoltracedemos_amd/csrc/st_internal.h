@@ -134,5 +134,8 @@ hipError_t launch_flux_watts(hipStream_t s, const int64_t* flux, int32_t nbins, 
                              double num_rays, double bin_area, double* out, double* power_per_ray_dev);
 
 int device_cu_count();
+// one launch of the VALU calibration kernel over `blocks` blocks of 256 lanes
+// (out: blocks * 256 floats); *lane_ops = FMAs it issues
+hipError_t launch_valu_calibrate(hipStream_t s, float* out, int blocks, double* lane_ops);
 
 }  // namespace st

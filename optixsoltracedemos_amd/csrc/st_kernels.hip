@@ -1436,6 +1436,30 @@ __global__ void k_flux_watts(const int64_t* __restrict__ flux, int32_t nbins, co
 }
 
 // ---------------------------------------------------------------------------
+// VALU issue calibration: kCalChains independent FP32 FMA chains per lane
+// (plain v_fma_f32; packed FP32 is disabled in this build), so the issue rate
+// of plain vector instructions bounds the time.
+constexpr int kCalChains = 8, kCalIters = 4096;
+__global__ __launch_bounds__(256) void k_valu_calibrate(float* __restrict__ out, float a, float b) {
+    float x[kCalChains];
+#pragma unroll
+    for (int c = 0; c < kCalChains; ++c) x[c] = (float)threadIdx.x * 1e-3f + (float)c;
+    for (int i = 0; i < kCalIters; ++i) {
+#pragma unroll
+        for (int c = 0; c < kCalChains; ++c) x[c] = fmaf(x[c], a, b);
+    }
+    float sum = 0.f;
+#pragma unroll
+    for (int c = 0; c < kCalChains; ++c) sum += x[c];
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = sum;
+}
+
+hipError_t launch_valu_calibrate(hipStream_t s, float* out, int blocks, double* lane_ops) {
+    hipLaunchKernelGGL(k_valu_calibrate, dim3(blocks), dim3(256), 0, s, out, 1.0001f, 0.5f);
+    *lane_ops = (double)blocks * 256.0 * kCalIters * kCalChains;
+    return hipGetLastError();
+}
+
 int device_cu_count() {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 256;

@@ -33,6 +33,12 @@ class Context:
             call("st_context_destroy", self.h)
             self.h = C.c_void_p()
 
+    def calibrate_valu(self) -> float:
+        """Measured plain FP32 vector issue rate of the device, lane operations per second."""
+        v = C.c_double()
+        call("st_calibrate_valu", self.h, C.byref(v))
+        return v.value
+
     def __del__(self):
         try:
             self.close()

@@ -362,6 +362,13 @@ def test_batch_lists_equal_bvh(gpu_ctx, sampling):
     assert listed < 0.25 * walked
 
 
+def test_valu_calibration(gpu_ctx):
+    """Plain FP32 issue rate of the device: between a tenth and the whole of the
+    published FP32 vector peak (78.6e12 FMA lane-ops/s counting packed FP32)."""
+    rate = gpu_ctx.calibrate_valu()
+    assert 7.8e12 < rate < 80e12, rate
+
+
 def test_invalid_trace_requests_are_rejected(gpu_ctx):
     w = scenes.three_heliostats()
     sc = tracer.Scene(gpu_ctx, w.elements)
