@@ -444,13 +444,13 @@ __device__ __forceinline__ void traverse(const float4* __restrict__ nodes, const
 
 // ---------------------------------------------------------------------------
 
-// base[idx] += 1 for every lane that calls it (lanes may diverge around the
-// call). The calling lanes are grouped by index, one device-scope atomic per
-// group, for up to ROUNDS groups; lanes left after that add one each. Coherent
-// waves share few indices (the mirrors a wave's rays hit), so most calls end
-// after a round or two.
-template <int ROUNDS>
-__device__ __forceinline__ void add_one_grouped(unsigned long long* base, uint32_t idx) {
+// emit(idx, n) for every group of the calling lanes that share an index, with
+// n = the group's size, for up to ROUNDS groups; lanes left after that emit
+// (idx, 1) each (lanes may diverge around the call). Coherent waves share few
+// indices (the mirrors a wave's rays hit), so most calls end after a round or
+// two.
+template <int ROUNDS, class Emit>
+__device__ __forceinline__ void add_one_grouped(uint32_t idx, Emit emit) {
     const int lane = (int)(threadIdx.x & 63);
     bool done = false;
     for (int r = 0; r < ROUNDS; ++r) {
@@ -461,11 +461,40 @@ __device__ __forceinline__ void add_one_grouped(unsigned long long* base, uint32
             const uint32_t lidx = __builtin_amdgcn_readfirstlane(idx);
             const bool mine = idx == lidx;
             const unsigned long long same = __ballot(mine);
-            if (lane == __ffsll((long long)__ballot(1)) - 1) gadd(base + lidx, (unsigned long long)__popcll(same));
+            if (lane == __ffsll((long long)__ballot(1)) - 1) emit(lidx, (uint32_t)__popcll(same));
             done = mine;
         }
     }
-    if (!done) gadd(base + idx, 1ull);
+    if (!done) emit(idx, 1u);
+}
+
+// Per-wave log of pending output atomics in LDS: word 0 = entries, then one
+// u32 per entry: kind << 31 (1 = flux bin, 0 = element hit) | (count - 1) << 25
+// | index. On this architecture a wave's vector-memory counter retires loads,
+// stores and atomics in issue order, so an atomic sent to the memory-side
+// units from inside the trace would hold up the wave's next load; logged, the
+// atomics go out in bursts (kLogCap - kLogHeadroom entries or at the chunk's
+// end), each a full-width vector atomic. One trace step adds at most
+// kLogHeadroom entries (a lane adds at most two).
+constexpr uint32_t kLogCap = 320, kLogHeadroom = 128;
+__device__ __forceinline__ void log_add(uint32_t* log, const JobDev& J, bool flux, uint32_t idx, uint32_t n) {
+    if (idx < (1u << 25) && n <= 64u) {
+        const uint32_t pos = atomicAdd(log, 1u);
+        if (pos < kLogCap) {
+            log[1 + pos] = ((uint32_t)flux << 31) | ((n - 1u) << 25) | idx;
+            return;
+        }
+    }
+    gadd(flux ? (void*)(J.flux + idx) : (void*)(J.hits + idx), (unsigned long long)n);
+}
+// every lane of the wave calls it (uniform control flow)
+__device__ __forceinline__ void log_flush(uint32_t* log, const JobDev& J) {
+    const uint32_t n = min(log[0], kLogCap);
+    for (uint32_t i = threadIdx.x & 63; i < n; i += 64) {
+        const uint32_t e = log[1 + i], idx = e & 0x1FFFFFFu, cnt = ((e >> 25) & 63u) + 1u;
+        gadd((e >> 31) ? (void*)(J.flux + idx) : (void*)(J.hits + idx), (unsigned long long)cnt);
+    }
+    if ((threadIdx.x & 63) == 0) log[0] = 0;
 }
 
 __device__ __forceinline__ void record(const JobDev& J, uint64_t ray, int slot, float tag, V3 p, uint32_t elem) {
@@ -530,8 +559,8 @@ __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame
 // the next segment), false when it ended.
 template <int STRIDE>
 __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int32_t* stk, uint32_t* wc,
-                                              uint32_t* wh, V3& o, V3& d, float& tmin, int& depth, int& last,
-                                              SegList L, int& key) {
+                                              uint32_t* wh, uint32_t* wl, V3& o, V3& d, float& tmin, int& depth,
+                                              int& last, SegList L, int& key) {
     const uint32_t rlo = (uint32_t)ray, rhi = (uint32_t)(ray >> 32);
     count(wc, ST_STAT_SEGMENTS);
     float t = kTmax;
@@ -565,7 +594,7 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
             count_if(wc, kAggCounter, slot == J.agg_slot);
             if (slot != J.agg_slot && J.hits) {
                 if (wh && elem < (uint32_t)kLdsHitElements) atomicAdd(wh + elem, 1u);
-                else gadd(J.hits + elem, 1ull);
+                else log_add(wl, J, false, elem, 1u);
             }
             count(wc, ST_STAT_RECEIVER_HITS);
             const bool direct = depth == 0;
@@ -590,7 +619,8 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
                 int ix = (int)floorf(fx), iy = (int)floorf(fy);
                 ix = min(max(ix, 0), J.flux_nx - 1);
                 iy = min(max(iy, 0), J.flux_ny - 1);
-                add_one_grouped<2>((unsigned long long*)J.flux, (uint32_t)(iy * J.flux_nx + ix));
+                add_one_grouped<2>((uint32_t)(iy * J.flux_nx + ix),
+                                   [&](uint32_t b, uint32_t n) { log_add(wl, J, true, b, n); });
                 count(wc, ST_STAT_FLUX_BINNED);
             }
         } else {
@@ -607,7 +637,7 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
     last = nd;
     if (J.hits) {
         if (wh && elem < (uint32_t)kLdsHitElements) atomicAdd(wh + elem, 1u);
-        else add_one_grouped<64>((unsigned long long*)J.hits, elem);
+        else add_one_grouped<64>(elem, [&](uint32_t e, uint32_t n) { log_add(wl, J, false, e, n); });
     }
     count(wc, ST_STAT_MIRROR_HITS);
     // draws of bounce nd: block 2nd-1 = (reflectivity, slope xr, slope xp,
@@ -673,31 +703,18 @@ __device__ __forceinline__ void finish_ray(const JobDev& J, uint64_t ray, int la
 // traced in full waves of 64 continuing rays (the rest at the chunk's end).
 // Both give identical results: every per-ray quantity depends on the ray index
 // and its own path only, and all accumulations are integer sums.
-constexpr int kQueueCap = 128;  // <= 63 waiting + 64 newly parked
-
-// park the lanes with `cont` set at queue[qn ...]; returns the new count
-__device__ __forceinline__ uint32_t queue_push(float* q, uint32_t qn, bool cont, V3 o, V3 d, uint32_t tag) {
-    const unsigned long long m = __ballot(cont);
-    if (cont) {
-        const uint32_t i = qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        q[0 * kQueueCap + i] = o.x;
-        q[1 * kQueueCap + i] = o.y;
-        q[2 * kQueueCap + i] = o.z;
-        q[3 * kQueueCap + i] = d.x;
-        q[4 * kQueueCap + i] = d.y;
-        q[5 * kQueueCap + i] = d.z;
-        q[6 * kQueueCap + i] = __uint_as_float(tag);
-    }
-    return qn + (uint32_t)__popcll(m);
-}
+constexpr int kQueueCap = 64;  // < 64 parked between steps (lanes refill from the queue first)
 
 // tag of a parked ray: offset in the chunk (< 2^11) | depth << 11 | (key + 1)
 // << 15, key = the list key of its next segment (0: none, or a key >= 2^17 - 1,
 // whose segment is then traced through the BVH). A parked ray's last record
 // slot is its depth.
-__device__ __forceinline__ uint32_t queue_tag(uint64_t offset, int depth, int key) {
+__device__ __forceinline__ uint32_t queue_tag(uint32_t offset, int depth, int key) {
     const uint32_t f = (uint32_t)(key + 1) < (1u << 17) ? (uint32_t)(key + 1) : 0u;
-    return (uint32_t)offset | ((uint32_t)depth << 11) | (f << 15);
+    return offset | ((uint32_t)depth << 11) | (f << 15);
+}
+__device__ __forceinline__ uint32_t lane_rank(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 // LHITS > 0: element hit counts of scenes with at most LHITS elements are
@@ -710,12 +727,15 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
     __shared__ int32_t lds_stack[STACK * BLOCK];
     __shared__ uint32_t lds_counters[BLOCK / 64][kNumCounters];
     __shared__ float lds_queue[QUEUED ? BLOCK / 64 : 1][QUEUED ? 7 * kQueueCap : 1];
+    __shared__ uint32_t lds_log[BLOCK / 64][1 + kLogCap];
     __shared__ uint32_t lds_hits[LHITS > 0 ? BLOCK / 64 : 1][LHITS > 0 ? LHITS : 1];
     int32_t* stk = lds_stack + threadIdx.x;
     const int lane = threadIdx.x & 63;
     uint32_t* wc = lds_counters[threadIdx.x >> 6];
     if (lane < kNumCounters) wc[lane] = 0;
     uint32_t* wh = LHITS > 0 ? lds_hits[threadIdx.x >> 6] : nullptr;
+    uint32_t* wl = lds_log[threadIdx.x >> 6];
+    if (lane == 0) wl[0] = 0;
     if (LHITS > 0)
         for (int i = lane; i < LHITS; i += 64) wh[i] = 0;
     const uint32_t waves = gridDim.x * (BLOCK / 64);
@@ -749,65 +769,110 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
         const uint64_t end = min(first + (uint64_t)chunk_rays, J.ray_count);
         if (QUEUED) {
             float* q = lds_queue[threadIdx.x >> 6];
-            uint32_t qn = 0;  // wave-uniform queue length
+            uint32_t qn = 0;  // wave-uniform queue length, < kQueueCap between steps
             const uint64_t base_ray = J.ray_begin + first;
-            // one full (or, at the end, partial) wave of parked rays
-            auto trace_parked = [&](uint32_t take) {
+            bool alive = false;  // this lane holds a ray with a next segment
+            V3 o = v3(0.f, 0.f, 0.f), d = o;
+            uint32_t off = 0;    // its offset in the chunk
+            int depth = 0, key = -1;
+            // lanes without a ray take parked ones (up to `want` in total)
+            auto refill = [&](uint32_t want) {
+                const unsigned long long dead = __ballot(!alive);
+                const uint32_t take = min(qn, want);
+                if (!alive) {
+                    const uint32_t r = lane_rank(dead);
+                    if (r < take) {
+                        const uint32_t i = qn - take + r;
+                        o = v3(q[0 * kQueueCap + i], q[1 * kQueueCap + i], q[2 * kQueueCap + i]);
+                        d = v3(q[3 * kQueueCap + i], q[4 * kQueueCap + i], q[5 * kQueueCap + i]);
+                        const uint32_t tag = __float_as_uint(q[6 * kQueueCap + i]);
+                        off = tag & 0x7FFu;
+                        depth = (int)((tag >> 11) & 15u);
+                        key = (int)(tag >> 15) - 1;
+                        alive = true;
+                    }
+                }
                 qn -= take;
-                bool cont = false;
-                V3 o = v3(0.f, 0.f, 0.f), d = o;
-                uint32_t off = 0;
-                int depth = 0, key = -1;
-                if ((uint32_t)lane < take) {
-                    const uint32_t i = qn + (uint32_t)lane;
-                    o = v3(q[0 * kQueueCap + i], q[1 * kQueueCap + i], q[2 * kQueueCap + i]);
-                    d = v3(q[3 * kQueueCap + i], q[4 * kQueueCap + i], q[5 * kQueueCap + i]);
-                    const uint32_t tag = __float_as_uint(q[6 * kQueueCap + i]);
-                    off = tag & 0x7FFu;
-                    depth = (int)((tag >> 11) & 15u);
-                    key = (int)(tag >> 15) - 1;
+            };
+            // the next segment of every held ray
+            auto step = [&]() {
+                if (wl[0] > kLogCap - kLogHeadroom) log_flush(wl, J);
+                if (alive) {
+                    const uint64_t ray = base_ray + off;
                     int last = depth;
                     float tmin = kTminBounce;
-                    const uint64_t ray = base_ray + off;
-                    cont = trace_segment<BLOCK>(J, ray, stk, wc, wh, o, d, tmin, depth, last, next_list(J, depth, key),
-                                                key);
-                    if (!cont) finish_ray(J, ray, last);
+                    alive = trace_segment<BLOCK>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
+                                                 next_list(J, depth, key), key);
+                    if (!alive) finish_ray(J, ray, last);
                 }
-                qn = queue_push(q, qn, cont, o, d, queue_tag(off, depth, key));
             };
             for (uint64_t b = first; b < end; b += 64) {
+                if (wl[0] > kLogCap - kLogHeadroom) log_flush(wl, J);
                 const uint64_t i = b + (uint64_t)lane;
-                bool cont = false;
-                V3 o = v3(0.f, 0.f, 0.f), d = o;
-                int depth = 0, last = 0, key = -1;
+                alive = false;
                 if (i < end) {
                     const uint64_t ray = J.ray_begin + i;
                     uint32_t cell;
                     sun_ray(J, S, ray, wc, o, d, cell);
                     float tmin = kTminPrimary;
-                    cont = trace_segment<BLOCK>(J, ray, stk, wc, wh, o, d, tmin, depth, last, cell_list(J, cell), key);
-                    if (!cont) finish_ray(J, ray, last);
+                    int last = 0;
+                    depth = 0;
+                    key = -1;
+                    off = (uint32_t)(i - first);
+                    alive = trace_segment<BLOCK>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last, cell_list(J, cell),
+                                                 key);
+                    if (!alive) finish_ray(J, ray, last);
                 }
-                qn = queue_push(q, qn, cont, o, d, queue_tag(i - first, depth, key));
-                while (qn >= 64) trace_parked(64);
+                // full waves while held and parked rays fill one
+                for (;;) {
+                    const uint32_t na = (uint32_t)__popcll(__ballot(alive));
+                    if (na + qn < 64u) break;
+                    refill(64u - na);
+                    step();
+                }
+                // park the rest (fewer than 64 in all)
+                const unsigned long long m = __ballot(alive);
+                if (alive) {
+                    const uint32_t i2 = qn + lane_rank(m);
+                    q[0 * kQueueCap + i2] = o.x;
+                    q[1 * kQueueCap + i2] = o.y;
+                    q[2 * kQueueCap + i2] = o.z;
+                    q[3 * kQueueCap + i2] = d.x;
+                    q[4 * kQueueCap + i2] = d.y;
+                    q[5 * kQueueCap + i2] = d.z;
+                    q[6 * kQueueCap + i2] = __uint_as_float(queue_tag(off, depth, key));
+                }
+                qn += (uint32_t)__popcll(m);
+                alive = false;
             }
-            while (qn > 0) trace_parked(min(qn, 64u));
+            // chunk end: partial waves until every ray is done
+            while (qn > 0 || __ballot(alive)) {
+                refill(64u);
+                step();
+            }
         } else {
-            for (uint64_t i = first + lane; i < end; i += 64) {
+            for (uint64_t i0 = first; i0 < end; i0 += 64) {
+                // a batch of 64 whole rays can add more entries than the log
+                // holds (two per segment): flush when half full; entries that
+                // find it full go out as direct atomics
+                if (wl[0] > kLogCap / 2) log_flush(wl, J);
+                const uint64_t i = i0 + (uint64_t)lane;
+                if (i >= end) continue;
                 const uint64_t ray = J.ray_begin + i;
                 V3 o, d;
                 uint32_t cell;
                 sun_ray(J, S, ray, wc, o, d, cell);
                 float tmin = kTminPrimary;
                 int depth = 0, last = 0, key = -1;
-                if (trace_segment<BLOCK>(J, ray, stk, wc, wh, o, d, tmin, depth, last, cell_list(J, cell), key))
-                    while (trace_segment<BLOCK>(J, ray, stk, wc, wh, o, d, tmin, depth, last, next_list(J, depth, key),
+                if (trace_segment<BLOCK>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last, cell_list(J, cell), key))
+                    while (trace_segment<BLOCK>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last, next_list(J, depth, key),
                                                 key)) {
                     }
                 finish_ray(J, ray, last);
             }
         }
         // flush this chunk's counters (same wave wrote them: LDS order suffices)
+        log_flush(wl, J);
         if (LHITS > 0)
             for (int i = lane; i < LHITS; i += 64) {
                 const uint32_t h = wh[i];
