@@ -6,7 +6,7 @@ bins, at most 4 interactions per ray). Scene, BVH and outputs are resident in
 HBM before timing starts. Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--rays 1e8]
-                  [--no-cpu-baseline] [--cpu-sample 1e7] [--no-serve]
+                  [--no-cpu-baseline] [--cpu-sample 4e7] [--no-serve]
 """
 from __future__ import annotations
 
@@ -71,7 +71,7 @@ def cpu_baseline(workload, sample: int, pos: int):
     dt = time.perf_counter() - t0
     return {"value": sample / dt, "unit": "rays/s", "cores": threads, "kind": "port",
             "sample": f"rays [0, {sample}) of the same 1e8-ray field workload, FP64 scalar oracle, "
-                      f"{threads} threads, {dt:.2f} s"}
+                      f"{threads} threads, {dt:.2f} s wall (~{dt * threads:.0f} thread-seconds)"}
 
 
 def main():
@@ -82,7 +82,7 @@ def main():
     ap.add_argument("--rays", type=float, default=1e8)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=float, default=1e7)
+    ap.add_argument("--cpu-sample", type=float, default=4e7)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-vs-plain annual sweep")
     ap.add_argument("--no-autotune", action="store_true")

@@ -160,6 +160,26 @@ def test_queued_and_whole_ray_loops_bitwise(gpu_ctx, scene):
     assert ref["stats"][0] == n and ray_outcomes_balance(ref["stats"].astype(np.int64))
 
 
+@pytest.mark.parametrize("n_helio", [3, 100])
+def test_lds_hit_counts_equal_device_atomics(gpu_ctx, n_helio):
+    """Scenes of <= 128 elements count element hits per wave in LDS (default
+    launch shape); an explicit block size takes the device-atomic path. Both
+    give the same counts, flux and statistics, under i.i.d. and stratified
+    sun positions and for chunked ray ranges."""
+    w = scenes.three_heliostats() if n_helio == 3 else scenes.heliostat_field(n_helio, seed=4)
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    for pos in (_abi.POS_PHILOX, _abi.POS_STRATIFIED):
+        sc.set_sun(scenes.sun_vector(35.0, 200.0), 4.65e-3, position_sampling=pos)
+        kw = dict(max_depth=5, flux_nx=32, flux_ny=16, seed=7, job_id=3)
+        lds = sc.trace(3_000_000, **kw).numpy()
+        dev = sc.trace(3_000_000, block_size=256, grid_blocks=512, **kw).numpy()
+        for k in ("flux", "element_hits", "stats"):
+            assert np.array_equal(lds[k], dev[k]), (n_helio, pos, k)
+        assert lds["element_hits"].sum() > 0
+        part = [sc.trace(3_000_000, ray_begin=b, ray_count=1_000_000, **kw).numpy() for b in (0, 1_000_000, 2_000_000)]
+        assert np.array_equal(sum(p["element_hits"] for p in part), lds["element_hits"])
+
+
 @pytest.mark.parametrize("case", ["field_stratified", "field_philox", "field_low_sun", "field_curved_rough",
                                   "dish_gaussian", "three_halton"])
 def test_candidate_lists_equal_bvh(gpu_ctx, case):
