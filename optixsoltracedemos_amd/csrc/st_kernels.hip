@@ -739,33 +739,15 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
     if (LHITS > 0)
         for (int i = lane; i < LHITS; i += 64) wh[i] = 0;
     const uint32_t waves = gridDim.x * (BLOCK / 64);
-    const uint32_t wave_id = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
-    for (bool first_pass = true;; first_pass = false) {
-        // static_chunks: every wave of the (fully resident) grid traces chunk
-        // wave_id only, and the shared counter is not touched
-        uint64_t c = wave_id;
-        if (static_chunks) {
-            if (!first_pass) break;
-        } else {
-            uint32_t f = 0;
-            if (lane == 0) f = gadd32(&work_counter[0], 1u);
-            c = __builtin_amdgcn_readfirstlane(f);
-        }
-        if (c >= total_chunks) break;
-        int lo = 0;
-        uint64_t base = 0;
-        if (jobs) {
-            int hi = n_jobs;  // chunk_prefix[lo] <= c < chunk_prefix[hi]
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (chunk_prefix[mid] <= c) lo = mid;
-                else hi = mid;
-            }
-            base = chunk_prefix[lo];
-        }
-        const JobDev& J = jobs ? jobs[lo] : single;
+    const uint32_t wave_id = __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6));
+    // one chunk of job J (the chunk's index within the job: cj). It is
+    // expanded at two call sites, one per source of the job record, so that
+    // each copy reads J from one address space (constant: scalar loads, never
+    // reloaded after the kernel's own stores); one copy over a pointer that
+    // may be either would read every field through generic loads.
+    auto run_chunk = [&](const JobDev& J, uint64_t cj) __attribute__((always_inline)) {
         const ST_CONST SunFrame& S = *(const ST_CONST SunFrame*)J.sun;
-        const uint64_t first = (c - base) * (uint64_t)chunk_rays;
+        const uint64_t first = cj * (uint64_t)chunk_rays;
         const uint64_t end = min(first + (uint64_t)chunk_rays, J.ray_count);
         if (QUEUED) {
             float* q = lds_queue[threadIdx.x >> 6];
@@ -889,6 +871,31 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
             } else if (lane == kAggCounter && J.hits && J.agg_slot >= 0) {
                 gadd(J.hits + meta_elem(ld1(J.meta + J.agg_slot)), (unsigned long long)v);
             }
+        }
+    };
+    for (bool first_pass = true;; first_pass = false) {
+        // static_chunks: every wave of the (fully resident) grid traces chunk
+        // wave_id only, and the shared counter is not touched
+        uint64_t c = wave_id;
+        if (static_chunks) {
+            if (!first_pass) break;
+        } else {
+            uint32_t f = 0;
+            if (lane == 0) f = gadd32(&work_counter[0], 1u);
+            c = __builtin_amdgcn_readfirstlane(f);
+        }
+        if (c >= total_chunks) break;
+        if (jobs) {
+            int lo = 0, hi = n_jobs;  // chunk_prefix[lo] <= c < chunk_prefix[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (chunk_prefix[mid] <= c) lo = mid;
+                else hi = mid;
+            }
+            lo = __builtin_amdgcn_readfirstlane(lo);
+            run_chunk(*(const JobDev*)(const ST_CONST JobDev*)(jobs + lo), c - chunk_prefix[lo]);
+        } else {
+            run_chunk(single, c);
         }
     }
     if (!static_chunks && lane == 0) {
