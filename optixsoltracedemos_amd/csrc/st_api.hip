@@ -391,6 +391,26 @@ int scene_bounce_lists(st_scene s) {
     return ST_OK;
 }
 
+int route_flux_copies(DevBuf& buf, JobDev* jobs, int32_t n, uint64_t bins, uint64_t rays, hipStream_t st,
+                      uint32_t* copies) {
+    uint32_t c = flux_copies(bins, rays);
+    while (c > 1 && (uint64_t)n * c * bins * sizeof(int64_t) > kFluxCopyBytes) c >>= 1;
+    *copies = c;
+    if (c == 1) return ST_OK;
+    const size_t need = (size_t)n * c * bins * sizeof(int64_t);
+    if (buf.bytes < need) {
+        // fresh memory: cleared once; every gather leaves the copies zero
+        ST_TRY(buf.ensure(need, "flux copies"));
+        HIP_TRY(launch_fill_u64(st, buf.ptr, buf.bytes / sizeof(int64_t), 0), "clear flux copies");
+    }
+    for (int32_t k = 0; k < n; ++k) {
+        if (!jobs[k].flux) continue;
+        jobs[k].flux = (int64_t*)buf.ptr + (size_t)k * c * bins;
+        jobs[k].flux_copy_mask = c - 1;
+    }
+    return ST_OK;
+}
+
 int scene_grid(st_scene s) {
     hipStream_t st = s->ctx->stream;
     s->grid_valid = false;
@@ -665,6 +685,8 @@ int fill_job(st_scene s, const st_trace_desc* d, const st_outputs* out, const vo
     J->flags = d->flags;
     J->flux_nx = d->flux_nx;
     J->flux_ny = d->flux_ny;
+    J->flux_bins = (uint32_t)std::max<int64_t>((int64_t)d->flux_nx * d->flux_ny, 0);
+    J->flux_copy_mask = 0;
     ST_TRY(resolve_flux_element(s, d->flux_element, d->flux_nx, d->flux_ny, &J->flux_slot));
     // the hottest counter is the main receiver's: aggregate it in registers
     // whether or not a flux map is requested (one atomic per wave and chunk)
@@ -726,7 +748,11 @@ extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* ou
     const uint64_t chunks = (J.ray_count + sh.chunk - 1) / sh.chunk;
     if (chunks >= (1ull << 32)) return fail(ST_ERR_INVALID, "too many rays for one call");
     if (launch_injected()) return fail(ST_ERR_INJECTED, "injected launch failure (trace)");
+    int64_t* const flux_out = J.flux;
+    uint32_t copies = 1;
+    if (J.flux) ST_TRY(route_flux_copies(s->d_flux_copies, &J, 1, J.flux_bins, J.ray_count, st, &copies));
     HIP_TRY(launch_trace(st, nullptr, nullptr, 1, chunks, (uint32_t*)s->d_counter.ptr, sh, J), "trace kernel");
+    if (copies > 1) HIP_TRY(launch_flux_gather(st, J.flux, copies, J.flux_bins, 1, flux_out), "flux gather");
     if (!d->stream) HIP_TRY(hipStreamSynchronize(st), "trace sync");
     return ST_OK;
 }
@@ -775,6 +801,8 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
     ST_TRY(resolve_flux_element(s, -1, 256, 128, &fslot));
     if (fslot < 0) d.flux_nx = d.flux_ny = 0;
     ST_TRY(fill_job(s, &d, &o, s->d_prims.ptr, s->d_nodes.ptr, s->d_sun.ptr, &J));
+    uint32_t copies = 1;
+    if (J.flux) ST_TRY(route_flux_copies(s->d_flux_copies, &J, 1, J.flux_bins, sample_rays, st, &copies));
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0), "event");
     HIP_TRY(hipEventCreate(&e1), "event");
@@ -834,6 +862,9 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
                 }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    // clear the copies (their sums go to the tuning buffer, discarded)
+    if (copies > 1) HIP_TRY(launch_flux_gather(st, J.flux, copies, J.flux_bins, 1, o.flux), "flux gather");
+    HIP_TRY(hipStreamSynchronize(st), "tune sync");
     if (rc != ST_OK) return rc;
     s->tuned = best;
     s->tuned_valid = true;
@@ -994,6 +1025,9 @@ static int batch_enqueue(st_scene s, hipStream_t st) {
     HIP_TRY(launch_trace(st, (const JobDev*)b.d_jobs.ptr, (const uint64_t*)b.d_prefix.ptr, K, b.total_chunks,
                          (uint32_t*)b.d_counter.ptr, b.shape, dummy),
             "batched trace");
+    if (b.flux_copies > 1)
+        HIP_TRY(launch_flux_gather(st, (int64_t*)b.d_flux_copies.ptr, b.flux_copies, b.nbins, K, b.flux_out),
+                "batched flux gather");
     return ST_OK;
 }
 
@@ -1105,6 +1139,9 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
         }
         b.total_chunks = prefix[K];
         if (b.total_chunks >= (1ull << 32)) return fail(ST_ERR_INVALID, "too many rays for one sweep");
+        b.flux_copies = 1;
+        if (b.nbins)
+            ST_TRY(route_flux_copies(b.d_flux_copies, jobs.data(), K, b.nbins, total_rays, st, &b.flux_copies));
         HIP_TRY(hipMemcpyAsync(b.d_jobs.ptr, jobs.data(), sizeof(JobDev) * K, hipMemcpyHostToDevice, st), "upload");
         HIP_TRY(hipMemcpyAsync(b.d_prefix.ptr, prefix.data(), sizeof(uint64_t) * (K + 1), hipMemcpyHostToDevice, st),
                 "upload");

@@ -59,9 +59,22 @@ struct BatchCache {
     uint32_t grid_cap = 0;
     DevBuf d_frames, d_grid_counts, d_grid_start, d_grid_cursor, d_grid_stats, d_grid_items, d_bounce_range,
         d_bounce_scratch;
+    // per-step copies of small flux maps (route_flux_copies)
+    DevBuf d_flux_copies;
+    uint32_t flux_copies = 1;
 };
 // device bytes a batch may spend on its per-step grids and reflection lists
 constexpr uint64_t kBatchListBytes = 3ull << 30;
+constexpr uint64_t kFluxCopyBytes = 256ull << 20;  // bound on the flux-map copies of one launch
+
+// Routes the flux of jobs[0, n) (maps of `bins` bins; J.flux = the output, or
+// null; `rays` in the launch) to flux_copies(bins, rays) copies per job in buf
+// (grown and cleared as needed; fewer copies when they would exceed
+// kFluxCopyBytes). *copies = the number used (1: none, jobs unchanged);
+// launch_flux_gather(buf, *copies, bins, n, outputs) after the trace adds them
+// into the outputs.
+int route_flux_copies(DevBuf& buf, JobDev* jobs, int32_t n, uint64_t bins, uint64_t rays, hipStream_t st,
+                      uint32_t* copies);
 
 }  // namespace st
 
@@ -110,6 +123,7 @@ struct st_scene_s {
     // sun-plane candidate grid (rebuilt with the sun plane; unused when its
     // lists would be long, e.g. at grazing sun elevations)
     st::DevBuf d_grid_counts, d_grid_start, d_grid_cursor, d_grid_items, d_grid_stats;
+    st::DevBuf d_flux_copies;  // copies of a small flux map (route_flux_copies)
     bool grid_valid = false;
     uint32_t grid_items = 0, grid_longest = 0;
     // candidate lists of first reflections (per element) and second

@@ -28,6 +28,8 @@ struct JobDev {
     int32_t flux_slot;       // primitive slot of the flux element (-1 none)
     int32_t agg_slot;        // receiver whose hit count is kept in registers (-1 none)
     int32_t flux_nx, flux_ny;
+    uint32_t flux_bins;      // flux_nx * flux_ny
+    uint32_t flux_copy_mask; // flux holds flux_copy_mask + 1 copies of the map; wave w adds to copy w & mask
     int64_t* flux;
     uint64_t* hits;          // [n_elements]
     uint64_t* stats;         // [ST_STAT_COUNT]
@@ -130,6 +132,25 @@ hipError_t launch_cone_lists(hipStream_t s, const st_element_f64* elems, int32_t
 hipError_t launch_list_compact(hipStream_t s, int32_t tasks, const uint2* range, const int32_t* scratch,
                                int32_t* items);
 hipError_t launch_fill_u64(hipStream_t s, void* p, uint64_t n_u64, uint64_t value);
+// A small flux map's bins are few and its hot spot fewer: atomics of every
+// wave on them would queue at the same addresses. A map of fewer than
+// kFluxDirectBins bins is accumulated in flux_copies(bins) copies (a power of
+// two, so that the copies span at least kFluxSpreadBins bins, at most
+// kFluxMaxCopies of them), which launch_flux_gather adds into the output
+// [jobs][bins] (copies [jobs][copies][bins]) and clears for the next launch.
+// Larger maps spread their atomics by themselves (copies measured no gain), and
+// launches of fewer than kFluxCopyMinRays rays queue too few atomics to pay for
+// the gather.
+constexpr uint64_t kFluxDirectBins = 1u << 15, kFluxSpreadBins = 1u << 17, kFluxCopyMinRays = 1u << 20;
+constexpr uint32_t kFluxMaxCopies = 64;
+inline uint32_t flux_copies(uint64_t bins, uint64_t rays) {
+    uint32_t c = 1;
+    if (rays < kFluxCopyMinRays) return c;
+    while (bins && bins < kFluxDirectBins && c < kFluxMaxCopies && c * bins < kFluxSpreadBins) c <<= 1;
+    return c;
+}
+hipError_t launch_flux_gather(hipStream_t s, int64_t* copies, uint32_t n_copies, uint64_t bins, int32_t jobs,
+                              int64_t* out);
 hipError_t launch_flux_watts(hipStream_t s, const int64_t* flux, int32_t nbins, const SunFrame* sun,
                              double num_rays, double bin_area, double* out, double* power_per_ray_dev);
 

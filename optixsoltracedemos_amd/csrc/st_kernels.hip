@@ -477,6 +477,11 @@ __device__ __forceinline__ void add_one_grouped(uint32_t idx, Emit emit) {
 // end), each a full-width vector atomic. One trace step adds at most
 // kLogHeadroom entries (a lane adds at most two).
 constexpr uint32_t kLogCap = 320, kLogHeadroom = 128;
+// the calling wave's copy of the flux map
+__device__ __forceinline__ int64_t* wave_flux(const JobDev& J) {
+    const uint32_t w = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    return J.flux + (size_t)(w & J.flux_copy_mask) * J.flux_bins;
+}
 __device__ __forceinline__ void log_add(uint32_t* log, const JobDev& J, bool flux, uint32_t idx, uint32_t n) {
     if (idx < (1u << 25) && n <= 64u) {
         const uint32_t pos = atomicAdd(log, 1u);
@@ -485,14 +490,15 @@ __device__ __forceinline__ void log_add(uint32_t* log, const JobDev& J, bool flu
             return;
         }
     }
-    gadd(flux ? (void*)(J.flux + idx) : (void*)(J.hits + idx), (unsigned long long)n);
+    gadd(flux ? (void*)(wave_flux(J) + idx) : (void*)(J.hits + idx), (unsigned long long)n);
 }
 // every lane of the wave calls it (uniform control flow)
 __device__ __forceinline__ void log_flush(uint32_t* log, const JobDev& J) {
     const uint32_t n = min(log[0], kLogCap);
+    int64_t* const fl = J.flux ? wave_flux(J) : nullptr;
     for (uint32_t i = threadIdx.x & 63; i < n; i += 64) {
         const uint32_t e = log[1 + i], idx = e & 0x1FFFFFFu, cnt = ((e >> 25) & 63u) + 1u;
-        gadd((e >> 31) ? (void*)(J.flux + idx) : (void*)(J.hits + idx), (unsigned long long)cnt);
+        gadd((e >> 31) ? (void*)(fl + idx) : (void*)(J.hits + idx), (unsigned long long)cnt);
     }
     if ((threadIdx.x & 63) == 0) log[0] = 0;
 }
@@ -1499,6 +1505,20 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t value) {
         p[i] = value;
 }
 
+// out[j][i] += sum of copies[j][c][i] over the copies c, which are cleared
+__global__ void k_flux_gather(int64_t* __restrict__ copies, uint32_t n_copies, uint64_t bins, uint64_t total,
+                              int64_t* __restrict__ out) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    int64_t* c = copies + (g / bins) * n_copies * bins + g % bins;
+    int64_t sum = 0;
+    for (uint32_t k = 0; k < n_copies; ++k) {
+        sum += c[(size_t)k * bins];
+        c[(size_t)k * bins] = 0;
+    }
+    out[g] += sum;
+}
+
 __global__ void k_flux_watts(const int64_t* __restrict__ flux, int32_t nbins, const SunFrame* __restrict__ sun,
                              double num_rays, double bin_area, double* __restrict__ out, double* ppr) {
     const double power_per_ray = sun->dni * sun->area / num_rays;
@@ -1726,6 +1746,15 @@ hipError_t launch_fill_u64(hipStream_t s, void* p, uint64_t n_u64, uint64_t valu
     uint64_t blocks = (n_u64 + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)blocks), dim3(256), 0, s, (uint64_t*)p, n_u64, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_flux_gather(hipStream_t s, int64_t* copies, uint32_t n_copies, uint64_t bins, int32_t jobs,
+                              int64_t* out) {
+    const uint64_t total = bins * (uint64_t)jobs;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_flux_gather, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, copies, n_copies, bins,
+                       total, out);
     return hipGetLastError();
 }
 

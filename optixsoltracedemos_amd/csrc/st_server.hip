@@ -48,6 +48,8 @@ double now_s() {
 // grid and reflection lists; pooled per scene
 struct WorkBufs {
     DevBuf elems, aim_in, ref_aabb, bounds, prims, nodes, sun, frames, grid_start, grid_items, list_range, list_items;
+    DevBuf flux_copies;   // copies of a small flux map, gathered into the job's flux when it completes
+    uint32_t copies = 1;
 };
 // a job's outputs, kept until st_server_release; pooled
 struct OutBufs {
@@ -105,8 +107,8 @@ struct st_server_s {
 namespace {
 
 // device bytes of a job: the re-aimed scene copy with its sun plane, grid
-// (at most kGridMaxAverage entries per cell) and, for long jobs, frames and
-// reflection lists; plus the outputs
+// (at most kGridMaxAverage entries per cell), copies of a small flux map and,
+// for long jobs, frames and reflection lists; plus the outputs
 size_t job_scene_bytes(st_scene sc, const st_job_desc& d) {
     const size_t n = (size_t)sc->n, nn = (size_t)sc->n_nodes;
     size_t b = n * (sizeof(st_element_f64) + 6 * sizeof(double) + 6 * sizeof(float) + 4 * sizeof(float4)) +
@@ -114,6 +116,9 @@ size_t job_scene_bytes(st_scene sc, const st_job_desc& d) {
                (kGridCells + 1 + (size_t)kGridMaxAverage * kGridCells) * sizeof(uint32_t);
     if (d.num_rays >= kServerListRays)
         b += n * (9 * sizeof(double) + sizeof(uint2) + kMaxBounceList * sizeof(int32_t));
+    const size_t nbins = (size_t)std::max(d.flux_nx, 0) * (size_t)std::max(d.flux_ny, 0);
+    const uint32_t copies = flux_copies(nbins, d.num_rays);
+    if (copies > 1) b += (size_t)copies * nbins * sizeof(int64_t);
     return b;
 }
 
@@ -261,6 +266,14 @@ int finish_setup(st_server srv, ServerJob* j, uint32_t grid_total, uint32_t grid
     out.element_hits = (uint64_t*)o.hits.ptr;
     out.stats = (uint64_t*)o.stats.ptr;
     if ((rc = fill_job(sc, &td, &out, w.prims.ptr, w.nodes.ptr, w.sun.ptr, &j->dev)) != ST_OK) return rc;
+    w.copies = 1;
+    if (j->dev.flux) {
+        if ((rc = route_flux_copies(w.flux_copies, &j->dev, 1, nbins, d.num_rays, st, &w.copies)) != ST_OK)
+            return rc;
+        // a pooled buffer may hold a cancelled job's partial sums
+        if (w.copies > 1 && (e = launch_fill_u64(st, w.flux_copies.ptr, (uint64_t)w.copies * nbins, 0)) != hipSuccess)
+            return hip_fail(e, "job flux copies clear");
+    }
     if (grid) {
         j->dev.grid_start = (const uint32_t*)w.grid_start.ptr;
         j->dev.grid_items = (const int32_t*)w.grid_items.ptr;
@@ -445,7 +458,10 @@ void worker_main(st_server srv) {
         frames.resize(finished.size());
         for (size_t i = 0; i < finished.size() && e == hipSuccess; ++i) {
             ServerJob* j = finished[i];
-            e = hipMemcpyAsync(j->stats, j->out->stats.ptr, sizeof(j->stats), hipMemcpyDeviceToHost, st);
+            if (j->work->copies > 1)
+                e = launch_flux_gather(st, j->dev.flux, j->work->copies, j->dev.flux_bins, 1, (int64_t*)j->out->flux.ptr);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(j->stats, j->out->stats.ptr, sizeof(j->stats), hipMemcpyDeviceToHost, st);
             if (e == hipSuccess) e = hipMemcpyAsync(&frames[i], j->work->sun.ptr, sizeof(SunFrame), hipMemcpyDeviceToHost, st);
         }
         if (e == hipSuccess) e = hipStreamSynchronize(st);

@@ -180,6 +180,26 @@ def test_lds_hit_counts_equal_device_atomics(gpu_ctx, n_helio):
         assert np.array_equal(sum(p["element_hits"] for p in part), lds["element_hits"])
 
 
+def test_flux_copies_cleared_between_calls(gpu_ctx):
+    """A small flux map is accumulated in per-wave copies and gathered after
+    the launch: calls that accumulate into one output add exactly, and every
+    binned hit lands in the map."""
+    w = scenes.three_heliostats()
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=_abi.POS_PHILOX)
+    kw = dict(max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny, seed=3)
+    one = sc.trace(2_000_000, **kw).numpy()
+    st = tracer.stats_dict(one["stats"])
+    assert one["flux"].sum() == st["flux_binned"] > 0
+    desc = sc.make_desc(2_000_000, zero=False, **kw)
+    res = sc.alloc_outputs(desc)
+    sc.trace_into(desc, res)
+    sc.trace_into(desc, res)
+    two = res.numpy()
+    assert np.array_equal(two["flux"], 2 * one["flux"])
+    assert np.array_equal(sc.trace(2_000_000, **kw).numpy()["flux"], one["flux"])
+
+
 @pytest.mark.parametrize("case", ["field_stratified", "field_philox", "field_low_sun", "field_curved_rough",
                                   "dish_gaussian", "three_halton"])
 def test_candidate_lists_equal_bvh(gpu_ctx, case):
