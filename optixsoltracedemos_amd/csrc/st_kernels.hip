@@ -267,14 +267,18 @@ __device__ __forceinline__ bool isect_cylinder(const float4& q0, const float4& q
     return true;
 }
 
-__device__ __forceinline__ bool isect_prim(const float4* __restrict__ prims, uint32_t kind, int slot, V3 o, V3 d,
-                                          float tmin, float tmax, float& t, int& cap) {
-    const float4 q0 = ld4(prims + 4 * slot + 0), q1 = ld4(prims + 4 * slot + 1), q2 = ld4(prims + 4 * slot + 2),
-                 q3 = ld4(prims + 4 * slot + 3);
+__device__ __forceinline__ bool isect_record(uint32_t kind, const float4& q0, const float4& q1, const float4& q2,
+                                            const float4& q3, V3 o, V3 d, float tmin, float tmax, float& t, int& cap) {
     cap = 0;
     if (kind == kKindFlat) return isect_flat(q0, q1, q2, q3, o, d, tmin, tmax, t);
     if (kind == kKindParabolic) return isect_parabolic(q0, q1, q2, q3, o, d, tmin, tmax, t);
     return isect_cylinder(q0, q1, q2, q3, o, d, tmin, tmax, t, cap);
+}
+__device__ __forceinline__ bool isect_prim(const float4* __restrict__ prims, uint32_t kind, int slot, V3 o, V3 d,
+                                          float tmin, float tmax, float& t, int& cap) {
+    const float4 q0 = ld4(prims + 4 * slot + 0), q1 = ld4(prims + 4 * slot + 1), q2 = ld4(prims + 4 * slot + 2),
+                 q3 = ld4(prims + 4 * slot + 3);
+    return isect_record(kind, q0, q1, q2, q3, o, d, tmin, tmax, t, cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -345,10 +349,59 @@ __device__ __forceinline__ void test_flat(const float4* __restrict__ prims, cons
 // every step. Every primitive such a ray can reach is listed (conservative
 // binning, k_grid_* / k_cone_lists), so the result equals a full BVH
 // traversal. bk = list index of the winner.
+// The same search over a list that every lane of the wave walks (the sun-plane
+// cell of a wave-coherent group of sun rays): b, e and items are wave-uniform,
+// so entries, meta words and primitive records come through scalar loads, one
+// per wave instead of one per lane, and the loop runs on the scalar unit.
+__device__ __forceinline__ void closest_in_shared_list(const JobDev& J, const int32_t* items, uint32_t b, uint32_t e,
+                                                       bool headed, V3 o, V3 d, float tmin, float& tbest, int& bslot,
+                                                       int& bcap, uint32_t& bk, uint32_t* wk) {
+    const ST_CONST uint32_t* it = (const ST_CONST uint32_t*)items;
+    const ST_CONST uint32_t* M = (const ST_CONST uint32_t*)J.meta;
+    const ST_CONST f4v* P = (const ST_CONST f4v*)J.prims;
+    uint32_t k = b, fe = b;  // headed: flat entries [b + 1, fe)
+    if (headed) {
+        fe = b + 1 + it[b];
+        k = b + 1;
+    }
+    for (; k < e; ++k) {
+        const int slot = (int)it[k];
+        const uint32_t m = M[slot];
+        const uint32_t kind = k < fe ? (uint32_t)kKindFlat : meta_kind(m);
+        if (wk) count(wk, ST_STAT_FLAT_TESTS + (int)kind);
+        const f4v a0 = P[4 * slot + 0], a1 = P[4 * slot + 1], a2 = P[4 * slot + 2], a3 = P[4 * slot + 3];
+        const float4 q0 = make_float4(a0.x, a0.y, a0.z, a0.w), q1 = make_float4(a1.x, a1.y, a1.z, a1.w),
+                     q2 = make_float4(a2.x, a2.y, a2.z, a2.w), q3 = make_float4(a3.x, a3.y, a3.z, a3.w);
+        const float lim = bslot >= 0 ? next_up(tbest) : tbest;
+        float t;
+        int cap;
+        if (isect_record(kind, q0, q1, q2, q3, o, d, tmin, lim, t, cap)) {
+            if (bslot < 0 || t < tbest || (t == tbest && meta_elem(m) < meta_elem(ld1(J.meta + bslot)))) {
+                tbest = t;
+                bslot = slot;
+                bcap = cap;
+                bk = k;
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ void closest_in_list(const JobDev& J, const int32_t* __restrict__ items, uint32_t b,
                                                 uint32_t e, bool headed, V3 o, V3 d, float tmin, float& tbest,
                                                 int& bslot, int& bcap, uint32_t& bk, uint32_t* wc) {
     uint32_t* wk = (J.flags & ST_TRACE_COUNT_WORK) ? wc : nullptr;
+    {
+        const uint64_t ip = (uint64_t)(uintptr_t)items;
+        const uint32_t b0 = __builtin_amdgcn_readfirstlane(b), e0 = __builtin_amdgcn_readfirstlane(e);
+        const uint32_t i0 = __builtin_amdgcn_readfirstlane((uint32_t)ip),
+                       i1 = __builtin_amdgcn_readfirstlane((uint32_t)(ip >> 32));
+        const int h0 = __builtin_amdgcn_readfirstlane((int)headed);
+        if (!__ballot(b != b0 || e != e0 || (uint32_t)ip != i0 || (uint32_t)(ip >> 32) != i1 || (int)headed != h0)) {
+            closest_in_shared_list(J, (const int32_t*)(uintptr_t)(((uint64_t)i1 << 32) | i0), b0, e0, h0 != 0, o, d,
+                                   tmin, tbest, bslot, bcap, bk, wk);
+            return;
+        }
+    }
     uint32_t k = b;
     if (headed) {
         const uint32_t fe = b + 1 + ld1((const uint32_t*)items + b);
