@@ -609,7 +609,6 @@ __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame
         }
         d = normalize(fma3(ct, w, st * fma3(cp, su, sp * sv)));
     }
-    count(wc, ST_STAT_RAYS);
     record(J, ray, 0, 0.f, o, 0xFFFFFFFEu);
 }
 
@@ -621,7 +620,6 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
                                               uint32_t* wh, uint32_t* wl, V3& o, V3& d, float& tmin, int& depth,
                                               int& last, SegList L, int& key) {
     const uint32_t rlo = (uint32_t)ray, rhi = (uint32_t)(ray >> 32);
-    count(wc, ST_STAT_SEGMENTS);
     float t = kTmax;
     int slot = -1, cap = 0;
     uint32_t bk = 0xFFFFFFFFu;
@@ -678,8 +676,12 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
                 int ix = (int)floorf(fx), iy = (int)floorf(fy);
                 ix = min(max(ix, 0), J.flux_nx - 1);
                 iy = min(max(iy, 0), J.flux_ny - 1);
-                add_one_grouped<2>((uint32_t)(iy * J.flux_nx + ix),
-                                   [&](uint32_t b, uint32_t n) { log_add(wl, J, true, b, n); });
+                // a wave's hits share bins only on small maps
+                if (J.flux_bins < kFluxDirectBins)
+                    add_one_grouped<2>((uint32_t)(iy * J.flux_nx + ix),
+                                       [&](uint32_t b, uint32_t n) { log_add(wl, J, true, b, n); });
+                else
+                    log_add(wl, J, true, (uint32_t)(iy * J.flux_nx + ix), 1u);
                 count(wc, ST_STAT_FLUX_BINNED);
             }
         } else {
@@ -922,7 +924,13 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                     if (J.hits) gadd(J.hits + i, (unsigned long long)h);
                 }
             }
-        const uint32_t v = lane < kNumCounters ? wc[lane] : 0u;
+        // rays and segments are not counted as they happen: a chunk's rays are
+        // known, and every segment ends in exactly one of five outcomes
+        uint32_t v = lane < kNumCounters ? wc[lane] : 0u;
+        if (lane == ST_STAT_RAYS) v = (uint32_t)(end - first);
+        if (lane == ST_STAT_SEGMENTS)
+            v = wc[ST_STAT_ESCAPED] + wc[ST_STAT_BACK_FACE] + wc[ST_STAT_RECEIVER_HITS] + wc[ST_STAT_TRUNCATED] +
+                wc[ST_STAT_MIRROR_HITS];
         if (lane < kNumCounters) wc[lane] = 0;
         if (v) {
             if (lane < kAggCounter) {

@@ -83,6 +83,28 @@ def test_parabolic_dish_parity(gpu_ctx, oracle_mod):
     assert g["stats"][9] > 0 and g["stats"][4] > 0          # caps and direct-sun hits occur
 
 
+@pytest.mark.parametrize("scene", ["dish", "three"])
+def test_small_flux_maps_in_copies_match_oracle(gpu_ctx, oracle_mod, scene):
+    """2^21 rays on a 128x64 or 64x64 map: the launch accumulates the map in
+    per-wave copies and gathers them; the map, counts and statistics agree
+    with the oracle on the same draws (bins differ only by edge rays)."""
+    w = scenes.parabolic_dish() if scene == "dish" else scenes.three_heliostats()
+    n = 1 << 21
+    kw = dict(seed=5, job_id=2, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny)
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=_abi.POS_PHILOX)
+    g = sc.trace(n, **kw).numpy()
+    os_ = oracle_mod.Scene(w.elements)
+    os_.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=_abi.POS_PHILOX)
+    o = os_.trace(n, **kw)
+    assert g["flux"].sum() > n // 100
+    assert flux_l1(g["flux"], o["flux"]) <= FLUX_L1_MAX
+    slack = int(ID_MISMATCH_MAX * n) + 2
+    assert np.abs(g["stats"].astype(np.int64)[:11] - o["stats"].astype(np.int64)[:11]).max() <= 2 * slack
+    assert np.abs(g["element_hits"].astype(np.int64) - o["element_hits"].astype(np.int64)).sum() <= 4 * slack
+    assert ray_outcomes_balance(g["stats"])
+
+
 def test_field_10k_parity(gpu_ctx, oracle_mod):
     w = scenes.heliostat_field(10_000)
     n = 200_000
