@@ -627,10 +627,7 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
     if (listed) closest_in_list(J, L.items, L.lb, L.le, L.headed, o, d, tmin, t, slot, cap, bk, wc);
     else traverse<STRIDE>(J.nodes, J.prims, J.meta, o, d, tmin, t, slot, cap, stk,
                           (J.flags & ST_TRACE_COUNT_WORK) ? wc : nullptr);
-    if (slot < 0) {  // __miss__ms, cuda/materials.cu:236-253
-        count(wc, ST_STAT_ESCAPED);
-        return false;
-    }
+    if (slot < 0) return false;  // __miss__ms, cuda/materials.cu:236-253 (counted at the chunk's end)
     const int nd = depth + 1;
     const uint32_t m = ld1(J.meta + slot);
     const uint32_t kind = meta_kind(m), elem = meta_elem(m);
@@ -924,13 +921,16 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                     if (J.hits) gadd(J.hits + i, (unsigned long long)h);
                 }
             }
-        // rays and segments are not counted as they happen: a chunk's rays are
-        // known, and every segment ends in exactly one of five outcomes
+        // rays, segments and escapes are not counted as they happen: every ray
+        // of the chunk has ended, each in exactly one of escaped, absorbed, back
+        // face, truncated or receiver hit, and each segment after the first
+        // follows a mirror hit that was not absorbed
+        const uint32_t rays = (uint32_t)(end - first);
         uint32_t v = lane < kNumCounters ? wc[lane] : 0u;
-        if (lane == ST_STAT_RAYS) v = (uint32_t)(end - first);
-        if (lane == ST_STAT_SEGMENTS)
-            v = wc[ST_STAT_ESCAPED] + wc[ST_STAT_BACK_FACE] + wc[ST_STAT_RECEIVER_HITS] + wc[ST_STAT_TRUNCATED] +
-                wc[ST_STAT_MIRROR_HITS];
+        if (lane == ST_STAT_RAYS) v = rays;
+        if (lane == ST_STAT_SEGMENTS) v = rays + wc[ST_STAT_MIRROR_HITS] - wc[ST_STAT_ABSORBED];
+        if (lane == ST_STAT_ESCAPED)
+            v = rays - wc[ST_STAT_ABSORBED] - wc[ST_STAT_BACK_FACE] - wc[ST_STAT_TRUNCATED] - wc[ST_STAT_RECEIVER_HITS];
         if (lane < kNumCounters) wc[lane] = 0;
         if (v) {
             if (lane < kAggCounter) {
