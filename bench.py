@@ -196,7 +196,9 @@ def main():
     if not args.no_serve:
         try:
             from optixsoltracedemos_amd import serving
-            line["latency"] = serving.quick_latency(ctx)
+            scs = serving.build_serving_scenes(ctx)
+            line["latency"] = serving.quick_latency(ctx, serving_scenes=scs)
+            line["served_vs_plain"] = serving.compare_plain(ctx, n_jobs=400, seed=args.seed, serving_scenes=scs)
         except Exception as ex:  # reported, never silently replaced
             line["latency"] = {"error": f"{type(ex).__name__}: {ex}"}
     if not args.no_sweep:

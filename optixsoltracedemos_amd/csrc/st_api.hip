@@ -77,7 +77,7 @@ int DevBuf::ensure(size_t n, const char* what) {
 }
 
 LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel, int32_t bvh_depth,
-                       const LaunchShape* tuned, int32_t max_elements) {
+                       const LaunchShape* tuned, int32_t min_elements) {
     LaunchShape sh;
     const int cus = device_cu_count();
     // ST_KERNEL_MEGA: whole ray per lane; ST_KERNEL_QUEUED: continuing rays
@@ -99,9 +99,9 @@ LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kerne
         sh.waves = sh.queued ? 3 : 4;
         if (!trace_variant_exists(sh.block, sh.stack, sh.waves, sh.queued)) sh.block = 128;
     }
-    if (sh.queued && max_elements <= kLdsHitElements && bvh_depth <= 8 && block <= 0) {
+    if (sh.queued && min_elements <= kLdsHitElements && bvh_depth <= 16 && block <= 0) {
         sh.block = 256;
-        sh.stack = 8;
+        sh.stack = bvh_depth <= 8 ? 8 : 16;
         sh.waves = 5;
         sh.lds_hits = true;
     }

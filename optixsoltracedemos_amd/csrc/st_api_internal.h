@@ -32,10 +32,12 @@ struct DevBuf {
 
 // launch shape for `rays` rays; block/grid > 0 override, else the scene's tuned
 // shape when there is one, else defaults (16-entry stack when the BVH allows);
-// max_elements: the largest scene in the launch (small ones count element
-// hits in LDS)
+// bvh_depth: the deepest scene in the launch; min_elements: the smallest scene
+// in the launch (when it has at most kLdsHitElements elements, the launch
+// counts element hits below that index per wave in LDS: a small scene's few
+// counters would otherwise take every wave's device atomics)
 LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel, int32_t bvh_depth,
-                       const LaunchShape* tuned, int32_t max_elements);
+                       const LaunchShape* tuned, int32_t min_elements);
 
 struct BatchCache {
     bool valid = false;

@@ -65,8 +65,8 @@ struct LaunchShape {
     int stack = 32;      // traversal stack entries per lane (16 or 32, >= BVH depth)
     int waves = 4;       // waves per SIMD the register allocation is built for
     bool queued = false; // continuing rays parked in LDS, traced in full waves
-    bool lds_hits = false;  // per-element hit counts kept per wave in LDS (scenes of
-                            // <= kLdsHitElements elements; queued, block 256, stack 8)
+    bool lds_hits = false;  // hit counts of elements below kLdsHitElements kept per wave in
+                            // LDS (launches with a scene that small; queued, block 256, stack 8 or 16)
 };
 // Scenes this small count element hits in LDS: their few counters would
 // otherwise take a device-scope atomic from nearly every wave of the launch.

@@ -1718,7 +1718,8 @@ bool trace_variant_exists(int block, int stack, int waves, bool queued) {
     ST_VARIANT(256, 16, 5, true, 0)    \
     ST_VARIANT(64, 32, 3, true, 0)     \
     ST_VARIANT(128, 32, 3, true, 0)    \
-    ST_VARIANT(256, 8, 5, true, kLdsHitElements)
+    ST_VARIANT(256, 8, 5, true, kLdsHitElements) \
+    ST_VARIANT(256, 16, 5, true, kLdsHitElements)
 
 int trace_resident_waves(const LaunchShape& shape) {
     const int st = shape.stack, w = shape.waves, b = shape.block, h = shape.lds_hits ? kLdsHitElements : 0;

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
@@ -421,13 +422,13 @@ void worker_main(st_server srv) {
             slice.push_back(sl);
             round_rays += sl;
         }
-        int32_t depth = 0, max_n = 0;
+        int32_t depth = 0, min_n = INT32_MAX;
         for (ServerJob* j : round) {
             depth = std::max(depth, j->scene->topo.depth);
-            max_n = std::max(max_n, j->scene->n);
+            min_n = std::min(min_n, j->scene->n);
         }
         const LaunchShape sh = pick_shape(round_rays, srv->cfg.block_size, srv->cfg.grid_blocks, ST_KERNEL_AUTO, depth,
-                                          nullptr, max_n);
+                                          nullptr, min_n);
         srv->h_prefix[0] = 0;
         for (size_t i = 0; i < round.size(); ++i) {
             JobDev J = round[i]->dev;

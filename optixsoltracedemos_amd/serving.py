@@ -112,8 +112,9 @@ def poisson_jobs(n_jobs=2000, rate=200.0, seed=0, weights=(0.4, 0.3, 0.2, 0.1), 
 
 
 def run_stream(ctx, n_jobs=2000, rate=200.0, seed=0, hbm_budget_bytes=64 << 30, max_active_jobs=64,
-               chunk_rays=8 << 20, flux=(64, 32), rays_hi=1e8, serving_scenes=None):
-    """Submit a Poisson stream at its arrival times, wait for all, report latency and throughput."""
+               chunk_rays=8 << 20, flux=(64, 32), rays_hi=1e8, serving_scenes=None, sampling=_abi.POS_STRATIFIED):
+    """Submit a Poisson stream at its arrival times, wait for all, report latency and throughput.
+    Jobs sample sun positions stratified (wave-coherent 64-ray groups) unless `sampling` says otherwise."""
     sc_list = serving_scenes or build_serving_scenes(ctx)
     srv = Server(ctx, hbm_budget_bytes, max_active_jobs, chunk_rays)
     slots = [srv.add_scene(s) for _, _, s in sc_list]
@@ -129,7 +130,7 @@ def run_stream(ctx, n_jobs=2000, rate=200.0, seed=0, hbm_budget_bytes=64 << 30, 
                 time.sleep(dt)
             try:
                 handles[i] = srv.submit(slots[kind[i]], suns[i], int(rays[i]), seed=seed, job_id=i,
-                                        flux_nx=flux[0], flux_ny=flux[1])
+                                        flux_nx=flux[0], flux_ny=flux[1], position_sampling=sampling)
             except Exception as ex:  # recorded, the job counts as failed
                 errors.append((i, str(ex)))
 
@@ -159,9 +160,66 @@ def run_stream(ctx, n_jobs=2000, rate=200.0, seed=0, hbm_budget_bytes=64 << 30, 
         "aggregate_rays_per_s": traced / span if ok else None,
         "launches": int(st["launches"]), "peak_resident_mb": st["peak_resident_bytes"] / 2**20,
         "gpu_busy_s": st["busy_s"], "span_s": span,
+        "sampling": {_abi.POS_PHILOX: "philox", _abi.POS_HALTON: "halton", _abi.POS_STRATIFIED: "stratified"}[sampling],
     }
 
 
-def quick_latency(ctx, n_jobs=400, seed=0):
+def quick_latency(ctx, n_jobs=400, seed=0, serving_scenes=None):
     """Short horizon of the serving stream for bench.py (~2 s of arrivals at 200 jobs/s)."""
-    return run_stream(ctx, n_jobs=n_jobs, seed=seed)
+    return run_stream(ctx, n_jobs=n_jobs, seed=seed, serving_scenes=serving_scenes)
+
+
+def compare_plain(ctx, n_jobs=200, seed=0, flux=(64, 32), max_active_jobs=64, serving_scenes=None, plain=True,
+                  weights=(0.4, 0.3, 0.2, 0.1), sampling=_abi.POS_STRATIFIED, rays_hi=1e8, **server_kw):
+    """Aggregate throughput of the served form against the plain per-scene loop
+    on the same jobs of the serving mix (all submitted at once: throughput, not
+    latency). Plain loop: per job, on a device-resident scene of its kind, set
+    the sun, re-aim the field, trace synchronously and copy the flux map to the
+    host (what a caller of lib/soltrace_system.cpp's update()/run() does per
+    scene state). Both give every job's flux map bitwise the same."""
+    import torch
+    sc_list = serving_scenes or build_serving_scenes(ctx)
+    _, kind, suns, rays = poisson_jobs(n_jobs, seed=seed, weights=weights, rays_hi=rays_hi)
+    # served: one server, every job submitted, then waited for and fetched
+    srv = Server(ctx, max_active_jobs=max_active_jobs, **server_kw)
+    slots = [srv.add_scene(s) for _, _, s in sc_list]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hs = [srv.submit(slots[kind[i]], suns[i], int(rays[i]), seed=seed, job_id=i, flux_nx=flux[0], flux_ny=flux[1],
+                     position_sampling=sampling) for i in range(n_jobs)]
+    served = []
+    for i, h in enumerate(hs):
+        r = srv.wait(h)
+        if r["status"] != 0:
+            raise RuntimeError(f"job {i} failed: {r['status']}")
+        n_el = len(sc_list[kind[i]][1].elements)
+        served.append(srv.fetch(h, n_el, (flux[1], flux[0]))[0])
+        srv.release(h)
+    t_served = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    if not plain:
+        return {"jobs": n_jobs, "served_s": t_served, "launches": int(st["launches"]), "gpu_busy_s": st["busy_s"]}
+    # plain loop on scenes of its own
+    plain_scenes = []
+    for _, w, _ in sc_list:
+        ps = tracer.Scene(ctx, w.elements)
+        ps.set_sun(w.sun_vector, w.sun_param)
+        plain_scenes.append((w, ps))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain = []
+    for i in range(n_jobs):
+        w, ps = plain_scenes[kind[i]]
+        ps.set_sun(suns[i], 4.65e-3, position_sampling=sampling)
+        ps.aim_heliostats(suns[i], w.elements[w.receiver]["origin"])
+        r = ps.trace(int(rays[i]), seed=seed, job_id=i, max_depth=5, flux_nx=flux[0], flux_ny=flux[1])
+        plain.append(r.flux.cpu().numpy())
+    t_plain = time.perf_counter() - t0
+    same = all(np.array_equal(a, b) for a, b in zip(served, plain))
+    total = int(rays.sum())
+    return {"jobs": n_jobs, "total_rays": total, "served_s": t_served, "plain_loop_s": t_plain,
+            "max_active_jobs": max_active_jobs,
+            "served_rays_per_s": total / t_served, "plain_rays_per_s": total / t_plain,
+            "speedup": t_plain / t_served, "identical_flux": bool(same)}
+

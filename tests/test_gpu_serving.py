@@ -116,3 +116,12 @@ def test_short_poisson_stream(gpu_ctx, mix):
     assert m["completed"] == 60 and m["rejected"] == 0 and m["errors"] == 0
     assert 0 < m["p50_latency_ms"] <= m["p99_latency_ms"]
     assert m["aggregate_rays_per_s"] > 0
+
+
+def test_served_and_plain_loop_give_identical_maps(gpu_ctx, mix):
+    """The throughput comparison traces the same jobs both ways: served (all
+    submitted at once) and one by one on scenes of their own."""
+    for samp in (_abi.POS_STRATIFIED, _abi.POS_PHILOX):
+        r = serving.compare_plain(gpu_ctx, n_jobs=24, seed=3, serving_scenes=mix, rays_hi=3e6, sampling=samp)
+        assert r["identical_flux"] and r["served_s"] > 0 and r["plain_loop_s"] > 0
+
