@@ -159,6 +159,12 @@ extern "C" int st_context_create(int device, st_context* out) {
     auto ctx = std::make_unique<st_context_s>();
     ctx->device = device;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (hipEventCreateWithFlags(&ctx->setup_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->trace_done, hipEventDisableTiming) != hipSuccess) {
+        if (ctx->setup_done) (void)hipEventDestroy(ctx->setup_done);
+        (void)hipStreamDestroy(ctx->stream);
+        return fail(ST_ERR_DEVICE, "hipEventCreate failed");
+    }
     *out = ctx.release();
     return ST_OK;
 }
@@ -169,8 +175,31 @@ void context_release(st_context ctx) {
     if (ctx->refs.fetch_sub(1) != 1) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    (void)hipEventDestroy(ctx->setup_done);
+    (void)hipEventDestroy(ctx->trace_done);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+static bool capturing(hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+int order_before_trace(st_context ctx, hipStream_t user) {
+    if (!user || user == ctx->stream || capturing(user)) return ST_OK;
+    std::lock_guard<std::mutex> lk(ctx->order_mu);
+    HIP_TRY(hipEventRecord(ctx->setup_done, ctx->stream), "set-up event");
+    HIP_TRY(hipStreamWaitEvent(user, ctx->setup_done, 0), "wait for set-up");
+    return ST_OK;
+}
+
+int order_after_trace(st_context ctx, hipStream_t user) {
+    if (!user || user == ctx->stream || capturing(user)) return ST_OK;
+    std::lock_guard<std::mutex> lk(ctx->order_mu);
+    HIP_TRY(hipEventRecord(ctx->trace_done, user), "trace event");
+    HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->trace_done, 0), "wait for trace");
+    return ST_OK;
 }
 }  // namespace st
 
@@ -738,6 +767,7 @@ extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* ou
     JobDev J;
     ST_TRY(fill_job(s, d, out, s->d_prims.ptr, s->d_nodes.ptr, s->d_sun.ptr, &J));
     hipStream_t st = d->stream ? (hipStream_t)d->stream : s->ctx->stream;
+    ST_TRY(order_before_trace(s->ctx, st));
     if ((d->flags & ST_TRACE_ZERO_OUTPUTS) && out) {
         if (J.flux) HIP_TRY(launch_fill_u64(st, J.flux, (uint64_t)d->flux_nx * d->flux_ny, 0), "zero flux");
         if (out->element_hits) HIP_TRY(launch_fill_u64(st, out->element_hits, (uint64_t)s->n, 0), "zero hits");
@@ -753,6 +783,7 @@ extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* ou
     if (J.flux) ST_TRY(route_flux_copies(s->d_flux_copies, &J, 1, J.flux_bins, J.ray_count, st, &copies));
     HIP_TRY(launch_trace(st, nullptr, nullptr, 1, chunks, (uint32_t*)s->d_counter.ptr, sh, J), "trace kernel");
     if (copies > 1) HIP_TRY(launch_flux_gather(st, J.flux, copies, J.flux_bins, 1, flux_out), "flux gather");
+    ST_TRY(order_after_trace(s->ctx, st));
     if (!d->stream) HIP_TRY(hipStreamSynchronize(st), "trace sync");
     return ST_OK;
 }
@@ -897,6 +928,7 @@ extern "C" int st_flux_to_watts(st_scene s, const st_trace_desc* d, const int64_
     }
     HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
     hipStream_t st = d->stream ? (hipStream_t)d->stream : s->ctx->stream;
+    ST_TRY(order_before_trace(s->ctx, st));
     double* ppr_dev = nullptr;
     DevBuf tmp;
     if (power_per_ray) {
@@ -1054,6 +1086,7 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
     if (fslot >= 0 && !flux) return fail(ST_ERR_INVALID, "flux grid requested without a flux buffer");
     HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
     hipStream_t st = d->stream ? (hipStream_t)d->stream : s->ctx->stream;
+    ST_TRY(order_before_trace(s->ctx, st));
     BatchCache& b = s->batch;
     const bool reuse = b.valid && same_batch(b, d) && b.flux_out == flux && b.hits_out == element_hits &&
                        b.stats_out == stats;
@@ -1171,6 +1204,7 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
     } else {
         ST_TRY(batch_enqueue(s, st));
     }
+    ST_TRY(order_after_trace(s->ctx, st));
     if (!d->stream) HIP_TRY(hipStreamSynchronize(st), "batch sync");
     return ST_OK;
 }

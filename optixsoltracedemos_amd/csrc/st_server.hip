@@ -137,6 +137,8 @@ int setup_job(st_server srv, ServerJob* j, int slot) {
     st_scene sc = j->scene;
     const size_t n = (size_t)sc->n, nn = (size_t)sc->n_nodes;
     hipStream_t st = srv->stream;
+    // the job's copy of the scene follows the scene's own queued set-up work
+    if (int rc0 = order_before_trace(sc->ctx, st); rc0 != ST_OK) return rc0;
     const st_job_desc& d = j->desc;
     auto& pool = srv->work_pool[d.scene_slot];
     if (!pool.empty()) {

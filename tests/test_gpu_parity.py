@@ -105,6 +105,30 @@ def test_small_flux_maps_in_copies_match_oracle(gpu_ctx, oracle_mod, scene):
     assert ray_outcomes_balance(g["stats"])
 
 
+def test_trace_on_caller_stream_follows_scene_setup(gpu_ctx):
+    """Scene set-up (sun plane, candidate grid) runs on the context's stream; a
+    trace on the caller's stream right after it waits for that work. A fresh
+    scene's grid memory holds garbage until its fill pass has run."""
+    import torch
+    w = scenes.parabolic_dish()
+    kw = dict(seed=4, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny)
+    ref_sc = tracer.Scene(gpu_ctx, w.elements)
+    ref_sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=_abi.POS_STRATIFIED)
+    ref = ref_sc.trace(2_000_000, **kw).numpy()
+    stream = torch.cuda.Stream(device=gpu_ctx.device)
+    for _ in range(2):
+        sc = tracer.Scene(gpu_ctx, w.elements)
+        sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=_abi.POS_STRATIFIED)
+        desc = sc.make_desc(2_000_000, zero=True, stream=stream.cuda_stream, **kw)
+        res = sc.alloc_outputs(desc)
+        torch.cuda.synchronize()
+        sc.trace_into(desc, res)
+        stream.synchronize()
+        got = res.numpy()
+        for k in ("flux", "element_hits", "stats"):
+            assert np.array_equal(got[k], ref[k]), k
+
+
 def test_field_10k_parity(gpu_ctx, oracle_mod):
     w = scenes.heliostat_field(10_000)
     n = 200_000
