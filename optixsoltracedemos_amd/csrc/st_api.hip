@@ -159,9 +159,7 @@ extern "C" int st_context_create(int device, st_context* out) {
     auto ctx = std::make_unique<st_context_s>();
     ctx->device = device;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
-    if (hipEventCreateWithFlags(&ctx->setup_done, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->trace_done, hipEventDisableTiming) != hipSuccess) {
-        if (ctx->setup_done) (void)hipEventDestroy(ctx->setup_done);
+    if (hipEventCreateWithFlags(&ctx->setup_done, hipEventDisableTiming) != hipSuccess) {
         (void)hipStreamDestroy(ctx->stream);
         return fail(ST_ERR_DEVICE, "hipEventCreate failed");
     }
@@ -176,7 +174,6 @@ void context_release(st_context ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipEventDestroy(ctx->setup_done);
-    (void)hipEventDestroy(ctx->trace_done);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -188,19 +185,15 @@ static bool capturing(hipStream_t s) {
 
 int order_before_trace(st_context ctx, hipStream_t user) {
     if (!user || user == ctx->stream || capturing(user)) return ST_OK;
+    // nothing queued on the context stream: no dependency (a wait would cost
+    // the caller's stream a barrier in front of the trace)
+    if (hipStreamQuery(ctx->stream) == hipSuccess) return ST_OK;
     std::lock_guard<std::mutex> lk(ctx->order_mu);
     HIP_TRY(hipEventRecord(ctx->setup_done, ctx->stream), "set-up event");
     HIP_TRY(hipStreamWaitEvent(user, ctx->setup_done, 0), "wait for set-up");
     return ST_OK;
 }
 
-int order_after_trace(st_context ctx, hipStream_t user) {
-    if (!user || user == ctx->stream || capturing(user)) return ST_OK;
-    std::lock_guard<std::mutex> lk(ctx->order_mu);
-    HIP_TRY(hipEventRecord(ctx->trace_done, user), "trace event");
-    HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->trace_done, 0), "wait for trace");
-    return ST_OK;
-}
 }  // namespace st
 
 extern "C" int st_context_destroy(st_context ctx) {
@@ -766,6 +759,7 @@ extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* ou
         ST_TRY(scene_ensure_lists(s, d->ray_count ? d->ray_count : d->num_rays - d->ray_begin, false));
     JobDev J;
     ST_TRY(fill_job(s, d, out, s->d_prims.ptr, s->d_nodes.ptr, s->d_sun.ptr, &J));
+    set_trace_order(&J, s->sun.position_sampling);
     hipStream_t st = d->stream ? (hipStream_t)d->stream : s->ctx->stream;
     ST_TRY(order_before_trace(s->ctx, st));
     if ((d->flags & ST_TRACE_ZERO_OUTPUTS) && out) {
@@ -783,7 +777,6 @@ extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* ou
     if (J.flux) ST_TRY(route_flux_copies(s->d_flux_copies, &J, 1, J.flux_bins, J.ray_count, st, &copies));
     HIP_TRY(launch_trace(st, nullptr, nullptr, 1, chunks, (uint32_t*)s->d_counter.ptr, sh, J), "trace kernel");
     if (copies > 1) HIP_TRY(launch_flux_gather(st, J.flux, copies, J.flux_bins, 1, flux_out), "flux gather");
-    ST_TRY(order_after_trace(s->ctx, st));
     if (!d->stream) HIP_TRY(hipStreamSynchronize(st), "trace sync");
     return ST_OK;
 }
@@ -832,6 +825,7 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
     ST_TRY(resolve_flux_element(s, -1, 256, 128, &fslot));
     if (fslot < 0) d.flux_nx = d.flux_ny = 0;
     ST_TRY(fill_job(s, &d, &o, s->d_prims.ptr, s->d_nodes.ptr, s->d_sun.ptr, &J));
+    set_trace_order(&J, s->sun.position_sampling);
     uint32_t copies = 1;
     if (J.flux) ST_TRY(route_flux_copies(s->d_flux_copies, &J, 1, J.flux_bins, sample_rays, st, &copies));
     hipEvent_t e0, e1;
@@ -1160,6 +1154,7 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
             ST_TRY(fill_job(s, &td, &o, (const float4*)b.d_prims.ptr + (size_t)k * n * 4,
                             (const float4*)b.d_nodes.ptr + (size_t)k * s->n_nodes * 4,
                             (const SunFrame*)b.d_sun.ptr + k, &jobs[k]));
+            set_trace_order(&jobs[k], b.sun.position_sampling);
             // grid pointers are set by each run's grid scan; lists of step k
             // are indexed by element within the step's range block
             jobs[k].grid_start = nullptr;
@@ -1204,7 +1199,6 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
     } else {
         ST_TRY(batch_enqueue(s, st));
     }
-    ST_TRY(order_after_trace(s->ctx, st));
     if (!d->stream) HIP_TRY(hipStreamSynchronize(st), "batch sync");
     return ST_OK;
 }

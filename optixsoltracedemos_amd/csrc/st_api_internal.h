@@ -85,8 +85,7 @@ int route_flux_copies(DevBuf& buf, JobDev* jobs, int32_t n, uint64_t bins, uint6
 struct st_context_s {
     int device = 0;
     hipStream_t stream = nullptr;  // scene set-up work (sun planes, grids, lists, refits)
-    // ordering between the context stream and a caller's trace stream
-    hipEvent_t setup_done = nullptr, trace_done = nullptr;
+    hipEvent_t setup_done = nullptr;  // orders a caller's trace stream after queued set-up work
     std::mutex order_mu;
     std::atomic<int> refs{1};
 };
@@ -97,13 +96,13 @@ constexpr double kListSavingPerRay = 2.0e-11;      // seconds a list saves per t
 void context_retain(st_context ctx);
 void context_release(st_context ctx);
 // A call that traces on a caller's stream (not the context's) first makes that
-// stream wait for the set-up work already queued on the context stream (sun
-// plane, grid, lists of the scene it reads), and afterwards makes the context
-// stream wait for the trace, so that later set-up work cannot overwrite what
-// the trace still reads. Events only, no host wait; skipped while the caller's
-// stream is being captured into a graph (the caller orders such work).
+// stream wait for the set-up work still queued on the context stream (sun
+// plane, grid, lists of the scene it reads): an event, no host wait; skipped
+// while the caller's stream is being captured into a graph (the caller orders
+// such work). The other direction is the caller's: set-up calls on a scene
+// (sun, aim, elements, lists) must not overlap a trace of it still running on
+// the caller's stream.
 int order_before_trace(st_context ctx, hipStream_t user);
-int order_after_trace(st_context ctx, hipStream_t user);
 }  // namespace st
 
 namespace st {

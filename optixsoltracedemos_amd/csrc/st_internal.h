@@ -28,6 +28,8 @@ struct JobDev {
     int32_t flux_slot;       // primitive slot of the flux element (-1 none)
     int32_t agg_slot;        // receiver whose hit count is kept in registers (-1 none)
     int32_t flux_nx, flux_ny;
+    uint32_t order_head;     // Halton trace order (set_trace_order): slots before the first whole block
+    uint32_t order_blocks;   // whole blocks of kHaltonBlock indices traced cell by cell (0: index order)
     uint32_t flux_bins;      // flux_nx * flux_ny
     uint32_t flux_copy_mask; // flux holds flux_copy_mask + 1 copies of the map; wave w adds to copy w & mask
     int64_t* flux;
@@ -44,6 +46,23 @@ struct JobDev {
                                  // list entry was that item (begin > end: none). Null: BVH
     const int32_t* pair_items;
 };
+
+// Halton sun positions: indices congruent modulo kHaltonCells = 2^8 * 3^5 share
+// the first 8 binary digits of a and the first 5 ternary digits of b, so their
+// sun-plane points lie in one grid column and two adjacent rows. A job's range
+// is traced block by block of kHaltonBlock aligned indices, 64 rays of one such
+// class per wave, when its indices (the sequence's 32-bit index) stay below 2^32.
+constexpr uint32_t kHaltonCells = 256u * 243u, kHaltonBlock = 64u * kHaltonCells;
+inline void set_trace_order(JobDev* J, int32_t position_sampling) {
+    J->order_head = 0;
+    J->order_blocks = 0;
+    const uint64_t b = J->ray_begin, e = J->ray_begin + J->ray_count;
+    if (position_sampling != ST_POS_HALTON || e > (1ull << 32)) return;
+    const uint64_t b1 = (b + kHaltonBlock - 1) / kHaltonBlock * kHaltonBlock, e1 = e / kHaltonBlock * kHaltonBlock;
+    if (e1 <= b1) return;
+    J->order_head = (uint32_t)(b1 - b);
+    J->order_blocks = (uint32_t)((e1 - b1) / kHaltonBlock);
+}
 
 constexpr int kGridRes = 256;  // sun-plane candidate grid, the stratified-sampling cells
 constexpr int kGridCells = kGridRes * kGridRes;

@@ -140,15 +140,34 @@ __device__ __forceinline__ float fast_atan2(float y, float x) {
 __device__ __forceinline__ float halton2(uint32_t i) {
     return (float)((double)__brev(i) * 2.3283064365386962890625e-10);
 }
+// Base-3 radical inverse num / 3^digits of i, five digits at a time: a block
+// r of five ternary digits (least significant first) contributes its reversal
+// kRev5[r]. Trailing zero digits scale num and den alike, so the quotient, a
+// correctly rounded double division of the same rational, is the digit-by-digit
+// one's exactly; i < 2^32 has at most 21 digits (five blocks, den = 3^25).
+struct Rev5Table {
+    uint8_t v[243];
+    constexpr Rev5Table() : v() {
+        for (int r = 0; r < 243; ++r) {
+            int x = r, rev = 0;
+            for (int d = 0; d < 5; ++d) {
+                rev = rev * 3 + x % 3;
+                x /= 3;
+            }
+            v[r] = (uint8_t)rev;
+        }
+    }
+};
+__constant__ Rev5Table kRev5 = Rev5Table();
 __device__ __forceinline__ float halton3(uint32_t i) {
-    uint64_t num = 0, den = 1;
-    while (i > 0) {
-        uint32_t q = i / 3u;
-        num = num * 3u + (i - 3u * q);
-        den *= 3u;
+    uint64_t num = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+        const uint32_t q = i / 243u;
+        num = num * 243u + kRev5.v[i - 243u * q];
         i = q;
     }
-    return den > 1 ? (float)((double)num / (double)den) : 0.f;
+    return (float)((double)num / 847288609443.0);  // 3^25
 }
 
 // cell of the stratified sun-plane sampling: 64 consecutive rays per cell of a
@@ -741,6 +760,24 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
     return true;
 }
 
+// Ray index of slot s of a job's range; slots are traced in order. Inside the
+// range's whole blocks of kHaltonBlock aligned indices (Halton positions only,
+// order_blocks > 0) a block is traced cell by cell: its 64 slots 64k..64k+63
+// hold the indices base + k + j * kHaltonCells, j < 64, which share a Halton
+// cell. Elsewhere slot s is index ray_begin + s.
+__device__ __forceinline__ uint64_t ray_of_slot(const JobDev& J, uint64_t s) {
+    if (J.order_blocks && s >= J.order_head) {
+        const uint64_t t = s - J.order_head;
+        if (t < (uint64_t)J.order_blocks * kHaltonBlock) {
+            const uint32_t u = (uint32_t)t;  // the range lies below 2^32
+            const uint32_t q = u / kHaltonBlock, r = u - q * kHaltonBlock;
+            return J.ray_begin + J.order_head + (uint64_t)q * kHaltonBlock + (uint64_t)(r & 63u) * kHaltonCells +
+                   (r >> 6);
+        }
+    }
+    return J.ray_begin + s;
+}
+
 // empty slots are all-zero (the reference memsets the buffer, lib/soltrace_system.cpp:125)
 __device__ __forceinline__ void finish_ray(const JobDev& J, uint64_t ray, int last) {
     if (J.records)
@@ -810,7 +847,6 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
         if (QUEUED) {
             float* q = lds_queue[threadIdx.x >> 6];
             uint32_t qn = 0;  // wave-uniform queue length, < kQueueCap between steps
-            const uint64_t base_ray = J.ray_begin + first;
             bool alive = false;  // this lane holds a ray with a next segment
             V3 o = v3(0.f, 0.f, 0.f), d = o;
             uint32_t off = 0;    // its offset in the chunk
@@ -838,7 +874,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
             auto step = [&]() {
                 if (wl[0] > kLogCap - kLogHeadroom) log_flush(wl, J);
                 if (alive) {
-                    const uint64_t ray = base_ray + off;
+                    const uint64_t ray = ray_of_slot(J, first + off);
                     int last = depth;
                     float tmin = kTminBounce;
                     alive = trace_segment<BLOCK>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
@@ -851,7 +887,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                 const uint64_t i = b + (uint64_t)lane;
                 alive = false;
                 if (i < end) {
-                    const uint64_t ray = J.ray_begin + i;
+                    const uint64_t ray = ray_of_slot(J, i);
                     uint32_t cell;
                     sun_ray(J, S, ray, wc, o, d, cell);
                     float tmin = kTminPrimary;
@@ -898,7 +934,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                 if (wl[0] > kLogCap / 2) log_flush(wl, J);
                 const uint64_t i = i0 + (uint64_t)lane;
                 if (i >= end) continue;
-                const uint64_t ray = J.ray_begin + i;
+                const uint64_t ray = ray_of_slot(J, i);
                 V3 o, d;
                 uint32_t cell;
                 sun_ray(J, S, ray, wc, o, d, cell);

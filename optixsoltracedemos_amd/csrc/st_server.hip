@@ -437,6 +437,7 @@ void worker_main(st_server srv) {
             J.ray_begin = round[i]->dispatched;
             J.ray_count = slice[i];
             J.record_base = J.ray_begin;
+            set_trace_order(&J, round[i]->desc.sun.position_sampling);
             srv->h_jobs[i] = J;
             srv->h_prefix[i + 1] = srv->h_prefix[i] + (slice[i] + sh.chunk - 1) / sh.chunk;
         }

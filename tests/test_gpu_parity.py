@@ -129,6 +129,31 @@ def test_trace_on_caller_stream_follows_scene_setup(gpu_ctx):
             assert np.array_equal(got[k], ref[k]), k
 
 
+@pytest.mark.parametrize("begin", [0, 777])
+def test_halton_cell_order_equals_index_order(gpu_ctx, begin):
+    """Halton positions: a range spanning whole blocks of 64 * 2^8 * 3^5 aligned
+    indices is traced cell by cell; the same range traced as slices shorter
+    than a block (index order) gives the same map, counts and statistics."""
+    block = 64 * 256 * 243
+    w = scenes.three_heliostats()
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=_abi.POS_HALTON)
+    kw = dict(max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny, seed=2)
+    count = 2 * block + 4321
+    total = begin + count
+    one = sc.trace(total, ray_begin=begin, ray_count=count, **kw).numpy()
+    res = None
+    for b in range(begin, total, 3_000_000):
+        d = sc.make_desc(total, ray_begin=b, ray_count=min(3_000_000, total - b), zero=res is None, **kw)
+        if res is None:
+            res = sc.alloc_outputs(d)
+        sc.trace_into(d, res)
+    sliced = res.numpy()
+    for k in ("flux", "element_hits", "stats"):
+        assert np.array_equal(one[k], sliced[k]), k
+    assert one["stats"][0] == count
+
+
 def test_field_10k_parity(gpu_ctx, oracle_mod):
     w = scenes.heliostat_field(10_000)
     n = 200_000
