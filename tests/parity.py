@@ -28,6 +28,34 @@ def compare_records(g_rec, g_id, o_rec, o_id, per_ray=False):
     return frac, float(err.max()), same
 
 
+def compare_directions(g_rec, g_id, o_rec, o_id):
+    """Per-ray largest angle (radians) between the GPU's and the oracle's
+    segment directions, over rays whose element sequences agree. A segment runs
+    between consecutive recorded points: the sun-plane origin (slot 0), then
+    every hit; its direction is the unit vector from one point to the next."""
+    g_id = np.asarray(g_id).astype(np.uint32)
+    o_id = np.asarray(o_id).astype(np.uint32)
+    same = np.all(g_id == o_id, axis=1)
+    ids = g_id[same]
+    g = np.asarray(g_rec, dtype=np.float64)[same][..., 1:]
+    o = np.asarray(o_rec, dtype=np.float64)[same][..., 1:]
+    if len(g) == 0:
+        return np.zeros(0)
+    valid = ids != 0xFFFFFFFF
+    seg = valid[:, :-1] & valid[:, 1:]
+
+    def unit(v):
+        n = np.linalg.norm(v, axis=-1, keepdims=True)
+        return v / np.where(n > 0, n, 1.0)
+
+    dg = unit(g[:, 1:] - g[:, :-1])
+    do = unit(o[:, 1:] - o[:, :-1])
+    cross = np.linalg.norm(np.cross(dg, do), axis=-1)
+    ang = np.arctan2(cross, np.sum(dg * do, axis=-1))
+    ang = np.where(seg, ang, 0.0)
+    return ang.max(axis=1)
+
+
 def flux_l1(a, b) -> float:
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

@@ -9,6 +9,9 @@ Bar (stated here, checked below):
     scene scale: near-tangent cylinder / paraboloid hits are ill-conditioned,
     their FP32 intersection parameter errs like the square root of the rounding;
   * per-element counts and statistics differ only by rays whose IDs differ;
+  * segment directions (consecutive recorded points: sun-plane origin, hits)
+    of agreeing rays within DIR_TOL radians for all but DIR_OUTLIER_MAX of
+    them, every one within DIR_TOL_GRAZE;
   * flux maps: L1(gpu - oracle) / hits <= FLUX_L1_MAX (bin flips at edges).
 Full-size runs are checked through size-independent properties.
 """
@@ -16,7 +19,7 @@ import numpy as np
 import pytest
 
 from optixsoltracedemos_amd import _abi, scenes, tracer
-from tests.parity import compare_records, flux_l1, ray_outcomes_balance
+from tests.parity import compare_directions, compare_records, flux_l1, ray_outcomes_balance
 
 pytestmark = pytest.mark.gpu
 
@@ -25,6 +28,9 @@ POS_TOL_REL = 2e-6
 POS_OUTLIER_MAX = 1e-4
 POS_TOL_GRAZE = 1e-4
 FLUX_L1_MAX = 2e-3
+DIR_TOL = 1e-4          # radians: segment directions of agreeing rays
+DIR_OUTLIER_MAX = 1e-4  # fraction of rays allowed beyond DIR_TOL (grazing hits)
+DIR_TOL_GRAZE = 2e-2    # radians: every agreeing ray
 
 
 def scene_scale(w):
@@ -60,6 +66,10 @@ def check_parity(w, g, o, n):
     assert dh <= 4 * bad
     assert flux_l1(g["flux"], o["flux"]) <= FLUX_L1_MAX
     assert ray_outcomes_balance(g["stats"])
+    ang = compare_directions(g["records"], g["record_elements"], o["records"], o["record_elements"])
+    beyond = float(np.mean(ang > DIR_TOL)) if len(ang) else 0.0
+    assert beyond <= DIR_OUTLIER_MAX, f"{beyond:.2e} of rays beyond {DIR_TOL} rad (max {ang.max():.3e})"
+    assert ang.max() <= DIR_TOL_GRAZE, f"max direction error {ang.max():.3e} rad"
 
 
 @pytest.mark.parametrize("pos", [_abi.POS_PHILOX, _abi.POS_HALTON, _abi.POS_STRATIFIED])
