@@ -6,7 +6,7 @@ bins, at most 4 interactions per ray). Scene, BVH and outputs are resident in
 HBM before timing starts. Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--rays 1e8]
-                  [--no-cpu-baseline] [--cpu-sample 4e7] [--no-serve]
+                  [--no-cpu-baseline] [--cpu-sample N] [--no-serve]
 """
 from __future__ import annotations
 
@@ -61,17 +61,31 @@ def read_traffic():
 
 
 def cpu_baseline(workload, sample: int, pos: int):
+    """The oracle timed over rays [0, sample) of the workload; returns the
+    baseline record and the oracle's outputs (flux, element hits, statistics)."""
     from oracle import oracle as O
     threads = max(1, min(os.cpu_count() or 1, 16))
     s = O.Scene(workload.elements)
     s.set_sun(workload.sun_vector, workload.sun_param, workload.sun_shape, position_sampling=pos)
     t0 = time.perf_counter()
-    s.trace(workload.num_rays, ray_begin=0, ray_count=sample, max_depth=workload.max_depth,
-            flux_nx=workload.flux_nx, flux_ny=workload.flux_ny, threads=threads)
+    out = s.trace(workload.num_rays, ray_begin=0, ray_count=sample, max_depth=workload.max_depth,
+                  flux_nx=workload.flux_nx, flux_ny=workload.flux_ny, threads=threads)
     dt = time.perf_counter() - t0
-    return {"value": sample / dt, "unit": "rays/s", "cores": threads, "kind": "port",
-            "sample": f"rays [0, {sample}) of the same 1e8-ray field workload, FP64 scalar oracle, "
-                      f"{threads} threads, {dt:.2f} s wall (~{dt * threads:.0f} thread-seconds)"}
+    whole = "all" if sample == workload.num_rays else "the first"
+    rec = {"value": sample / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+           "sample": f"{whole} {sample:.0e} rays of the same {workload.num_rays:.0e}-ray field workload, "
+                     f"FP64 scalar oracle, {threads} threads, {dt:.2f} s wall (~{dt * threads:.0f} thread-seconds)"}
+    return rec, out
+
+
+def flux_accuracy(gpu, ora, rays: int) -> dict:
+    """One GPU launch against the oracle over the same rays (shared Philox draws)."""
+    import numpy as np
+    g, o = gpu["flux"].astype(np.float64), ora["flux"].astype(np.float64)
+    dh = np.abs(gpu["element_hits"].astype(np.int64) - ora["element_hits"].astype(np.int64)).sum()
+    return {"rays": rays, "flux_l1_vs_oracle": float(np.abs(g - o).sum() / max(o.sum(), 1.0)),
+            "receiver_hits_gpu": int(gpu["stats"][3]), "receiver_hits_oracle": int(ora["stats"][3]),
+            "element_hits_l1_per_ray": float(dh) / rays}
 
 
 def main():
@@ -82,7 +96,9 @@ def main():
     ap.add_argument("--rays", type=float, default=1e8)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=float, default=4e7)
+    ap.add_argument("--cpu-sample", type=float, default=0,
+                    help="rays the oracle traces for the CPU baseline (default: the whole workload, "
+                         "which also gives the flux-map accuracy of one GPU launch)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-vs-plain annual sweep")
     ap.add_argument("--no-autotune", action="store_true")
@@ -105,8 +121,8 @@ def main():
         print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": a.numel() * 4}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (rays shard by index range; "
-              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (rays would shard by index "
+              "range, with one integer all-reduce of the flux map). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -208,7 +224,10 @@ def main():
         except Exception as ex:  # reported, never silently replaced
             line["sweep"] = {"error": f"{type(ex).__name__}: {ex}"}
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(w, int(args.cpu_sample), pos)
+        sample = int(args.cpu_sample) or rays
+        line["cpu_baseline"], ora = cpu_baseline(w, sample, pos)
+        if sample == rays:
+            line["accuracy"] = flux_accuracy(wres.numpy(), ora, rays)
     print(json.dumps(line), flush=True)
 
 

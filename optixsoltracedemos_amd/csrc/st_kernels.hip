@@ -549,6 +549,10 @@ __device__ __forceinline__ void add_one_grouped(uint32_t idx, Emit emit) {
 // end), each a full-width vector atomic. One trace step adds at most
 // kLogHeadroom entries (a lane adds at most two).
 constexpr uint32_t kLogCap = 320, kLogHeadroom = 128;
+// rank of the calling lane among the set lanes of m below it
+__device__ __forceinline__ uint32_t lane_rank(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
 // the calling wave's copy of the flux map
 __device__ __forceinline__ int64_t* wave_flux(const JobDev& J) {
     const uint32_t w = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -564,15 +568,65 @@ __device__ __forceinline__ void log_add(uint32_t* log, const JobDev& J, bool flu
     }
     gadd(flux ? (void*)(wave_flux(J) + idx) : (void*)(J.hits + idx), (unsigned long long)n);
 }
-// every lane of the wave calls it (uniform control flow)
-__device__ __forceinline__ void log_flush(uint32_t* log, const JobDev& J) {
-    const uint32_t n = min(log[0], kLogCap);
+// sends the first n entries; every lane of the wave calls it (uniform control flow)
+__device__ __forceinline__ void log_send(const uint32_t* log, const JobDev& J, uint32_t n) {
     int64_t* const fl = J.flux ? wave_flux(J) : nullptr;
     for (uint32_t i = threadIdx.x & 63; i < n; i += 64) {
         const uint32_t e = log[1 + i], idx = e & 0x1FFFFFFu, cnt = ((e >> 25) & 63u) + 1u;
         gadd((e >> 31) ? (void*)(fl + idx) : (void*)(J.hits + idx), (unsigned long long)cnt);
     }
+}
+__device__ __forceinline__ void log_flush(uint32_t* log, const JobDev& J) {
+    log_send(log, J, min(log[0], kLogCap));
     if ((threadIdx.x & 63) == 0) log[0] = 0;
+}
+
+// The queued form logs where the wave has reconverged instead: a segment
+// leaves at most one pending element hit and one pending flux bin per lane
+// (kNoEntry: none), and the wave appends them after the segment with every
+// lane present. The log's length is then wave-uniform and lives in a scalar
+// register (len), so no LDS atomic with its round trip reserves the slots.
+// Element hits are grouped by element (a coherent wave hits few mirrors);
+// flux bins take one entry per lane, grouped over two rounds on small maps,
+// where a wave's receiver hits share bins. Adds at most 128 entries.
+constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
+__device__ __forceinline__ void log_pending(uint32_t* log, uint32_t& len, const JobDev& J, uint32_t ph, uint32_t pf) {
+    const uint32_t lane = threadIdx.x & 63u;
+    bool done = ph == kNoEntry;
+    for (unsigned long long left = __ballot(!done); left; left = __ballot(!done)) {
+        const uint32_t leader = (uint32_t)__ffsll((long long)left) - 1u;
+        const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)ph, (int)leader);
+        const bool mine = !done && ph == e;
+        const uint32_t n = (uint32_t)__popcll(__ballot(mine));
+        if (e < (1u << 25)) {
+            if (lane == leader) log[1 + len] = ((n - 1u) << 25) | e;
+            ++len;
+        } else if (lane == leader) {
+            gadd(J.hits + e, (unsigned long long)n);
+        }
+        done = done || mine;
+    }
+    done = pf == kNoEntry;
+    if (J.flux_bins < kFluxDirectBins) {
+        for (int r = 0; r < 2; ++r) {
+            const unsigned long long left = __ballot(!done);
+            if (!left) break;
+            const uint32_t leader = (uint32_t)__ffsll((long long)left) - 1u;
+            const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)pf, (int)leader);
+            const bool mine = !done && pf == b;
+            const uint32_t n = (uint32_t)__popcll(__ballot(mine));
+            if (lane == leader) log[1 + len] = (1u << 31) | ((n - 1u) << 25) | b;
+            ++len;
+            done = done || mine;
+        }
+    }
+    const unsigned long long m = __ballot(!done);
+    if (J.flux_bins <= (1u << 25)) {
+        if (!done) log[1 + len + lane_rank(m)] = (1u << 31) | pf;
+        len += (uint32_t)__popcll(m);
+    } else if (!done) {
+        gadd(wave_flux(J) + pf, 1ull);
+    }
 }
 
 __device__ __forceinline__ void record(const JobDev& J, uint64_t ray, int slot, float tag, V3 p, uint32_t elem) {
@@ -634,10 +688,12 @@ __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame
 // One segment of a ray: closest hit from o along d, then the hit element's
 // program. Returns true when the ray continues (o, d, tmin, depth updated for
 // the next segment), false when it ended.
-template <int STRIDE>
+// DEFER: element hits and flux bins are left in ph / pf for log_pending
+// instead of being logged here.
+template <int STRIDE, bool DEFER>
 __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int32_t* stk, uint32_t* wc,
                                               uint32_t* wh, uint32_t* wl, V3& o, V3& d, float& tmin, int& depth,
-                                              int& last, SegList L, int& key) {
+                                              int& last, SegList L, int& key, uint32_t& ph, uint32_t& pf) {
     const uint32_t rlo = (uint32_t)ray, rhi = (uint32_t)(ray >> 32);
     float t = kTmax;
     int slot = -1, cap = 0;
@@ -667,6 +723,7 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
             count_if(wc, kAggCounter, slot == J.agg_slot);
             if (slot != J.agg_slot && J.hits) {
                 if (wh && elem < (uint32_t)kLdsHitElements) atomicAdd(wh + elem, 1u);
+                else if (DEFER) ph = elem;
                 else log_add(wl, J, false, elem, 1u);
             }
             count(wc, ST_STAT_RECEIVER_HITS);
@@ -693,7 +750,9 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
                 ix = min(max(ix, 0), J.flux_nx - 1);
                 iy = min(max(iy, 0), J.flux_ny - 1);
                 // a wave's hits share bins only on small maps
-                if (J.flux_bins < kFluxDirectBins)
+                if (DEFER)
+                    pf = (uint32_t)(iy * J.flux_nx + ix);
+                else if (J.flux_bins < kFluxDirectBins)
                     add_one_grouped<2>((uint32_t)(iy * J.flux_nx + ix),
                                        [&](uint32_t b, uint32_t n) { log_add(wl, J, true, b, n); });
                 else
@@ -714,6 +773,7 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
     last = nd;
     if (J.hits) {
         if (wh && elem < (uint32_t)kLdsHitElements) atomicAdd(wh + elem, 1u);
+        else if (DEFER) ph = elem;
         else add_one_grouped<64>(elem, [&](uint32_t e, uint32_t n) { log_add(wl, J, false, e, n); });
     }
     count(wc, ST_STAT_MIRROR_HITS);
@@ -808,9 +868,6 @@ __device__ __forceinline__ uint32_t queue_tag(uint32_t offset, int depth, int ke
     const uint32_t f = (uint32_t)(key + 1) < (1u << 17) ? (uint32_t)(key + 1) : 0u;
     return offset | ((uint32_t)depth << 11) | (f << 15);
 }
-__device__ __forceinline__ uint32_t lane_rank(unsigned long long m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
 
 // LHITS > 0: element hit counts of scenes with at most LHITS elements are
 // kept per wave in LDS and added to the output once per chunk.
@@ -844,6 +901,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
         const ST_CONST SunFrame& S = *(const ST_CONST SunFrame*)J.sun;
         const uint64_t first = cj * (uint64_t)chunk_rays;
         const uint64_t end = min(first + (uint64_t)chunk_rays, J.ray_count);
+        uint32_t len = 0;  // queued form: entries in the wave's log (wave-uniform)
         if (QUEUED) {
             float* q = lds_queue[threadIdx.x >> 6];
             uint32_t qn = 0;  // wave-uniform queue length, < kQueueCap between steps
@@ -871,21 +929,30 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                 qn -= take;
             };
             // the next segment of every held ray
+            auto flush_if_full = [&]() {
+                if (len > kLogCap - kLogHeadroom) {
+                    log_send(wl, J, len);
+                    len = 0;
+                }
+            };
             auto step = [&]() {
-                if (wl[0] > kLogCap - kLogHeadroom) log_flush(wl, J);
+                flush_if_full();
+                uint32_t ph = kNoEntry, pf = kNoEntry;
                 if (alive) {
                     const uint64_t ray = ray_of_slot(J, first + off);
                     int last = depth;
                     float tmin = kTminBounce;
-                    alive = trace_segment<BLOCK>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
-                                                 next_list(J, depth, key), key);
+                    alive = trace_segment<BLOCK, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
+                                                       next_list(J, depth, key), key, ph, pf);
                     if (!alive) finish_ray(J, ray, last);
                 }
+                log_pending(wl, len, J, ph, pf);
             };
             for (uint64_t b = first; b < end; b += 64) {
-                if (wl[0] > kLogCap - kLogHeadroom) log_flush(wl, J);
+                flush_if_full();
                 const uint64_t i = b + (uint64_t)lane;
                 alive = false;
+                uint32_t ph = kNoEntry, pf = kNoEntry;
                 if (i < end) {
                     const uint64_t ray = ray_of_slot(J, i);
                     uint32_t cell;
@@ -895,10 +962,11 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                     depth = 0;
                     key = -1;
                     off = (uint32_t)(i - first);
-                    alive = trace_segment<BLOCK>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last, cell_list(J, cell),
-                                                 key);
+                    alive = trace_segment<BLOCK, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
+                                                       cell_list(J, cell), key, ph, pf);
                     if (!alive) finish_ray(J, ray, last);
                 }
+                log_pending(wl, len, J, ph, pf);
                 // full waves while held and parked rays fill one
                 for (;;) {
                     const uint32_t na = (uint32_t)__popcll(__ballot(alive));
@@ -940,15 +1008,18 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                 sun_ray(J, S, ray, wc, o, d, cell);
                 float tmin = kTminPrimary;
                 int depth = 0, last = 0, key = -1;
-                if (trace_segment<BLOCK>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last, cell_list(J, cell), key))
-                    while (trace_segment<BLOCK>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last, next_list(J, depth, key),
-                                                key)) {
+                uint32_t ph = kNoEntry, pf = kNoEntry;  // unused: this form logs as it goes
+                if (trace_segment<BLOCK, false>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last, cell_list(J, cell),
+                                                key, ph, pf))
+                    while (trace_segment<BLOCK, false>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
+                                                       next_list(J, depth, key), key, ph, pf)) {
                     }
                 finish_ray(J, ray, last);
             }
         }
         // flush this chunk's counters (same wave wrote them: LDS order suffices)
-        log_flush(wl, J);
+        if (QUEUED) log_send(wl, J, len);
+        else log_flush(wl, J);
         if (LHITS > 0)
             for (int i = lane; i < LHITS; i += 64) {
                 const uint32_t h = wh[i];

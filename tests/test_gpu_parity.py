@@ -383,6 +383,30 @@ def test_full_size_field_properties(gpu_ctx):
     assert abs(ab - 0.07 * m) < 5 * np.sqrt(m * 0.07 * 0.93)
 
 
+def test_full_size_field_matches_oracle(gpu_ctx, oracle_mod):
+    """The headline workload at its full size (1e8 stratified rays, 10k field,
+    256x128 map) against the oracle over the same 1e8 rays: flux L1 / hits
+    within FLUX_L1_MAX (bin flips of edge-grazing rays; the accuracy target of
+    this workload is 5e-3), per-element counts and statistics within the
+    grazing-ray fraction ID_MISMATCH_MAX."""
+    w = scenes.heliostat_field(10_000)
+    n = w.num_rays
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=_abi.POS_STRATIFIED)
+    g = sc.trace(n, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny).numpy()
+    os_ = oracle_mod.Scene(w.elements)
+    os_.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=_abi.POS_STRATIFIED)
+    o = os_.trace(n, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny, threads=16)
+    l1 = flux_l1(g["flux"], o["flux"])
+    print(f"full-size flux L1 = {l1:.2e}, receiver hits gpu {int(g['stats'][3])} oracle {int(o['stats'][3])}")
+    assert l1 <= FLUX_L1_MAX, l1
+    dh = np.abs(g["element_hits"].astype(np.int64) - o["element_hits"].astype(np.int64)).sum()
+    assert dh <= 4 * ID_MISMATCH_MAX * n, dh
+    ds = np.abs(g["stats"].astype(np.int64)[:11] - o["stats"].astype(np.int64)[:11])
+    assert ds.max() <= 2 * ID_MISMATCH_MAX * n, ds
+    assert ray_outcomes_balance(g["stats"]) and ray_outcomes_balance(o["stats"])
+    sc.close()
+
 def test_flux_to_watts_normalisation(gpu_ctx):
     import torch
     w = scenes.heliostat_field(1000)
