@@ -1562,6 +1562,71 @@ __device__ __forceinline__ void sun_reflection_axis(const double* __restrict__ f
     for (int k = 0; k < 3; ++k) a[k] /= la;
 }
 
+// A flat mirror's rays leave from its rectangle R, not from its whole box, and
+// the set they sweep, S = R + {t u : t >= 0, angle(u, a) <= phi}, is convex. A
+// candidate box that a plane separates from S cannot be reached: the walk's
+// candidates are filtered with planes of normal m and m . u <= 0 for every
+// cone direction u, so that the largest m . x over S is the largest over R.
+// For any unit w normal to a, m = cos(phi) w - sin(phi) a qualifies; w runs
+// over the direction from R's centre to the box centre and both senses of R's
+// two axes, each taken normal to a. The mirror plane (normal oriented towards
+// a) qualifies when every cone direction leaves on its side, and drops boxes
+// wholly behind the mirror. R is widened by eps (FP32 hit points and the FP32
+// rectangle lie within it) and phi by 1e-6 rad (FP32 directions). Returns the
+// number kept, compacted in place.
+__device__ __forceinline__ uint32_t cull_by_rectangle(const st_element_f64& E, const double* __restrict__ f,
+                                                      const Cone& C, double phi, const float* __restrict__ bounds_slot,
+                                                      int32_t* items, uint32_t len) {
+    const double X[3] = {f[0], f[3], f[6]}, Y[3] = {f[1], f[4], f[7]}, N[3] = {f[2], f[5], f[8]};
+    const double* pc = E.origin;
+    const double hw = 0.5 * E.dim_x, hh = 0.5 * E.dim_y;
+    const double eps = 1e-3 + 2e-6 * (fabs(pc[0]) + fabs(pc[1]) + fabs(pc[2]) + hw + hh);
+    const double ph = phi + 1e-6, cph = cos(ph), sph = sin(ph);
+    auto dot3 = [](const double* u, const double* v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+    auto max_r = [&](const double m[3]) { return dot3(m, pc) + fabs(dot3(m, X)) * hw + fabs(dot3(m, Y)) * hh + eps; };
+    auto min_b = [&](const double m[3], const float* b) {
+        double v = 0.0;
+        for (int k = 0; k < 3; ++k) v += fmin(m[k] * (double)b[k], m[k] * (double)b[3 + k]);
+        return v;
+    };
+    // m from a direction v: w = unit part of v normal to a (none if tiny)
+    auto plane_of = [&](const double v[3], double sign, double m[3]) {
+        const double va = dot3(v, C.a);
+        double w[3] = {sign * (v[0] - va * C.a[0]), sign * (v[1] - va * C.a[1]), sign * (v[2] - va * C.a[2])};
+        const double l = sqrt(dot3(w, w));
+        if (l < 1e-9) return false;
+        for (int k = 0; k < 3; ++k) m[k] = cph * w[k] / l - sph * C.a[k];
+        return true;
+    };
+    double M[5][3];
+    int nm = 0;
+    for (int sgn = -1; sgn <= 1; sgn += 2) {
+        if (plane_of(X, (double)sgn, M[nm])) ++nm;
+        if (plane_of(Y, (double)sgn, M[nm])) ++nm;
+    }
+    const double na = dot3(N, C.a);
+    if (fabs(na) >= sph) {
+        const double s = na > 0.0 ? -1.0 : 1.0;
+        for (int k = 0; k < 3; ++k) M[nm][k] = s * N[k];
+        ++nm;
+    }
+    uint32_t kept = 0;
+    for (uint32_t i = 0; i < len; ++i) {
+        const int32_t v = items[i];
+        const float* b = bounds_slot + 6 * (size_t)v;
+        bool cut = false;
+        for (int j = 0; j < nm && !cut; ++j) cut = max_r(M[j]) < min_b(M[j], b);
+        if (!cut) {
+            const double d[3] = {0.5 * ((double)b[0] + b[3]) - pc[0], 0.5 * ((double)b[1] + b[4]) - pc[1],
+                                 0.5 * ((double)b[2] + b[5]) - pc[2]};
+            double m[3];
+            if (plane_of(d, 1.0, m)) cut = max_r(m) < min_b(m, b);
+        }
+        if (!cut) items[kept++] = v;
+    }
+    return kept;
+}
+
 // Task t of scene state k = t / tasks_per_step, i = t % tasks_per_step: first
 // reflections off element i (pair_items == null), or second reflections of
 // rays that left element pair_owner[t] (< 0: none) and hit the primitive
@@ -1638,12 +1703,14 @@ __global__ __launch_bounds__(kConeBlock) void k_cone_lists(const st_element_f64*
     const double reach = 2.0 * sqrt(ex * ex + ey * ey + ez * ez) + 2.0 * kConeFirst;
     C.ratio = pow(reach / kConeFirst, 1.0 / (kConePieces - 2));
     int32_t* list = scratch + (size_t)t * kMaxBounceList;
-    const uint32_t len = cone_walk<kConeBlock>(nodes, bounds_slot, bounds_slot + 6 * (size_t)elem_slot[from], C,
-                                               list + 1, kMaxBounceList - 1, lds_stack + threadIdx.x);
+    uint32_t len = cone_walk<kConeBlock>(nodes, bounds_slot, bounds_slot + 6 * (size_t)elem_slot[from], C,
+                                         list + 1, kMaxBounceList - 1, lds_stack + threadIdx.x);
     if (len > kMaxBounceList - 1) {
         none();
         return;
     }
+    if (elems[from].surface == ST_SURFACE_FLAT) len = cull_by_rectangle(elems[from], frames + 9 * (size_t)from, C, phi,
+                                                                        bounds_slot, list + 1, len);
     uint32_t nf = 0;  // flat candidates first
     for (uint32_t i = 0; i < len; ++i) {
         const int32_t v = list[1 + i];
