@@ -442,7 +442,8 @@ int scene_grid(st_scene s) {
     ST_TRY(s->d_grid_stats.ensure(sizeof(uint32_t) * 2, "grid stats"));
     const double tmax = sun_cone_tan(s->sun);
     HIP_TRY(launch_fill_u64(st, s->d_grid_counts.ptr, kGridCells / 2, 0), "grid clear");
-    HIP_TRY(launch_grid_count(st, (const float*)s->d_bounds.ptr, s->n, 1, (const SunFrame*)s->d_sun.ptr, tmax,
+    HIP_TRY(launch_grid_count(st, (const float*)s->d_bounds.ptr, (const float4*)s->d_prims.ptr,
+                              (const uint32_t*)s->d_meta.ptr, s->n, 1, (const SunFrame*)s->d_sun.ptr, tmax,
                               (uint32_t*)s->d_grid_counts.ptr, (uint32_t*)s->d_grid_start.ptr,
                               (uint32_t*)s->d_grid_stats.ptr, nullptr, nullptr, 0),
             "grid count kernels");
@@ -454,7 +455,8 @@ int scene_grid(st_scene s) {
     if (!grid_lists_pay(stats[0], stats[1])) return ST_OK;
     ST_TRY(s->d_grid_items.ensure(sizeof(int32_t) * stats[0], "grid items"));
     HIP_TRY(launch_fill_u64(st, s->d_grid_cursor.ptr, kGridCells / 2, 0), "grid cursor clear");
-    HIP_TRY(launch_grid_fill(st, (const float*)s->d_bounds.ptr, s->n, 1, (const SunFrame*)s->d_sun.ptr, tmax,
+    HIP_TRY(launch_grid_fill(st, (const float*)s->d_bounds.ptr, (const float4*)s->d_prims.ptr,
+                             (const uint32_t*)s->d_meta.ptr, s->n, 1, (const SunFrame*)s->d_sun.ptr, tmax,
                              (const uint32_t*)s->d_grid_start.ptr, (uint32_t*)s->d_grid_cursor.ptr,
                              (int32_t*)s->d_grid_items.ptr, stats[0]),
             "grid fill kernel");
@@ -1000,7 +1002,8 @@ static int batch_plan_lists(st_scene s, hipStream_t st) {
     ST_TRY(b.d_grid_stats.ensure(sizeof(uint32_t) * 2 * (size_t)K, "batch grid stats"));
     ST_TRY(batch_scene_states(s, st));
     HIP_TRY(launch_fill_u64(st, b.d_grid_counts.ptr, (uint64_t)K * kGridCells / 2, 0), "batch grid clear");
-    HIP_TRY(launch_grid_count(st, (const float*)b.d_bounds.ptr, n, K, (const SunFrame*)b.d_sun.ptr, sun_cone_tan(b.sun),
+    HIP_TRY(launch_grid_count(st, (const float*)b.d_bounds.ptr, (const float4*)b.d_prims.ptr,
+                              (const uint32_t*)s->d_meta.ptr, n, K, (const SunFrame*)b.d_sun.ptr, sun_cone_tan(b.sun),
                               (uint32_t*)b.d_grid_counts.ptr, (uint32_t*)b.d_grid_start.ptr,
                               (uint32_t*)b.d_grid_stats.ptr, nullptr, nullptr, 0),
             "batch grid count");
@@ -1027,13 +1030,15 @@ static int batch_enqueue(st_scene s, hipStream_t st) {
     ST_TRY(batch_scene_states(s, st));
     if (b.grids) {
         HIP_TRY(launch_fill_u64(st, b.d_grid_counts.ptr, (uint64_t)K * kGridCells / 2, 0), "batched grid clear");
-        HIP_TRY(launch_grid_count(st, (const float*)b.d_bounds.ptr, n, K, (const SunFrame*)b.d_sun.ptr,
+        HIP_TRY(launch_grid_count(st, (const float*)b.d_bounds.ptr, (const float4*)b.d_prims.ptr,
+                                  (const uint32_t*)s->d_meta.ptr, n, K, (const SunFrame*)b.d_sun.ptr,
                                   sun_cone_tan(b.sun), (uint32_t*)b.d_grid_counts.ptr, (uint32_t*)b.d_grid_start.ptr,
                                   (uint32_t*)b.d_grid_stats.ptr, (JobDev*)b.d_jobs.ptr,
                                   (const int32_t*)b.d_grid_items.ptr, b.grid_cap),
                 "batched grid count");
         HIP_TRY(launch_fill_u64(st, b.d_grid_cursor.ptr, (uint64_t)K * kGridCells / 2, 0), "batched grid cursor clear");
-        HIP_TRY(launch_grid_fill(st, (const float*)b.d_bounds.ptr, n, K, (const SunFrame*)b.d_sun.ptr,
+        HIP_TRY(launch_grid_fill(st, (const float*)b.d_bounds.ptr, (const float4*)b.d_prims.ptr,
+                                 (const uint32_t*)s->d_meta.ptr, n, K, (const SunFrame*)b.d_sun.ptr,
                                  sun_cone_tan(b.sun), (const uint32_t*)b.d_grid_start.ptr,
                                  (uint32_t*)b.d_grid_cursor.ptr, (int32_t*)b.d_grid_items.ptr, b.grid_cap),
                 "batched grid fill");

@@ -126,12 +126,14 @@ hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk
 // longest cell list. With jobs, job k's grid pointers are set (items at
 // items + k * item_cap) when grid_lists_pay and the total fits item_cap, else
 // cleared.
-hipError_t launch_grid_count(hipStream_t s, const float* bounds, int32_t n, int32_t steps, const SunFrame* sun,
-                             double tan_max, uint32_t* counts, uint32_t* start, uint32_t* stats, JobDev* jobs,
-                             const int32_t* items, uint32_t item_cap);
+// prims: the steps' primitive records (n * 4 float4 per step), meta shared.
+hipError_t launch_grid_count(hipStream_t s, const float* bounds, const float4* prims, const uint32_t* meta, int32_t n,
+                             int32_t steps, const SunFrame* sun, double tan_max, uint32_t* counts, uint32_t* start,
+                             uint32_t* stats, JobDev* jobs, const int32_t* items, uint32_t item_cap);
 // cursor zeroed (steps * kGridCells u32); steps whose total exceeds item_cap are skipped
-hipError_t launch_grid_fill(hipStream_t s, const float* bounds, int32_t n, int32_t steps, const SunFrame* sun,
-                            double tan_max, const uint32_t* start, uint32_t* cursor, int32_t* items, uint32_t item_cap);
+hipError_t launch_grid_fill(hipStream_t s, const float* bounds, const float4* prims, const uint32_t* meta, int32_t n,
+                            int32_t steps, const SunFrame* sun, double tan_max, const uint32_t* start, uint32_t* cursor,
+                            int32_t* items, uint32_t item_cap);
 // candidate lists of reflected rays (k_cone_lists). Task t belongs to scene
 // state k = t / tasks_per_step (frames n*9, bounds n*6, nodes n_nodes*4 and one
 // SunFrame per state) and is, with i = t % tasks_per_step: first reflections

@@ -1307,8 +1307,32 @@ __global__ __launch_bounds__(256) void k_sun_plane(const double* __restrict__ re
 // the plane, widened by the largest lateral drift of a ray in the sun cone over
 // the axial distance from the plane (tan_max * (distance - corner . sun)) plus
 // 1 mm and 1e-6 of the plane's extent for FP32 ray origins.
-__device__ __forceinline__ bool grid_range(const float* __restrict__ bounds, int slot, const SunFrame& F,
-                                           double tan_max, int r[4]) {
+// A flat primitive is listed more tightly: a sun ray hits it only at its
+// rectangle, whose projection along the sun is a parallelogram (centre c,
+// half-edge vectors ex, ey, in metres along the plane's unit axes); widened by
+// the same margin plus the rectangle's FP32 allowance (1 mm and 2e-6 of its
+// coordinates, as the traversal boxes), it bounds the cell range, and a cell
+// is listed only if no edge normal of the parallelogram separates the two
+// (cell_meets).
+struct Footprint {
+    int r[4];
+    int normals;             // 0: the cell range only; 2: parallelogram test
+    double c[2], n[2][2], h[2];  // centre, unit edge normals, reach along each normal
+    double cell[2];          // cell size in metres along the plane's axes
+};
+__device__ __forceinline__ bool cell_meets(const Footprint& f, int ix, int iy) {
+    const double x = (ix + 0.5) * f.cell[0] - f.c[0], y = (iy + 0.5) * f.cell[1] - f.c[1];
+    for (int j = 0; j < f.normals; ++j) {
+        const double ext = fabs(f.n[j][0]) * 0.5 * f.cell[0] + fabs(f.n[j][1]) * 0.5 * f.cell[1];
+        if (fabs(f.n[j][0] * x + f.n[j][1] * y) > f.h[j] + ext) return false;
+    }
+    return true;
+}
+__device__ __forceinline__ bool grid_range(const float* __restrict__ bounds, const float4* __restrict__ prims,
+                                           const uint32_t* __restrict__ meta, int slot, const SunFrame& F,
+                                           double tan_max, Footprint& fp) {
+    int* r = fp.r;
+    fp.normals = 0;
     const double e1[3] = {F.v1[0] - F.v0[0], F.v1[1] - F.v0[1], F.v1[2] - F.v0[2]};
     const double e2[3] = {F.v3[0] - F.v0[0], F.v3[1] - F.v0[1], F.v3[2] - F.v0[2]};
     const double l1 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
@@ -1329,30 +1353,70 @@ __device__ __forceinline__ bool grid_range(const float* __restrict__ bounds, int
     }
     const double extent = sqrt(l1) + sqrt(l2) + fabs(F.v0[0]) + fabs(F.v0[1]) + fabs(F.v0[2]);
     const double m = fmax(F.distance - smin, 0.0) * tan_max + 1e-3 + 1e-6 * extent;
-    const double ma = m / sqrt(l1), mb = m / sqrt(l2);
-    if (amax + ma < 0.0 || amin - ma > 1.0 || bmax + mb < 0.0 || bmin - mb > 1.0) return false;
-    r[0] = (int)fmax(0.0, floor((amin - ma) * kGridRes));
-    r[1] = (int)fmin(kGridRes - 1.0, floor((amax + ma) * kGridRes));
-    r[2] = (int)fmax(0.0, floor((bmin - mb) * kGridRes));
-    r[3] = (int)fmin(kGridRes - 1.0, floor((bmax + mb) * kGridRes));
+    const double L1 = sqrt(l1), L2 = sqrt(l2), ma = m / L1, mb = m / L2;
+    amin -= ma;
+    amax += ma;
+    bmin -= mb;
+    bmax += mb;
+    if (meta_kind(meta[slot]) == kKindFlat) {
+        const float4 q0 = prims[4 * slot], q1 = prims[4 * slot + 1], q2 = prims[4 * slot + 2];
+        const double hw = q0.w, hh = q1.w;
+        const double cen[3] = {q0.x, q0.y, q0.z}, ax[3] = {q1.x, q1.y, q1.z}, ay[3] = {q2.x, q2.y, q2.z};
+        const double u1[3] = {e1[0] / L1, e1[1] / L1, e1[2] / L1}, u2[3] = {e2[0] / L2, e2[1] / L2, e2[2] / L2};
+        const double q[3] = {cen[0] - F.v0[0], cen[1] - F.v0[1], cen[2] - F.v0[2]};
+        auto d3 = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+        const double M = m + 1e-3 + 2e-6 * (fabs(cen[0]) + fabs(cen[1]) + fabs(cen[2]) + hw + hh);
+        const double ex[2] = {hw * d3(ax, u1), hw * d3(ax, u2)}, ey[2] = {hh * d3(ay, u1), hh * d3(ay, u2)};
+        fp.c[0] = d3(q, u1);
+        fp.c[1] = d3(q, u2);
+        amin = fmax(amin, (fp.c[0] - fabs(ex[0]) - fabs(ey[0]) - M) / L1);
+        amax = fmin(amax, (fp.c[0] + fabs(ex[0]) + fabs(ey[0]) + M) / L1);
+        bmin = fmax(bmin, (fp.c[1] - fabs(ex[1]) - fabs(ey[1]) - M) / L2);
+        bmax = fmin(bmax, (fp.c[1] + fabs(ex[1]) + fabs(ey[1]) + M) / L2);
+        const double lx = sqrt(ex[0] * ex[0] + ex[1] * ex[1]), ly = sqrt(ey[0] * ey[0] + ey[1] * ey[1]);
+        // an edge seen end-on gives no normal; a normal that exists bounds the
+        // parallelogram by the other edge's reach
+        if (lx > 1e-9 && ly > 1e-9) {
+            fp.n[0][0] = -ex[1] / lx;
+            fp.n[0][1] = ex[0] / lx;
+            fp.h[0] = fabs(fp.n[0][0] * ey[0] + fp.n[0][1] * ey[1]) + M;
+            fp.n[1][0] = -ey[1] / ly;
+            fp.n[1][1] = ey[0] / ly;
+            fp.h[1] = fabs(fp.n[1][0] * ex[0] + fp.n[1][1] * ex[1]) + M;
+            fp.cell[0] = L1 / kGridRes;
+            fp.cell[1] = L2 / kGridRes;
+            fp.normals = 2;
+        }
+    }
+    if (amax < 0.0 || amin > 1.0 || bmax < 0.0 || bmin > 1.0) return false;
+    r[0] = (int)fmax(0.0, floor(amin * kGridRes));
+    r[1] = (int)fmin(kGridRes - 1.0, floor(amax * kGridRes));
+    r[2] = (int)fmax(0.0, floor(bmin * kGridRes));
+    r[3] = (int)fmin(kGridRes - 1.0, floor(bmax * kGridRes));
     return r[0] <= r[1] && r[2] <= r[3];
 }
 
 // kGridParts threads per (step, primitive): part p visits every kGridParts-th
 // cell of the primitive's footprint, so large footprints do not serialize
 constexpr int kGridParts = 16;
-__global__ void k_grid_count(const float* __restrict__ bounds, int32_t n, int32_t steps,
+// bounds and prims of step k at k * n * 6 floats and k * n * 4 float4; meta shared
+__global__ void k_grid_count(const float* __restrict__ bounds, const float4* __restrict__ prims,
+                             const uint32_t* __restrict__ meta, int32_t n, int32_t steps,
                              const SunFrame* __restrict__ sun, double tan_max, uint32_t* __restrict__ counts) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (int64_t)n * steps * kGridParts) return;
     const int part = (int)(g % kGridParts);
     const int64_t e = g / kGridParts;
     const int k = (int)(e / n), slot = (int)(e % n);
-    int r[4];
-    if (!grid_range(bounds + (size_t)k * n * 6, slot, sun[k], tan_max, r)) return;
+    Footprint fp;
+    if (!grid_range(bounds + (size_t)k * n * 6, prims + (size_t)k * n * 4, meta, slot, sun[k], tan_max, fp)) return;
+    const int* r = fp.r;
     uint32_t* C = counts + (size_t)k * kGridCells;
     const int w = r[1] - r[0] + 1, cells = w * (r[3] - r[2] + 1);
-    for (int i = part; i < cells; i += kGridParts) atomicAdd(&C[(r[2] + i / w) * kGridRes + r[0] + i % w], 1u);
+    for (int i = part; i < cells; i += kGridParts) {
+        const int ix = r[0] + i % w, iy = r[2] + i / w;
+        if (cell_meets(fp, ix, iy)) atomicAdd(&C[iy * kGridRes + ix], 1u);
+    }
 }
 
 // exclusive prefix sum of the kGridRes^2 counts of step blockIdx.x (one block
@@ -1405,7 +1469,8 @@ __global__ __launch_bounds__(1024) void k_grid_scan(const uint32_t* __restrict__
 
 // items of step k at items_all + k * item_cap; a step whose lists overflow
 // item_cap is skipped (its job gets no grid)
-__global__ void k_grid_fill(const float* __restrict__ bounds, int32_t n, int32_t steps,
+__global__ void k_grid_fill(const float* __restrict__ bounds, const float4* __restrict__ prims,
+                            const uint32_t* __restrict__ meta, int32_t n, int32_t steps,
                             const SunFrame* __restrict__ sun, double tan_max, const uint32_t* __restrict__ start_all,
                             uint32_t* __restrict__ cursor_all, int32_t* __restrict__ items_all, uint32_t item_cap) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1415,13 +1480,16 @@ __global__ void k_grid_fill(const float* __restrict__ bounds, int32_t n, int32_t
     const int k = (int)(e / n), slot = (int)(e % n);
     const uint32_t* start = start_all + (size_t)k * (kGridCells + 1);
     if (start[kGridCells] > item_cap) return;
-    int r[4];
-    if (!grid_range(bounds + (size_t)k * n * 6, slot, sun[k], tan_max, r)) return;
+    Footprint fp;
+    if (!grid_range(bounds + (size_t)k * n * 6, prims + (size_t)k * n * 4, meta, slot, sun[k], tan_max, fp)) return;
+    const int* r = fp.r;
     uint32_t* cursor = cursor_all + (size_t)k * kGridCells;
     int32_t* items = items_all + (size_t)k * item_cap;
     const int w = r[1] - r[0] + 1, cells = w * (r[3] - r[2] + 1);
     for (int i = part; i < cells; i += kGridParts) {
-        const int c = (r[2] + i / w) * kGridRes + r[0] + i % w;
+        const int ix = r[0] + i % w, iy = r[2] + i / w;
+        if (!cell_meets(fp, ix, iy)) continue;
+        const int c = iy * kGridRes + ix;
         items[start[c] + atomicAdd(&cursor[c], 1u)] = slot;
     }
 }
@@ -1934,23 +2002,24 @@ hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_grid_count(hipStream_t s, const float* bounds, int32_t n, int32_t steps, const SunFrame* sun,
-                             double tan_max, uint32_t* counts, uint32_t* start, uint32_t* stats, JobDev* jobs,
-                             const int32_t* items, uint32_t item_cap) {
+hipError_t launch_grid_count(hipStream_t s, const float* bounds, const float4* prims, const uint32_t* meta, int32_t n,
+                             int32_t steps, const SunFrame* sun, double tan_max, uint32_t* counts, uint32_t* start,
+                             uint32_t* stats, JobDev* jobs, const int32_t* items, uint32_t item_cap) {
     const int64_t total = (int64_t)n * steps * kGridParts;
-    hipLaunchKernelGGL(k_grid_count, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, bounds, n, steps, sun,
-                       tan_max, counts);
+    hipLaunchKernelGGL(k_grid_count, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, bounds, prims, meta, n,
+                       steps, sun, tan_max, counts);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_grid_scan, dim3(steps), dim3(1024), 0, s, counts, start, stats, jobs, items, item_cap);
     return hipGetLastError();
 }
 
-hipError_t launch_grid_fill(hipStream_t s, const float* bounds, int32_t n, int32_t steps, const SunFrame* sun,
-                            double tan_max, const uint32_t* start, uint32_t* cursor, int32_t* items, uint32_t item_cap) {
+hipError_t launch_grid_fill(hipStream_t s, const float* bounds, const float4* prims, const uint32_t* meta, int32_t n,
+                            int32_t steps, const SunFrame* sun, double tan_max, const uint32_t* start, uint32_t* cursor,
+                            int32_t* items, uint32_t item_cap) {
     const int64_t total = (int64_t)n * steps * kGridParts;
-    hipLaunchKernelGGL(k_grid_fill, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, bounds, n, steps, sun,
-                       tan_max, start, cursor, items, item_cap);
+    hipLaunchKernelGGL(k_grid_fill, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, bounds, prims, meta, n,
+                       steps, sun, tan_max, start, cursor, items, item_cap);
     return hipGetLastError();
 }
 

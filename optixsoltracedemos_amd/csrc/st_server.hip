@@ -214,7 +214,8 @@ int setup_job(st_server srv, ServerJob* j, int slot) {
     if ((e = launch_fill_u64(st, o.stats.ptr, ST_STAT_COUNT, 0)) != hipSuccess) return hip_fail(e, "job zero");
     if ((e = launch_fill_u64(st, srv->d_grid_counts.ptr, kGridCells / 2, 0)) != hipSuccess)
         return hip_fail(e, "job grid clear");
-    if ((e = launch_grid_count(st, (const float*)w.bounds.ptr, sc->n, 1, (const SunFrame*)w.sun.ptr,
+    if ((e = launch_grid_count(st, (const float*)w.bounds.ptr, (const float4*)w.prims.ptr,
+                               (const uint32_t*)sc->d_meta.ptr, sc->n, 1, (const SunFrame*)w.sun.ptr,
                                std::tan(sun_cone_angle(sun)), (uint32_t*)srv->d_grid_counts.ptr,
                                (uint32_t*)w.grid_start.ptr, (uint32_t*)srv->d_grid_stats.ptr + 2 * slot, nullptr,
                                nullptr, 0)) != hipSuccess)
@@ -239,7 +240,8 @@ int finish_setup(st_server srv, ServerJob* j, uint32_t grid_total, uint32_t grid
         if ((rc = w.grid_items.ensure(sizeof(int32_t) * grid_total, "job grid items")) != ST_OK) return rc;
         if ((e = launch_fill_u64(st, srv->d_grid_cursor.ptr, kGridCells / 2, 0)) != hipSuccess)
             return hip_fail(e, "job grid cursor clear");
-        if ((e = launch_grid_fill(st, (const float*)w.bounds.ptr, sc->n, 1, (const SunFrame*)w.sun.ptr,
+        if ((e = launch_grid_fill(st, (const float*)w.bounds.ptr, (const float4*)w.prims.ptr,
+                                  (const uint32_t*)sc->d_meta.ptr, sc->n, 1, (const SunFrame*)w.sun.ptr,
                                   std::tan(sun_cone_angle(sun)), (const uint32_t*)w.grid_start.ptr,
                                   (uint32_t*)srv->d_grid_cursor.ptr, (int32_t*)w.grid_items.ptr, grid_total)) !=
             hipSuccess)
