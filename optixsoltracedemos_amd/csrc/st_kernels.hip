@@ -1636,12 +1636,13 @@ __device__ __forceinline__ void sun_reflection_axis(const double* __restrict__ f
 // candidates are filtered with planes of normal m and m . u <= 0 for every
 // cone direction u, so that the largest m . x over S is the largest over R.
 // For any unit w normal to a, m = cos(phi) w - sin(phi) a qualifies; w runs
-// over the direction from R's centre to the box centre and both senses of R's
-// two axes, each taken normal to a. The mirror plane (normal oriented towards
-// a) qualifies when every cone direction leaves on its side, and drops boxes
-// wholly behind the mirror. R is widened by eps (FP32 hit points and the FP32
-// rectangle lie within it) and phi by 1e-6 rad (FP32 directions). Returns the
-// number kept, compacted in place.
+// over the separating axes of the two outlines seen along a (a x v for R's
+// axes and the three world axes, both senses) and the direction from R's
+// centre to the box centre normal to a. Also qualifying: -a (boxes wholly
+// behind R along a) and the mirror plane, oriented towards a, when every cone
+// direction leaves on its side (boxes wholly behind the mirror). R is widened
+// by eps (FP32 hit points and the FP32 rectangle lie within it) and phi by
+// 1e-6 rad (FP32 directions). Returns the number kept, compacted in place.
 __device__ __forceinline__ uint32_t cull_by_rectangle(const st_element_f64& E, const double* __restrict__ f,
                                                       const Cone& C, double phi, const float* __restrict__ bounds_slot,
                                                       int32_t* items, uint32_t len) {
@@ -1666,11 +1667,21 @@ __device__ __forceinline__ uint32_t cull_by_rectangle(const st_element_f64& E, c
         for (int k = 0; k < 3; ++k) m[k] = cph * w[k] / l - sph * C.a[k];
         return true;
     };
-    double M[5][3];
+    double M[12][3];
     int nm = 0;
-    for (int sgn = -1; sgn <= 1; sgn += 2) {
-        if (plane_of(X, (double)sgn, M[nm])) ++nm;
-        if (plane_of(Y, (double)sgn, M[nm])) ++nm;
+    {
+        const double* A = C.a;
+        const double ax_[3][3] = {{0.0, -A[2], A[1]}, {A[2], 0.0, -A[0]}, {-A[1], A[0], 0.0}};  // a x world axes
+        const double axX[3] = {A[1] * X[2] - A[2] * X[1], A[2] * X[0] - A[0] * X[2], A[0] * X[1] - A[1] * X[0]};
+        const double axY[3] = {A[1] * Y[2] - A[2] * Y[1], A[2] * Y[0] - A[0] * Y[2], A[0] * Y[1] - A[1] * Y[0]};
+        for (int sgn = -1; sgn <= 1; sgn += 2) {
+            if (plane_of(axX, (double)sgn, M[nm])) ++nm;
+            if (plane_of(axY, (double)sgn, M[nm])) ++nm;
+            for (int k = 0; k < 3; ++k)
+                if (plane_of(ax_[k], (double)sgn, M[nm])) ++nm;
+        }
+        for (int k = 0; k < 3; ++k) M[nm][k] = -A[k];
+        ++nm;
     }
     const double na = dot3(N, C.a);
     if (fabs(na) >= sph) {
