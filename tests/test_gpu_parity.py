@@ -282,7 +282,7 @@ def test_flux_copies_cleared_between_calls(gpu_ctx):
 
 
 @pytest.mark.parametrize("case", ["field_stratified", "field_philox", "field_low_sun", "field_curved_rough",
-                                  "dish_gaussian", "three_halton"])
+                                  "field_flat_rough", "dish_gaussian", "three_halton"])
 def test_candidate_lists_equal_bvh(gpu_ctx, case):
     """Hits found through the candidate lists (sun-plane cells for sun rays,
     per-mirror lists for first reflections) are exactly what a full BVH
@@ -297,6 +297,14 @@ def test_candidate_lists_equal_bvh(gpu_ctx, case):
         w.elements["slope_error"][m], w.elements["spec_error"][m] = 4e-3, 1e-3
         w.sun_shape, w.sun_param = _abi.SUN_GAUSSIAN, 2.51e-3
         n, pos = 1_000_000, _abi.POS_STRATIFIED
+    elif case == "field_flat_rough":
+        # flat mirrors, rotated about their normals, wide reflection cones: the
+        # quarter lists and the rectangle culling of flat mirrors at their limits
+        w = scenes.heliostat_field(3_000, elevation=22.0, azimuth=75.0)
+        m = w.elements["role"] == _abi.ROLE_MIRROR
+        w.elements["zrot_deg"][m] = np.linspace(-40.0, 40.0, int(m.sum()))
+        w.elements["slope_error"][m], w.elements["spec_error"][m] = 5e-3, 2e-3
+        n, pos = 1_000_000, _abi.POS_PHILOX
     elif case.startswith("field"):
         w = scenes.heliostat_field(10_000, elevation=12.0 if case == "field_low_sun" else 60.0)
         n = 1_000_000
