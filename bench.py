@@ -74,7 +74,14 @@ def cpu_baseline(workload, sample: int, pos: int):
     whole = "all" if sample == workload.num_rays else "the first"
     rec = {"value": sample / dt, "unit": "rays/s", "cores": threads, "kind": "port",
            "sample": f"{whole} {sample:.0e} rays of the same {workload.num_rays:.0e}-ray field workload, "
-                     f"FP64 scalar oracle, {threads} threads, {dt:.2f} s wall (~{dt * threads:.0f} thread-seconds)"}
+                     f"FP64 scalar oracle, {threads} threads, {dt:.2f} s wall (~{dt * threads:.0f} thread-seconds)",
+           "host_cpus": os.cpu_count()}
+    # one thread on a smaller prefix of the same rays
+    one = min(sample, 2_000_000)
+    t0 = time.perf_counter()
+    s.trace(workload.num_rays, ray_begin=0, ray_count=one, max_depth=workload.max_depth,
+            flux_nx=workload.flux_nx, flux_ny=workload.flux_ny, threads=1)
+    rec["single_thread"] = {"value": one / (time.perf_counter() - t0), "unit": "rays/s", "rays": one}
     return rec, out
 
 
