@@ -8,7 +8,8 @@ batched annual sweep at their full budgets, and served jobs on scenes of 3,
 No records at these sizes: flux maps, per-element counts and statistics, to the
 bar of test_gpu_parity: flux L1 / hits <= FLUX_L1_MAX (bin flips of
 edge-grazing rays), per-element counts and statistics within the grazing-ray
-fraction ID_MISMATCH_MAX of the rays.
+fraction ID_MISMATCH_MAX of the rays. Per-ray records are compared over the
+first 1e7 rays of the headline workload.
 """
 import numpy as np
 import pytest
@@ -107,3 +108,26 @@ def test_served_jobs_full_size(gpu_ctx, oracle_mod):
         check_full({"flux": fl, "element_hits": ht, "stats": res[i]["stats"]}, o, rays[i], f"served {n_h} heliostats")
         srv.release(h)
     srv.close()
+
+
+def test_field_per_ray_parity_1e7(gpu_ctx, oracle_mod):
+    """Per-ray parity (element sequences, hit points, segment directions) over
+    the first 1e7 rays of the headline workload (10k field, stratified
+    positions, 1e8-ray job), in five slices of 2e6 rays traced by index range
+    on both sides, to the bar of test_gpu_parity."""
+    from tests.test_gpu_parity import check_parity
+    w = scenes.heliostat_field(10_000)
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=_abi.POS_STRATIFIED)
+    sc.build_lists()  # the headline's path: sun-plane grid, reflection and pair lists
+    os_ = oracle_mod.Scene(w.elements)
+    os_.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=_abi.POS_STRATIFIED)
+    step = 2_000_000
+    for begin in range(0, 10_000_000, step):
+        kw = dict(max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny, ray_begin=begin, ray_count=step,
+                  records=True)
+        g = sc.trace(w.num_rays, **kw).numpy()
+        o = os_.trace(w.num_rays, threads=ORACLE_THREADS, **kw)
+        print(f"rays [{begin}, {begin + step}): checking", flush=True)
+        check_parity(w, g, o, step)
+    sc.close()

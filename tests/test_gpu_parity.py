@@ -68,6 +68,9 @@ def check_parity(w, g, o, n):
     assert ray_outcomes_balance(g["stats"])
     ang = compare_directions(g["records"], g["record_elements"], o["records"], o["record_elements"])
     beyond = float(np.mean(ang > DIR_TOL)) if len(ang) else 0.0
+    print(f"parity over {n} rays: element sequences differ {bad}, hit point error max {err.max():.2e} m "
+          f"(beyond {tight:.1e} m: {outliers:.1e}), direction error max {ang.max():.2e} rad "
+          f"(beyond {DIR_TOL:g}: {beyond:.1e})", flush=True)
     assert beyond <= DIR_OUTLIER_MAX, f"{beyond:.2e} of rays beyond {DIR_TOL} rad (max {ang.max():.3e})"
     assert ang.max() <= DIR_TOL_GRAZE, f"max direction error {ang.max():.3e} rad"
 
