@@ -213,4 +213,194 @@ ST_INLINE U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
 // 23-bit uniform ((x >> 9) + 0.5) / 2^23 in (0,1); u and 1 - u are exact in FP32
 ST_INLINE float u01(uint32_t x) { return ((float)(x >> 9) + 0.5f) * (1.0f / 8388608.0f); }
 
+// ---------------------------------------------------------------------------
+// The two conservative tests that shorten candidate lists. They run on the
+// device (k_grid_count / k_grid_fill, k_cone_lists) and are checked against
+// brute-force ray sampling on the host.
+
+constexpr int kGridRes = 256;  // sun-plane candidate grid, the stratified-sampling cells
+constexpr int kGridCells = kGridRes * kGridRes;
+
+// Sun-plane candidate grid: kGridRes x kGridRes cells over the sun plane's
+// (a, b) parameters. A primitive is listed in every cell from which a sun ray
+// can reach its traversal box: the box corners projected along the sun onto
+// the plane, widened by the largest lateral drift of a ray in the sun cone over
+// the axial distance from the plane (tan_max * (distance - corner . sun)) plus
+// 1 mm and 1e-6 of the plane's extent for FP32 ray origins.
+// A flat primitive is listed more tightly: a sun ray hits it only at its
+// rectangle, whose projection along the sun is a parallelogram (centre c,
+// half-edge vectors ex, ey, in metres along the plane's unit axes); widened by
+// the same margin plus the rectangle's FP32 allowance (1 mm and 2e-6 of its
+// coordinates, as the traversal boxes), it bounds the cell range, and a cell
+// is listed only if no edge normal of the parallelogram separates the two
+// (cell_meets).
+struct Footprint {
+    int r[4];
+    int normals;             // 0: the cell range only; 2: parallelogram test
+    double c[2], n[2][2], h[2];  // centre, unit edge normals, reach along each normal
+    double cell[2];          // cell size in metres along the plane's axes
+};
+ST_INLINE bool cell_meets(const Footprint& f, int ix, int iy) {
+    const double x = (ix + 0.5) * f.cell[0] - f.c[0], y = (iy + 0.5) * f.cell[1] - f.c[1];
+    for (int j = 0; j < f.normals; ++j) {
+        const double ext = fabs(f.n[j][0]) * 0.5 * f.cell[0] + fabs(f.n[j][1]) * 0.5 * f.cell[1];
+        if (fabs(f.n[j][0] * x + f.n[j][1] * y) > f.h[j] + ext) return false;
+    }
+    return true;
+}
+ST_INLINE bool grid_range(const float* __restrict__ bounds, const float4* __restrict__ prims,
+                                           const uint32_t* __restrict__ meta, int slot, const SunFrame& F,
+                                           double tan_max, Footprint& fp) {
+    int* r = fp.r;
+    fp.normals = 0;
+    const double e1[3] = {F.v1[0] - F.v0[0], F.v1[1] - F.v0[1], F.v1[2] - F.v0[2]};
+    const double e2[3] = {F.v3[0] - F.v0[0], F.v3[1] - F.v0[1], F.v3[2] - F.v0[2]};
+    const double l1 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+    const double l2 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+    if (!(l1 > 0.0) || !(l2 > 0.0)) return false;
+    double amin = 1e300, amax = -1e300, bmin = 1e300, bmax = -1e300, smin = 1e300;
+    for (int c = 0; c < 8; ++c) {
+        const double p[3] = {(double)bounds[6 * slot + ((c & 1) ? 3 : 0)], (double)bounds[6 * slot + 1 + ((c & 2) ? 3 : 0)],
+                             (double)bounds[6 * slot + 2 + ((c & 4) ? 3 : 0)]};
+        const double q[3] = {p[0] - F.v0[0], p[1] - F.v0[1], p[2] - F.v0[2]};
+        const double a = (q[0] * e1[0] + q[1] * e1[1] + q[2] * e1[2]) / l1;
+        const double b = (q[0] * e2[0] + q[1] * e2[1] + q[2] * e2[2]) / l2;
+        amin = fmin(amin, a);
+        amax = fmax(amax, a);
+        bmin = fmin(bmin, b);
+        bmax = fmax(bmax, b);
+        smin = fmin(smin, p[0] * F.sun[0] + p[1] * F.sun[1] + p[2] * F.sun[2]);
+    }
+    const double extent = sqrt(l1) + sqrt(l2) + fabs(F.v0[0]) + fabs(F.v0[1]) + fabs(F.v0[2]);
+    const double m = fmax(F.distance - smin, 0.0) * tan_max + 1e-3 + 1e-6 * extent;
+    const double L1 = sqrt(l1), L2 = sqrt(l2), ma = m / L1, mb = m / L2;
+    amin -= ma;
+    amax += ma;
+    bmin -= mb;
+    bmax += mb;
+    if (meta_kind(meta[slot]) == kKindFlat) {
+        const float4 q0 = prims[4 * slot], q1 = prims[4 * slot + 1], q2 = prims[4 * slot + 2];
+        const double hw = q0.w, hh = q1.w;
+        const double cen[3] = {q0.x, q0.y, q0.z}, ax[3] = {q1.x, q1.y, q1.z}, ay[3] = {q2.x, q2.y, q2.z};
+        const double u1[3] = {e1[0] / L1, e1[1] / L1, e1[2] / L1}, u2[3] = {e2[0] / L2, e2[1] / L2, e2[2] / L2};
+        const double q[3] = {cen[0] - F.v0[0], cen[1] - F.v0[1], cen[2] - F.v0[2]};
+        auto d3 = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+        const double M = m + 1e-3 + 2e-6 * (fabs(cen[0]) + fabs(cen[1]) + fabs(cen[2]) + hw + hh);
+        const double ex[2] = {hw * d3(ax, u1), hw * d3(ax, u2)}, ey[2] = {hh * d3(ay, u1), hh * d3(ay, u2)};
+        fp.c[0] = d3(q, u1);
+        fp.c[1] = d3(q, u2);
+        amin = fmax(amin, (fp.c[0] - fabs(ex[0]) - fabs(ey[0]) - M) / L1);
+        amax = fmin(amax, (fp.c[0] + fabs(ex[0]) + fabs(ey[0]) + M) / L1);
+        bmin = fmax(bmin, (fp.c[1] - fabs(ex[1]) - fabs(ey[1]) - M) / L2);
+        bmax = fmin(bmax, (fp.c[1] + fabs(ex[1]) + fabs(ey[1]) + M) / L2);
+        const double lx = sqrt(ex[0] * ex[0] + ex[1] * ex[1]), ly = sqrt(ey[0] * ey[0] + ey[1] * ey[1]);
+        // an edge seen end-on gives no normal; a normal that exists bounds the
+        // parallelogram by the other edge's reach
+        if (lx > 1e-9 && ly > 1e-9) {
+            fp.n[0][0] = -ex[1] / lx;
+            fp.n[0][1] = ex[0] / lx;
+            fp.h[0] = fabs(fp.n[0][0] * ey[0] + fp.n[0][1] * ey[1]) + M;
+            fp.n[1][0] = -ey[1] / ly;
+            fp.n[1][1] = ey[0] / ly;
+            fp.h[1] = fabs(fp.n[1][0] * ex[0] + fp.n[1][1] * ex[1]) + M;
+            fp.cell[0] = L1 / kGridRes;
+            fp.cell[1] = L2 / kGridRes;
+            fp.normals = 2;
+        }
+    }
+    if (amax < 0.0 || amin > 1.0 || bmax < 0.0 || bmin > 1.0) return false;
+    r[0] = (int)fmax(0.0, floor(amin * kGridRes));
+    r[1] = (int)fmin(kGridRes - 1.0, floor(amax * kGridRes));
+    r[2] = (int)fmax(0.0, floor(bmin * kGridRes));
+    r[3] = (int)fmin(kGridRes - 1.0, floor(bmax * kGridRes));
+    return r[0] <= r[1] && r[2] <= r[3];
+}
+
+// A flat mirror's rays leave from its rectangle R, not from its whole box, and
+// the set they sweep, S = R + {t u : t >= 0, angle(u, a) <= phi}, is convex. A
+// candidate box that a plane separates from S cannot be reached: the walk's
+// candidates are filtered with planes of normal m and m . u <= 0 for every
+// cone direction u, so that the largest m . x over S is the largest over R.
+// For any unit w normal to a, m = cos(phi) w - sin(phi) a qualifies; w runs
+// over the separating axes of the two outlines seen along a (a x v for R's
+// axes and the three world axes, both senses) and the direction from R's
+// centre to the box centre normal to a. Also qualifying: -a (boxes wholly
+// behind R along a) and the mirror plane, oriented towards a, when every cone
+// direction leaves on its side (boxes wholly behind the mirror). R is widened
+// by eps (FP32 hit points and the FP32 rectangle lie within it) and phi by
+// 1e-6 rad (FP32 directions). R: centre pc, half sizes hw, hh along the
+// frame's x and y axes (f: 3x3 row-major, columns x, y, z = normal); a: unit
+// cone axis.
+struct RectSweep {
+    double pc[3], X[3], Y[3];
+    double a[3];
+    double hw, hh, eps, cph, sph;
+    double M[12][3];  // the planes that do not depend on the box
+    int nm;
+};
+ST_INLINE double rs_dot(const double* u, const double* v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; }
+// largest m . x over R widened by eps
+ST_INLINE double rs_max_r(const RectSweep& S, const double m[3]) {
+    return rs_dot(m, S.pc) + fabs(rs_dot(m, S.X)) * S.hw + fabs(rs_dot(m, S.Y)) * S.hh + S.eps;
+}
+// smallest m . x over the box b = (lo, hi)
+ST_INLINE double rs_min_b(const double m[3], const float* b) {
+    double v = 0.0;
+    for (int k = 0; k < 3; ++k) v += fmin(m[k] * (double)b[k], m[k] * (double)b[3 + k]);
+    return v;
+}
+// m from a direction v: w = unit part of sign * v normal to a (none if tiny)
+ST_INLINE bool rs_plane_of(const RectSweep& S, const double v[3], double sign, double m[3]) {
+    const double va = rs_dot(v, S.a);
+    const double w[3] = {sign * (v[0] - va * S.a[0]), sign * (v[1] - va * S.a[1]), sign * (v[2] - va * S.a[2])};
+    const double l = sqrt(rs_dot(w, w));
+    if (l < 1e-9) return false;
+    for (int k = 0; k < 3; ++k) m[k] = S.cph * w[k] / l - S.sph * S.a[k];
+    return true;
+}
+ST_INLINE void rect_sweep_init(RectSweep& S, const double pc[3], double hw, double hh, const double* f,
+                               const double a[3], double phi) {
+    for (int k = 0; k < 3; ++k) {
+        S.pc[k] = pc[k];
+        S.X[k] = f[3 * k];
+        S.Y[k] = f[3 * k + 1];
+        S.a[k] = a[k];
+    }
+    const double N[3] = {f[2], f[5], f[8]};
+    S.hw = hw;
+    S.hh = hh;
+    S.eps = 1e-3 + 2e-6 * (fabs(pc[0]) + fabs(pc[1]) + fabs(pc[2]) + hw + hh);
+    const double ph = phi + 1e-6;
+    S.cph = cos(ph);
+    S.sph = sin(ph);
+    const double* A = S.a;
+    const double axw[3][3] = {{0.0, -A[2], A[1]}, {A[2], 0.0, -A[0]}, {-A[1], A[0], 0.0}};  // a x world axes
+    const double axX[3] = {A[1] * S.X[2] - A[2] * S.X[1], A[2] * S.X[0] - A[0] * S.X[2], A[0] * S.X[1] - A[1] * S.X[0]};
+    const double axY[3] = {A[1] * S.Y[2] - A[2] * S.Y[1], A[2] * S.Y[0] - A[0] * S.Y[2], A[0] * S.Y[1] - A[1] * S.Y[0]};
+    S.nm = 0;
+    for (int sgn = -1; sgn <= 1; sgn += 2) {
+        if (rs_plane_of(S, axX, (double)sgn, S.M[S.nm])) ++S.nm;
+        if (rs_plane_of(S, axY, (double)sgn, S.M[S.nm])) ++S.nm;
+        for (int k = 0; k < 3; ++k)
+            if (rs_plane_of(S, axw[k], (double)sgn, S.M[S.nm])) ++S.nm;
+    }
+    for (int k = 0; k < 3; ++k) S.M[S.nm][k] = -A[k];
+    ++S.nm;
+    const double na = rs_dot(N, A);
+    if (fabs(na) >= S.sph) {
+        const double s = na > 0.0 ? -1.0 : 1.0;
+        for (int k = 0; k < 3; ++k) S.M[S.nm][k] = s * N[k];
+        ++S.nm;
+    }
+}
+// true when a plane proves that no ray of the sweep reaches the box b = (lo, hi)
+ST_INLINE bool rect_sweep_misses(const RectSweep& S, const float* b) {
+    for (int j = 0; j < S.nm; ++j)
+        if (rs_max_r(S, S.M[j]) < rs_min_b(S.M[j], b)) return true;
+    const double d[3] = {0.5 * ((double)b[0] + b[3]) - S.pc[0], 0.5 * ((double)b[1] + b[4]) - S.pc[1],
+                         0.5 * ((double)b[2] + b[5]) - S.pc[2]};
+    double m[3];
+    return rs_plane_of(S, d, 1.0, m) && rs_max_r(S, m) < rs_min_b(m, b);
+}
+
 }  // namespace st

@@ -64,8 +64,6 @@ inline void set_trace_order(JobDev* J, int32_t position_sampling) {
     J->order_blocks = (uint32_t)((e1 - b1) / kHaltonBlock);
 }
 
-constexpr int kGridRes = 256;  // sun-plane candidate grid, the stratified-sampling cells
-constexpr int kGridCells = kGridRes * kGridRes;
 // a grid is used unless its lists grow long: a wave whose rays share a cell
 // tests its list in lockstep at ~40 vector instructions per entry, while a
 // BVH walk costs each lane ~40 per node over ~20 nodes of the 10k field, so
