@@ -403,4 +403,62 @@ ST_INLINE bool rect_sweep_misses(const RectSweep& S, const float* b) {
     return rs_plane_of(S, d, 1.0, m) && rs_max_r(S, m) < rs_min_b(m, b);
 }
 
+// Cone from the apex (origin) with unit axis a and half angle phi; pieces of
+// the cone are bounded by boxes over [0, first], then [t, t * ratio] up to the
+// far corner of the tested box (ratio fixed per launch from the scene size).
+struct Cone {
+    double a[3];
+    double tp, cp, sp;  // tan, cos, sin of phi
+    double ratio;
+};
+constexpr int kConePieces = 32;
+constexpr double kConeFirst = 0.25;
+
+ST_INLINE bool cone_meets_box(const Cone& C, const double lo[3], const double hi[3]) {
+    // the cone misses the box if it misses the box's bounding sphere: the
+    // sphere spans asin(r / |c|) around its centre direction
+    double r2 = 0.0, d2 = 0.0, ac = 0.0, far2 = 0.0, smin = 0.0, smax = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const double c = 0.5 * (lo[k] + hi[k]), h = 0.5 * (hi[k] - lo[k]);
+        r2 += h * h;
+        d2 += c * c;
+        ac += C.a[k] * c;
+        const double m = fmax(fabs(lo[k]), fabs(hi[k]));
+        far2 += m * m;
+        smin += fmin(C.a[k] * lo[k], C.a[k] * hi[k]);
+        smax += fmax(C.a[k] * lo[k], C.a[k] * hi[k]);
+    }
+    if (d2 > r2) {
+        const double d = sqrt(d2), sb = sqrt(r2) / d, cb = sqrt(fmax(0.0, 1.0 - sb * sb));
+        if (ac / d < C.cp * cb - C.sp * sb - 1e-9) return false;
+    }
+    // piece [t0, t1] holds the cone points whose coordinate along the axis is
+    // in [t0, t1]; the box's points have theirs in [smin, smax] (widened for
+    // rounding), so only the pieces overlapping that range are tested
+    const double slack = 1e-9 * (fabs(smin) + fabs(smax) + 1.0);
+    smin -= slack;
+    smax += slack;
+    if (smax < 0.0) return false;
+    const double T = sqrt(far2);
+    double t0 = 0.0, t1 = fmin(T, kConeFirst);
+    for (int i = 0; i < kConePieces; ++i) {
+        if (t0 > smax) break;
+        if (t1 >= smin) {
+            const double r0 = t0 * C.tp, r1 = t1 * C.tp;
+            // the piece's box overlaps the box along one axis (constant
+            // indices keep the cone in registers)
+            auto overlap = [&](double a, double blo, double bhi) __attribute__((always_inline)) {
+                const double c0 = a * t0, c1 = a * t1;
+                return !(fmax(c0 + r0, c1 + r1) < blo || fmin(c0 - r0, c1 - r1) > bhi);
+            };
+            if (overlap(C.a[0], lo[0], hi[0]) && overlap(C.a[1], lo[1], hi[1]) && overlap(C.a[2], lo[2], hi[2]))
+                return true;
+        }
+        if (t1 >= T) break;
+        t0 = t1;
+        t1 = i == kConePieces - 2 ? T : fmin(T, t1 * C.ratio);
+    }
+    return false;
+}
+
 }  // namespace st

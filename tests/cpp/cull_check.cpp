@@ -1,9 +1,12 @@
-// Brute-force checks of the two conservative tests that shorten candidate
-// lists (st_common.h): rect_sweep_misses (reflection lists of flat mirrors) and
-// grid_range / cell_meets (sun-plane grid cells of flat primitives). Both may
+// Brute-force checks of the conservative tests that build candidate lists
+// (st_common.h): cone_meets_box (the reflection-list BVH walk: a cone from a
+// mirror's box), rect_sweep_misses (reflection lists of flat mirrors) and
+// grid_range / cell_meets (sun-plane grid cells of flat primitives). Each may
 // only drop candidates that no ray can reach; this program samples rays that do
-// reach and counts every one the tests would have dropped.
+// reach and counts every one a test would have dropped.
 //
+//   cull_check cone <seed> <phi_scale>    cone_meets_box with the cone angle
+//                                          scaled by phi_scale
 //   cull_check sweep <seed> <phi_scale>   rect_sweep_misses with the cone angle
 //                                          scaled by phi_scale when building it
 //   cull_check grid <seed> <tan_scale>    grid_range / cell_meets with the cone's
@@ -152,6 +155,74 @@ int check_sweep(double phi_scale) {
     return 0;
 }
 
+// Reflection-list walk: rays leave a random box hb (a mirror's traversal box)
+// in directions within phi of a; a candidate box B is kept when the cone from
+// the origin meets B - hb (the walk's Minkowski difference, pieces bounded as
+// in a scene 2 km across). (i) Boxes around points such rays pass through must
+// be kept; (ii) boxes that are dropped are shot at with such rays, which must
+// all miss.
+int check_cone(double phi_scale) {
+    long reach_boxes = 0, reach_dropped = 0, dropped = 0, dropped_hit = 0, boxes = 0;
+    for (int cfg = 0; cfg < 150; ++cfg) {
+        const double c0[3] = {uni(-800, 800), uni(-800, 800), uni(-20, 40)};
+        const double h0[3] = {uni(0.2, 6.0), uni(0.2, 6.0), uni(0.2, 6.0)};
+        float hb[6];
+        box_around(c0, h0, hb);
+        const double phi = std::exp(uni(std::log(2e-3), std::log(0.3)));
+        double a[3];
+        random_unit(a);
+        Cone C;
+        for (int k = 0; k < 3; ++k) C.a[k] = a[k];
+        const double ph = phi * phi_scale;
+        C.tp = std::tan(ph);
+        C.cp = std::cos(ph);
+        C.sp = std::sin(ph);
+        const double reach = 2.0 * std::sqrt(3.0) * 2000.0 + 2.0 * kConeFirst;
+        C.ratio = std::pow(reach / kConeFirst, 1.0 / (kConePieces - 2));
+        auto meets = [&](const float b[6]) {
+            const double dlo[3] = {(double)b[0] - hb[3], (double)b[1] - hb[4], (double)b[2] - hb[5]};
+            const double dhi[3] = {(double)b[3] - hb[0], (double)b[4] - hb[1], (double)b[5] - hb[2]};
+            return cone_meets_box(C, dlo, dhi);
+        };
+        auto ray = [&](double o[3], double u[3]) {
+            for (int k = 0; k < 3; ++k) o[k] = uni((double)hb[k], (double)hb[3 + k]);
+            cone_direction(a, phi, u);
+        };
+        for (int i = 0; i < 1000; ++i) {  // (i) reachable boxes
+            double o[3], u[3];
+            ray(o, u);
+            const double t = std::exp(uni(std::log(1e-2), std::log(1500.0)));
+            const double h[3] = {uni(0.0, 2.0), uni(0.0, 2.0), uni(0.0, 2.0)};
+            double c[3];
+            for (int k = 0; k < 3; ++k) c[k] = o[k] + t * u[k] + uni(-h[k], h[k]);
+            float b[6];
+            box_around(c, h, b);
+            ++reach_boxes;
+            if (!meets(b)) ++reach_dropped;
+        }
+        for (int i = 0; i < 100; ++i) {  // (ii) random boxes around the apex
+            const double c[3] = {c0[0] + uni(-60, 60), c0[1] + uni(-60, 60), c0[2] + uni(-60, 60)};
+            const double h[3] = {uni(0.1, 6.0), uni(0.1, 6.0), uni(0.1, 6.0)};
+            float b[6];
+            box_around(c, h, b);
+            ++boxes;
+            if (meets(b)) continue;
+            ++dropped;
+            for (int r = 0; r < 1000; ++r) {
+                double o[3], u[3];
+                ray(o, u);
+                if (ray_hits_box(o, u, b)) {
+                    ++dropped_hit;
+                    break;
+                }
+            }
+        }
+    }
+    std::printf("cone reachable_boxes %ld reachable_dropped %ld boxes %ld dropped %ld dropped_but_hit %ld\n",
+                reach_boxes, reach_dropped, boxes, dropped, dropped_hit);
+    return 0;
+}
+
 // Sun-plane grid: a sun plane for a random sun, a random flat rectangle below
 // it; rays that hit the rectangle, traced back along directions within the sun
 // cone to their sun-plane origin, must start in a listed cell.
@@ -242,11 +313,12 @@ int check_grid(double tan_scale) {
 
 int main(int argc, char** argv) {
     if (argc != 4) {
-        std::fprintf(stderr, "usage: cull_check sweep|grid <seed> <scale>\n");
+        std::fprintf(stderr, "usage: cull_check cone|sweep|grid <seed> <scale>\n");
         return 2;
     }
     rng.seed(std::strtoull(argv[2], nullptr, 10));
     const double scale = std::atof(argv[3]);
+    if (!std::strcmp(argv[1], "cone")) return check_cone(scale);
     if (!std::strcmp(argv[1], "sweep")) return check_sweep(scale);
     if (!std::strcmp(argv[1], "grid")) return check_grid(scale);
     return 2;

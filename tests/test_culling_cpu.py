@@ -1,7 +1,10 @@
-"""The two conservative tests that shorten candidate lists, checked on the CPU
+"""The conservative tests that build candidate lists, checked on the CPU
 against brute-force ray sampling (tests/cpp/cull_check.cpp over st_common.h,
 the same functions the list-building kernels run):
 
+* cone_meets_box (the BVH walk of the reflection lists: the reflection cone
+  from a mirror's box against candidate boxes): no box around a point such a
+  ray passes through is dropped, and every dropped box is missed by sampled rays;
 * rect_sweep_misses (first- and second-reflection lists of flat mirrors): no
   box around a point that a ray from the mirror's rectangle, within the
   reflection cone, passes through is ever dropped, and every dropped box is
@@ -36,6 +39,18 @@ def run(exe, mode, seed, scale):
                           check=True).stdout.split()
     assert line[0] == mode
     return {line[i]: int(line[i + 1]) for i in range(1, len(line), 2)}
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_reflection_cone_walk_is_conservative(exe, seed):
+    r = run(exe, "cone", seed, 1.0)
+    assert r["reachable_boxes"] > 100_000 and r["reachable_dropped"] == 0, r
+    assert r["dropped"] > 10_000 and r["dropped_but_hit"] == 0, r
+
+
+def test_reflection_cone_check_catches_a_narrow_cone(exe):
+    r = run(exe, "cone", 1, 0.5)
+    assert r["reachable_dropped"] > 0 and r["dropped_but_hit"] > 0, r
 
 
 @pytest.mark.parametrize("seed", [1, 2])
