@@ -11,6 +11,8 @@
 //                                          scaled by phi_scale when building it
 //   cull_check grid <seed> <tan_scale>    grid_range / cell_meets with the cone's
 //                                          tangent scaled by tan_scale
+//   cull_check gridbox <seed> <tan_scale> the same for a curved primitive (box
+//                                          footprint), rays ending in its box
 // A scale of 1 is the shipped test (violations must be 0); a scale below 1
 // plants a too-narrow cone, which the sampling must catch.
 #include <cmath>
@@ -225,8 +227,10 @@ int check_cone(double phi_scale) {
 
 // Sun-plane grid: a sun plane for a random sun, a random flat rectangle below
 // it; rays that hit the rectangle, traced back along directions within the sun
-// cone to their sun-plane origin, must start in a listed cell.
-int check_grid(double tan_scale) {
+// cone to their sun-plane origin, must start in a listed cell. With flat =
+// false the primitive is listed as a curved one (the box footprint only) and
+// the sampled rays end anywhere in its traversal box.
+int check_grid(double tan_scale, bool flat) {
     long hits = 0, unlisted = 0, listed_cells = 0, range_cells = 0;
     for (int cfg = 0; cfg < 1000; ++cfg) {
         const double el = uni(5.0, 89.0) * M_PI / 180.0, az = uni(0.0, 2.0 * M_PI);
@@ -261,7 +265,7 @@ int check_grid(double tan_scale) {
         prim[1] = {(float)f[0], (float)f[3], (float)f[6], (float)hh};
         prim[2] = {(float)f[1], (float)f[4], (float)f[7], 0.f};
         prim[3] = {(float)f[2], (float)f[5], (float)f[8], 0.f};
-        const uint32_t meta = kKindFlat;
+        const uint32_t meta = flat ? kKindFlat : kKindCylinder;
         // traversal box: the rectangle's corners, padded as element boxes are
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
         for (int c = 0; c < 4; ++c)
@@ -288,7 +292,8 @@ int check_grid(double tan_scale) {
         for (int i = 0; i < 4000; ++i) {
             double q[3], u[3];
             const double x = uni(-hw, hw), y = uni(-hh, hh);
-            for (int k = 0; k < 3; ++k) q[k] = cen[k] + x * f[3 * k] + y * f[3 * k + 1];
+            for (int k = 0; k < 3; ++k)
+                q[k] = flat ? cen[k] + x * f[3 * k] + y * f[3 * k + 1] : uni((double)bounds[k], (double)bounds[3 + k]);
             cone_direction(w, alpha, u);
             const double t = (D - dot(q, s)) / -dot(u, s);  // back along u to the plane x . s = D
             double o[3], r[3];
@@ -304,8 +309,8 @@ int check_grid(double tan_scale) {
             if (!ok) ++unlisted;
         }
     }
-    std::printf("grid hits %ld unlisted %ld range_cells %ld listed_cells %ld\n", hits, unlisted, range_cells,
-                listed_cells);
+    std::printf("%s hits %ld unlisted %ld range_cells %ld listed_cells %ld\n", flat ? "grid" : "gridbox", hits,
+                unlisted, range_cells, listed_cells);
     return 0;
 }
 
@@ -313,13 +318,14 @@ int check_grid(double tan_scale) {
 
 int main(int argc, char** argv) {
     if (argc != 4) {
-        std::fprintf(stderr, "usage: cull_check cone|sweep|grid <seed> <scale>\n");
+        std::fprintf(stderr, "usage: cull_check cone|sweep|grid|gridbox <seed> <scale>\n");
         return 2;
     }
     rng.seed(std::strtoull(argv[2], nullptr, 10));
     const double scale = std::atof(argv[3]);
     if (!std::strcmp(argv[1], "cone")) return check_cone(scale);
     if (!std::strcmp(argv[1], "sweep")) return check_sweep(scale);
-    if (!std::strcmp(argv[1], "grid")) return check_grid(scale);
+    if (!std::strcmp(argv[1], "grid")) return check_grid(scale, true);
+    if (!std::strcmp(argv[1], "gridbox")) return check_grid(scale, false);
     return 2;
 }

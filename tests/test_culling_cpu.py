@@ -9,9 +9,9 @@ the same functions the list-building kernels run):
   box around a point that a ray from the mirror's rectangle, within the
   reflection cone, passes through is ever dropped, and every dropped box is
   missed by thousands of such rays;
-* grid_range / cell_meets (sun-plane grid cells of flat primitives): every ray
-  that hits the rectangle, traced back within the sun cone to the sun plane,
-  starts in a listed cell.
+* grid_range / cell_meets (sun-plane grid cells): every ray that hits a flat
+  primitive's rectangle, or enters a curved primitive's traversal box, traced
+  back within the sun cone to the sun plane, starts in a listed cell.
 
 Each check also runs with a planted mistake (a cone narrower than the real one
 when the test is built) and must then report violations, so the sampling has
@@ -74,4 +74,15 @@ def test_grid_cells_of_flat_primitives_are_conservative(exe, seed):
 
 def test_grid_check_catches_a_narrow_cone(exe):
     r = run(exe, "grid", 1, 0.3)
+    assert r["unlisted"] > 0, r
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_grid_cells_of_curved_primitives_are_conservative(exe, seed):
+    r = run(exe, "gridbox", seed, 1.0)
+    assert r["hits"] > 500_000 and r["unlisted"] == 0, r
+
+
+def test_grid_box_check_catches_a_narrow_cone(exe):
+    r = run(exe, "gridbox", 1, 0.3)
     assert r["unlisted"] > 0, r
