@@ -17,6 +17,11 @@
 //      overlaps and a short job does not wait behind long ones);
 //   4. retire jobs whose rays are all traced (statistics copied to the host,
 //      per-job scene copy back to the pool, outputs kept until released).
+// Rounds are pipelined: round k+1 is planned and queued on the stream while
+// round k runs, and round k is retired (its event waited for) only after that,
+// so the device does not idle between rounds. Each round uses one of two
+// pinned staging slots (job records, chunk prefix, finished jobs' statistics
+// and sun frames); a slot is reused two rounds later, after its round retired.
 // Ray draws are keyed by (seed, job_id, ray index) and the candidate lists
 // give the BVH's hits, so a job's result does not depend on the slices, on the
 // lists or on which other jobs shared its launches.
@@ -93,8 +98,12 @@ struct st_server_s {
     std::thread worker;
     hipStream_t stream = nullptr, fetch_stream = nullptr;
     DevBuf d_jobs, d_prefix, d_counter;
-    JobDev* h_jobs = nullptr;     // pinned
-    uint64_t* h_prefix = nullptr; // pinned
+    // pinned staging of the two round slots
+    JobDev* h_jobs[2] = {nullptr, nullptr};
+    uint64_t* h_prefix[2] = {nullptr, nullptr};
+    uint64_t* h_stats[2] = {nullptr, nullptr};   // ST_STAT_COUNT per job the round finishes
+    SunFrame* h_frames[2] = {nullptr, nullptr};  // one per job the round finishes
+    hipEvent_t round_done[2] = {nullptr, nullptr};
     // grid set-up scratch shared by the jobs set up in a round (worker stream)
     DevBuf d_grid_counts, d_grid_cursor, d_grid_stats;
     uint32_t* h_grid_stats = nullptr;  // pinned, 2 per active slot
@@ -313,6 +322,51 @@ void finish_job(st_server srv, ServerJob* j, int32_t status) {
 
 void worker_main(st_server srv) {
     (void)hipSetDevice(srv->ctx->device);
+    hipStream_t st = srv->stream;
+    // the round in flight: queued on the stream, not yet retired
+    struct Inflight {
+        bool valid = false;
+        int slot = 0;
+        std::vector<ServerJob*> jobs, finished;
+    } fl;
+    int next_slot = 0;
+    // caller holds srv->mu: every active job fails (the stream has faulted)
+    auto fail_all = [&](hipError_t e) {
+        srv->worker_error = hipGetErrorString(e);
+        for (ServerJob* j : srv->active) {
+            release_scene_copy(srv, j);
+            release_outputs(srv, j);
+            finish_job(srv, j, ST_ERR_DEVICE);
+        }
+        srv->active.clear();
+        fl.valid = false;
+        srv->cv_done.notify_all();
+    };
+    // waits for the round in flight and retires the jobs it finished
+    auto retire = [&]() -> bool {
+        if (!fl.valid) return true;
+        const hipError_t e = hipEventSynchronize(srv->round_done[fl.slot]);
+        std::lock_guard<std::mutex> lk(srv->mu);
+        fl.valid = false;
+        if (e != hipSuccess) {
+            fail_all(e);
+            return false;
+        }
+        for (size_t i = 0; i < fl.finished.size(); ++i) {
+            ServerJob* j = fl.finished[i];
+            memcpy(j->stats, srv->h_stats[fl.slot] + i * ST_STAT_COUNT, sizeof(j->stats));
+            const SunFrame& F = srv->h_frames[fl.slot][i];
+            j->power_per_ray = F.dni * F.area / (double)j->desc.num_rays;
+            release_scene_copy(srv, j);
+            finish_job(srv, j, ST_OK);
+            srv->active.erase(std::find(srv->active.begin(), srv->active.end(), j));
+        }
+        if (!fl.finished.empty()) srv->cv_done.notify_all();
+        return true;
+    };
+    auto in_flight = [&](const ServerJob* j) {
+        return fl.valid && std::find(fl.jobs.begin(), fl.jobs.end(), j) != fl.jobs.end();
+    };
     for (;;) {
         std::vector<ServerJob*> fresh;
         {
@@ -327,10 +381,11 @@ void worker_main(st_server srv) {
                        srv->stats.resident_bytes + h->footprint <= srv->cfg.hbm_budget_bytes;
             });
             if (srv->stop) break;
-            // drop cancelled active jobs (nothing of theirs is in flight between rounds)
+            // drop cancelled active jobs that the round in flight does not use
+            // (the others go once it has retired)
             for (size_t i = 0; i < srv->active.size();) {
                 ServerJob* j = srv->active[i];
-                if (j->cancel) {
+                if (j->cancel && !in_flight(j)) {
                     release_scene_copy(srv, j);
                     release_outputs(srv, j);
                     finish_job(srv, j, ST_ERR_CANCELLED);
@@ -373,9 +428,9 @@ void worker_main(st_server srv) {
             }
         }
         const double t0 = now_s();
-        hipStream_t st = srv->stream;
         // set-up of the admitted jobs: device work, one synchronisation for
-        // their grid totals, then grids, lists and job records
+        // their grid totals (which also drains the round in flight), then
+        // grids, lists and job records
         auto drop = [&](ServerJob* j, int rc) {
             std::lock_guard<std::mutex> lk(srv->mu);
             release_scene_copy(srv, j);
@@ -394,6 +449,7 @@ void worker_main(st_server srv) {
             hipError_t e = hipMemcpyAsync(srv->h_grid_stats, srv->d_grid_stats.ptr, sizeof(uint32_t) * 2 * staged.size(),
                                           hipMemcpyDeviceToHost, st);
             if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e == hipSuccess && !retire()) continue;
             for (size_t i = 0; i < staged.size(); ++i) {
                 const int rc = e == hipSuccess ? finish_setup(srv, staged[i], srv->h_grid_stats[2 * i],
                                                               srv->h_grid_stats[2 * i + 1])
@@ -408,9 +464,12 @@ void worker_main(st_server srv) {
         {
             std::lock_guard<std::mutex> lk(srv->mu);
             for (ServerJob* j : srv->active)
-                if (j->ready && j->dispatched < j->desc.num_rays) cand.push_back(j);
+                if (j->ready && !j->cancel && j->dispatched < j->desc.num_rays) cand.push_back(j);
         }
-        if (cand.empty()) continue;
+        if (cand.empty()) {
+            retire();  // nothing more to queue: wait for the round in flight
+            continue;
+        }
         std::stable_sort(cand.begin(), cand.end(), [](const ServerJob* x, const ServerJob* y) {
             return x->desc.num_rays - x->dispatched < y->desc.num_rays - y->dispatched;
         });
@@ -433,68 +492,66 @@ void worker_main(st_server srv) {
         }
         const LaunchShape sh = pick_shape(round_rays, srv->cfg.block_size, srv->cfg.grid_blocks, ST_KERNEL_AUTO, depth,
                                           nullptr, min_n);
-        srv->h_prefix[0] = 0;
+        // slot `slot` was last used two rounds ago, and that round has retired
+        const int slot = next_slot;
+        next_slot ^= 1;
+        JobDev* hj = srv->h_jobs[slot];
+        uint64_t* hp = srv->h_prefix[slot];
+        hp[0] = 0;
         for (size_t i = 0; i < round.size(); ++i) {
             JobDev J = round[i]->dev;
             J.ray_begin = round[i]->dispatched;
             J.ray_count = slice[i];
             J.record_base = J.ray_begin;
             set_trace_order(&J, round[i]->desc.sun.position_sampling);
-            srv->h_jobs[i] = J;
-            srv->h_prefix[i + 1] = srv->h_prefix[i] + (slice[i] + sh.chunk - 1) / sh.chunk;
+            hj[i] = J;
+            hp[i + 1] = hp[i] + (slice[i] + sh.chunk - 1) / sh.chunk;
         }
-        hipError_t e = hipMemcpyAsync(srv->d_jobs.ptr, srv->h_jobs, sizeof(JobDev) * round.size(),
-                                      hipMemcpyHostToDevice, st);
+        // the device copies are ordered after the round in flight on the stream
+        hipError_t e = hipMemcpyAsync(srv->d_jobs.ptr, hj, sizeof(JobDev) * round.size(), hipMemcpyHostToDevice, st);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(srv->d_prefix.ptr, srv->h_prefix, sizeof(uint64_t) * (round.size() + 1),
-                               hipMemcpyHostToDevice, st);
+            e = hipMemcpyAsync(srv->d_prefix.ptr, hp, sizeof(uint64_t) * (round.size() + 1), hipMemcpyHostToDevice, st);
         JobDev dummy;
         memset(&dummy, 0, sizeof(dummy));
         if (e == hipSuccess)
             e = launch_trace(st, (const JobDev*)srv->d_jobs.ptr, (const uint64_t*)srv->d_prefix.ptr,
-                             (int32_t)round.size(), srv->h_prefix[round.size()], (uint32_t*)srv->d_counter.ptr, sh,
-                             dummy);
+                             (int32_t)round.size(), hp[round.size()], (uint32_t*)srv->d_counter.ptr, sh, dummy);
         // statistics and sun frames of the jobs this round completes
         std::vector<ServerJob*> finished;
-        std::vector<SunFrame> frames;
         for (size_t i = 0; i < round.size(); ++i) {
             round[i]->dispatched += slice[i];
             if (round[i]->dispatched >= round[i]->desc.num_rays) finished.push_back(round[i]);
         }
-        frames.resize(finished.size());
         for (size_t i = 0; i < finished.size() && e == hipSuccess; ++i) {
             ServerJob* j = finished[i];
             if (j->work->copies > 1)
                 e = launch_flux_gather(st, j->dev.flux, j->work->copies, j->dev.flux_bins, 1, (int64_t*)j->out->flux.ptr);
             if (e == hipSuccess)
-                e = hipMemcpyAsync(j->stats, j->out->stats.ptr, sizeof(j->stats), hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipMemcpyAsync(&frames[i], j->work->sun.ptr, sizeof(SunFrame), hipMemcpyDeviceToHost, st);
+                e = hipMemcpyAsync(srv->h_stats[slot] + i * ST_STAT_COUNT, j->out->stats.ptr,
+                                   sizeof(uint64_t) * ST_STAT_COUNT, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(srv->h_frames[slot] + i, j->work->sun.ptr, sizeof(SunFrame), hipMemcpyDeviceToHost,
+                                   st);
         }
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        for (size_t i = 0; i < finished.size() && e == hipSuccess; ++i)
-            finished[i]->power_per_ray = frames[i].dni * frames[i].area / (double)finished[i]->desc.num_rays;
+        if (e == hipSuccess) e = hipEventRecord(srv->round_done[slot], st);
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(st);
+            std::lock_guard<std::mutex> lk(srv->mu);
+            fail_all(e);
+            continue;
+        }
+        // the new round is queued: retire the previous one while it runs
+        const bool ok = retire();
+        fl.valid = ok;
+        fl.slot = slot;
+        fl.jobs = std::move(round);
+        fl.finished = std::move(finished);
         std::lock_guard<std::mutex> lk(srv->mu);
         srv->stats.launches++;
         srv->stats.rays_traced += round_rays;
         srv->stats.busy_s += now_s() - t0;
-        if (e != hipSuccess) {
-            srv->worker_error = hipGetErrorString(e);
-            for (ServerJob* j : srv->active) {
-                release_scene_copy(srv, j);
-                release_outputs(srv, j);
-                finish_job(srv, j, ST_ERR_DEVICE);
-            }
-            srv->active.clear();
-            srv->cv_done.notify_all();
-            continue;
-        }
-        for (ServerJob* j : finished) {
-            release_scene_copy(srv, j);
-            finish_job(srv, j, ST_OK);
-            srv->active.erase(std::find(srv->active.begin(), srv->active.end(), j));
-        }
-        if (!finished.empty()) srv->cv_done.notify_all();
     }
+    retire();
 }
 
 void fill_result(const ServerJob* j, st_job_result* r) {
@@ -539,8 +596,16 @@ extern "C" int st_server_create(st_context ctx, const st_server_config* cfg, st_
     if ((rc = srv->d_prefix.ensure(sizeof(uint64_t) * (J + 1), "server prefix")) != ST_OK) return rc;
     if ((rc = srv->d_counter.ensure(sizeof(uint32_t) * 2, "server counter")) != ST_OK) return rc;
     HIP_TRY(hipMemset(srv->d_counter.ptr, 0, sizeof(uint32_t) * 2), "counter clear");
-    HIP_TRY(hipHostMalloc((void**)&srv->h_jobs, sizeof(JobDev) * J, hipHostMallocDefault), "pinned jobs");
-    HIP_TRY(hipHostMalloc((void**)&srv->h_prefix, sizeof(uint64_t) * (J + 1), hipHostMallocDefault), "pinned prefix");
+    for (int k = 0; k < 2; ++k) {
+        HIP_TRY(hipHostMalloc((void**)&srv->h_jobs[k], sizeof(JobDev) * J, hipHostMallocDefault), "pinned jobs");
+        HIP_TRY(hipHostMalloc((void**)&srv->h_prefix[k], sizeof(uint64_t) * (J + 1), hipHostMallocDefault),
+                "pinned prefix");
+        HIP_TRY(hipHostMalloc((void**)&srv->h_stats[k], sizeof(uint64_t) * ST_STAT_COUNT * J, hipHostMallocDefault),
+                "pinned job statistics");
+        HIP_TRY(hipHostMalloc((void**)&srv->h_frames[k], sizeof(SunFrame) * J, hipHostMallocDefault),
+                "pinned sun frames");
+        HIP_TRY(hipEventCreateWithFlags(&srv->round_done[k], hipEventDisableTiming), "round event");
+    }
     HIP_TRY(hipHostMalloc((void**)&srv->h_grid_stats, sizeof(uint32_t) * 2 * J, hipHostMallocDefault), "pinned grid stats");
     if ((rc = srv->d_grid_counts.ensure(sizeof(uint32_t) * kGridCells, "server grid counts")) != ST_OK) return rc;
     if ((rc = srv->d_grid_cursor.ensure(sizeof(uint32_t) * kGridCells, "server grid cursor")) != ST_OK) return rc;
@@ -563,8 +628,13 @@ extern "C" int st_server_destroy(st_server srv) {
     (void)hipSetDevice(srv->ctx->device);
     (void)hipStreamSynchronize(srv->stream);
     srv->jobs.clear();
-    if (srv->h_jobs) (void)hipHostFree(srv->h_jobs);
-    if (srv->h_prefix) (void)hipHostFree(srv->h_prefix);
+    for (int k = 0; k < 2; ++k) {
+        if (srv->h_jobs[k]) (void)hipHostFree(srv->h_jobs[k]);
+        if (srv->h_prefix[k]) (void)hipHostFree(srv->h_prefix[k]);
+        if (srv->h_stats[k]) (void)hipHostFree(srv->h_stats[k]);
+        if (srv->h_frames[k]) (void)hipHostFree(srv->h_frames[k]);
+        if (srv->round_done[k]) (void)hipEventDestroy(srv->round_done[k]);
+    }
     if (srv->h_grid_stats) (void)hipHostFree(srv->h_grid_stats);
     (void)hipStreamDestroy(srv->stream);
     (void)hipStreamDestroy(srv->fetch_stream);
