@@ -19,7 +19,9 @@
 //      per-job scene copy back to the pool, outputs kept until released).
 // Rounds are pipelined: round k+1 is planned and queued on the stream while
 // round k runs, and round k is retired (its event waited for) only after that,
-// so the device does not idle between rounds. Each round uses one of two
+// so the device does not idle between rounds. Set-up is queued the same way: a
+// batch of admitted jobs has its set-up work and the read-back of its grid
+// totals queued, and joins the rounds once that read-back has arrived. Each round uses one of two
 // pinned staging slots (job records, chunk prefix, finished jobs' statistics
 // and sun frames); a slot is reused two rounds later, after its round retired.
 // Ray draws are keyed by (seed, job_id, ray index) and the candidate lists
@@ -330,6 +332,12 @@ void worker_main(st_server srv) {
         std::vector<ServerJob*> jobs, finished;
     } fl;
     int next_slot = 0;
+    // admitted jobs waiting for set-up, and the batch whose grid totals are on
+    // their way back to h_grid_stats (one batch at a time)
+    std::vector<ServerJob*> unset, setting;
+    hipEvent_t setup_ev = nullptr;
+    bool setup_pending = false;
+    if (hipEventCreateWithFlags(&setup_ev, hipEventDisableTiming) != hipSuccess) setup_ev = nullptr;
     // caller holds srv->mu: every active job fails (the stream has faulted)
     auto fail_all = [&](hipError_t e) {
         srv->worker_error = hipGetErrorString(e);
@@ -340,6 +348,9 @@ void worker_main(st_server srv) {
         }
         srv->active.clear();
         fl.valid = false;
+        unset.clear();
+        setting.clear();
+        setup_pending = false;
         srv->cv_done.notify_all();
     };
     // waits for the round in flight and retires the jobs it finished
@@ -365,10 +376,10 @@ void worker_main(st_server srv) {
         return true;
     };
     auto in_flight = [&](const ServerJob* j) {
-        return fl.valid && std::find(fl.jobs.begin(), fl.jobs.end(), j) != fl.jobs.end();
+        return (fl.valid && std::find(fl.jobs.begin(), fl.jobs.end(), j) != fl.jobs.end()) ||
+               std::find(setting.begin(), setting.end(), j) != setting.end();
     };
     for (;;) {
-        std::vector<ServerJob*> fresh;
         {
             std::unique_lock<std::mutex> lk(srv->mu);
             // sleep unless there is active work or the head of the queue can be decided
@@ -390,6 +401,7 @@ void worker_main(st_server srv) {
                     release_outputs(srv, j);
                     finish_job(srv, j, ST_ERR_CANCELLED);
                     srv->active.erase(srv->active.begin() + i);
+                    unset.erase(std::remove(unset.begin(), unset.end(), j), unset.end());
                     srv->cv_done.notify_all();
                 } else {
                     ++i;
@@ -424,13 +436,10 @@ void worker_main(st_server srv) {
                 j->admitted = true;
                 j->start_s = now_s();
                 srv->active.push_back(j);
-                fresh.push_back(j);
+                unset.push_back(j);
             }
         }
         const double t0 = now_s();
-        // set-up of the admitted jobs: device work, one synchronisation for
-        // their grid totals (which also drains the round in flight), then
-        // grids, lists and job records
         auto drop = [&](ServerJob* j, int rc) {
             std::lock_guard<std::mutex> lk(srv->mu);
             release_scene_copy(srv, j);
@@ -439,23 +448,42 @@ void worker_main(st_server srv) {
             srv->active.erase(std::find(srv->active.begin(), srv->active.end(), j));
             srv->cv_done.notify_all();
         };
-        std::vector<ServerJob*> staged;
-        for (size_t i = 0; i < fresh.size(); ++i) {
-            const int rc = setup_job(srv, fresh[i], (int)staged.size());
-            if (rc != ST_OK) drop(fresh[i], rc);
-            else staged.push_back(fresh[i]);
-        }
-        if (!staged.empty()) {
-            hipError_t e = hipMemcpyAsync(srv->h_grid_stats, srv->d_grid_stats.ptr, sizeof(uint32_t) * 2 * staged.size(),
-                                          hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
-            if (e == hipSuccess && !retire()) continue;
-            for (size_t i = 0; i < staged.size(); ++i) {
-                const int rc = e == hipSuccess ? finish_setup(srv, staged[i], srv->h_grid_stats[2 * i],
+        // second half of a batch's set-up, once its grid totals are on the
+        // host: grids, lists and job records (queued), then the jobs join rounds
+        auto finish_batch = [&]() {
+            const hipError_t e = hipEventSynchronize(setup_ev);
+            for (size_t i = 0; i < setting.size(); ++i) {
+                const int rc = e == hipSuccess ? finish_setup(srv, setting[i], srv->h_grid_stats[2 * i],
                                                               srv->h_grid_stats[2 * i + 1])
                                                : hip_fail(e, "job set-up");
-                if (rc != ST_OK) drop(staged[i], rc);
-                else staged[i]->ready = true;
+                if (rc != ST_OK) drop(setting[i], rc);
+                else setting[i]->ready = true;
+            }
+            setting.clear();
+            setup_pending = false;
+        };
+        if (setup_pending && hipEventQuery(setup_ev) != hipErrorNotReady) finish_batch();
+        // first half of the next batch: set-up work and the read-back of its
+        // grid totals, queued behind the rounds already on the stream
+        if (!setup_pending && !unset.empty()) {
+            for (ServerJob* j : unset) {
+                const int rc = setup_job(srv, j, (int)setting.size());
+                if (rc != ST_OK) drop(j, rc);
+                else setting.push_back(j);
+            }
+            unset.clear();
+            if (!setting.empty()) {
+                hipError_t e = setup_ev ? hipSuccess : hipErrorInvalidResourceHandle;
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(srv->h_grid_stats, srv->d_grid_stats.ptr, sizeof(uint32_t) * 2 * setting.size(),
+                                       hipMemcpyDeviceToHost, st);
+                if (e == hipSuccess) e = hipEventRecord(setup_ev, st);
+                if (e == hipSuccess) {
+                    setup_pending = true;
+                } else {
+                    for (ServerJob* j : setting) drop(j, hip_fail(e, "job set-up"));
+                    setting.clear();
+                }
             }
         }
         // one launch over slices of the active jobs: shortest remaining first,
@@ -467,7 +495,9 @@ void worker_main(st_server srv) {
                 if (j->ready && !j->cancel && j->dispatched < j->desc.num_rays) cand.push_back(j);
         }
         if (cand.empty()) {
-            retire();  // nothing more to queue: wait for the round in flight
+            // nothing to queue: wait for a set-up batch, else for the round in flight
+            if (setup_pending) finish_batch();
+            else retire();
             continue;
         }
         std::stable_sort(cand.begin(), cand.end(), [](const ServerJob* x, const ServerJob* y) {
@@ -552,6 +582,8 @@ void worker_main(st_server srv) {
         srv->stats.busy_s += now_s() - t0;
     }
     retire();
+    if (setup_pending) (void)hipEventSynchronize(setup_ev);
+    if (setup_ev) (void)hipEventDestroy(setup_ev);
 }
 
 void fill_result(const ServerJob* j, st_job_result* r) {
