@@ -214,9 +214,10 @@ ST_INLINE U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
 ST_INLINE float u01(uint32_t x) { return ((float)(x >> 9) + 0.5f) * (1.0f / 8388608.0f); }
 
 // ---------------------------------------------------------------------------
-// The two conservative tests that shorten candidate lists. They run on the
-// device (k_grid_count / k_grid_fill, k_cone_lists) and are checked against
-// brute-force ray sampling on the host.
+// The conservative tests that build candidate lists: the sun-plane grid's
+// footprints, the reflection cone against boxes and the flat-mirror sweep. They
+// run on the device (k_grid_count / k_grid_fill, k_cone_lists) and are checked
+// against brute-force ray sampling on the host.
 
 constexpr int kGridRes = 256;  // sun-plane candidate grid, the stratified-sampling cells
 constexpr int kGridCells = kGridRes * kGridRes;
@@ -249,8 +250,8 @@ ST_INLINE bool cell_meets(const Footprint& f, int ix, int iy) {
     return true;
 }
 ST_INLINE bool grid_range(const float* __restrict__ bounds, const float4* __restrict__ prims,
-                                           const uint32_t* __restrict__ meta, int slot, const SunFrame& F,
-                                           double tan_max, Footprint& fp) {
+                          const uint32_t* __restrict__ meta, int slot, const SunFrame& F, double tan_max,
+                          Footprint& fp) {
     int* r = fp.r;
     fp.normals = 0;
     const double e1[3] = {F.v1[0] - F.v0[0], F.v1[1] - F.v0[1], F.v1[2] - F.v0[2]};
