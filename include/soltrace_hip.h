@@ -97,6 +97,10 @@ enum st_position_sampling {
 #define ST_KERNEL_MEGA 1       /* one lane per ray, all bounces in registers        */
 #define ST_KERNEL_QUEUED 2     /* sun rays in full waves; continuing rays parked in  
                                   LDS and traced in full waves of 64                */
+#define ST_KERNEL_QUEUED_GROUPED 3  /* the queued form tracing segments with a
+                                  candidate list and segments that walk the BVH in
+                                  separate steps (AUTO picks it where reflected
+                                  rays mix the two)                                 */
 
 /* statistics counters (uint64 each) */
 enum st_stat {
@@ -249,7 +253,9 @@ int st_scene_list_info(st_scene scene, uint64_t* grid_items, uint64_t* reflectio
 typedef struct st_tune_result {
     int32_t block, grid, stack, waves;
     int32_t trials;
-    int32_t queued;               /* 1: ST_KERNEL_QUEUED form chosen                */
+    int32_t queued;               /* 1: ST_KERNEL_QUEUED form chosen; 2: the same
+                                     with list and BVH segments traced in separate
+                                     steps                                        */
     double best_ms, default_ms;   /* kernel time of the sample, tuned and untuned */
 } st_tune_result;
 int st_scene_autotune(st_scene scene, uint64_t sample_rays, int32_t max_depth, st_tune_result* out);

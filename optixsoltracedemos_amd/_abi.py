@@ -36,7 +36,7 @@ TRACE_ZERO_OUTPUTS = 0x4
 TRACE_BVH_ONLY = 0x8
 TRACE_COUNT_WORK = 0x10
 
-KERNEL_AUTO, KERNEL_MEGA, KERNEL_QUEUED = 0, 1, 2
+KERNEL_AUTO, KERNEL_MEGA, KERNEL_QUEUED, KERNEL_QUEUED_GROUPED = 0, 1, 2, 3
 
 STAT_NAMES = [
     "rays", "segments", "mirror_hits", "receiver_hits", "direct_sun", "absorbed",

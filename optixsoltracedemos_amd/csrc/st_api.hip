@@ -410,6 +410,26 @@ int scene_bounce_lists(st_scene s) {
                                               (const int32_t*)s->d_pair_owner.ptr, c, scratch);
                             }));
     s->pair_valid = s->pair_items > 0;
+    // share of the mirrors in first-reflection lists whose pair list is missing
+    // (too long or too wide: those second reflections walk the BVH)
+    std::vector<int32_t> items((size_t)tasks);
+    HIP_TRY(hipMemcpyAsync(items.data(), s->d_bounce_items.ptr, sizeof(int32_t) * items.size(), hipMemcpyDeviceToHost,
+                           st),
+            "list items download");
+    HIP_TRY(hipStreamSynchronize(st), "list sync");
+    uint64_t entries = 0, missing = 0;
+    for (int32_t t = 0; t < tasks; ++t) {
+        if (s->h_pair_owner[t] < 0) continue;
+        const int32_t e = s->topo.slot_elem[items[t]];
+        if (s->host_elems[e].role != ST_ROLE_MIRROR) continue;
+        ++entries;
+        missing += s->h_pair_range[t].x > s->h_pair_range[t].y;
+    }
+    s->pair_missing = entries ? (double)missing / (double)entries : 1.0;
+    if (getenv("ST_LIST_LOG"))
+        fprintf(stderr, "lists: first %llu items, pairs %llu items, %llu of %llu entries without a pair list\n",
+                (unsigned long long)s->bounce_items, (unsigned long long)s->pair_items, (unsigned long long)missing,
+                (unsigned long long)entries);
     return ST_OK;
 }
 
@@ -745,7 +765,8 @@ int validate_trace(st_scene s, const st_trace_desc* d, const st_outputs* out) {
     if (d->block_size != 0 && d->block_size != 64 && d->block_size != 128 && d->block_size != 256)
         return fail(ST_ERR_INVALID, "block_size must be 0, 64, 128 or 256");
     if (d->grid_blocks < 0) return fail(ST_ERR_INVALID, "negative grid_blocks");
-    if (d->kernel < 0 || d->kernel > ST_KERNEL_QUEUED) return fail(ST_ERR_INVALID, "unknown kernel %d", d->kernel);
+    if (d->kernel < 0 || d->kernel > ST_KERNEL_QUEUED_GROUPED)
+        return fail(ST_ERR_INVALID, "unknown kernel %d", d->kernel);
     if ((d->flags & ST_TRACE_RECORDS) && (!out || !out->records))
         return fail(ST_ERR_INVALID, "ST_TRACE_RECORDS needs outputs->records");
     if (d->flux_nx > 0 && d->flux_ny > 0 && (!out || !out->flux))
@@ -769,8 +790,13 @@ extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* ou
         if (out->element_hits) HIP_TRY(launch_fill_u64(st, out->element_hits, (uint64_t)s->n, 0), "zero hits");
         if (out->stats) HIP_TRY(launch_fill_u64(st, out->stats, ST_STAT_COUNT, 0), "zero stats");
     }
-    const LaunchShape sh = pick_shape(J.ray_count, d->block_size, d->grid_blocks, d->kernel, s->topo.depth,
-                                      s->tuned_valid ? &s->tuned : nullptr, s->n);
+    const bool grouped = d->kernel == ST_KERNEL_QUEUED_GROUPED;
+    LaunchShape sh = pick_shape(J.ray_count, d->block_size, d->grid_blocks, grouped ? ST_KERNEL_QUEUED : d->kernel,
+                                s->topo.depth, s->tuned_valid ? &s->tuned : nullptr, s->n);
+    // grouped when the autotuner chose it for this scene, or when reflected rays
+    // mix listed segments with BVH ones (no or too few second-reflection lists)
+    use_grouped_bvh(sh, grouped || sh.group_bvh ||
+                            (J.bounce_range && (!J.pair_range || s->pair_missing > kPairMissingGrouped)));
     const uint64_t chunks = (J.ray_count + sh.chunk - 1) / sh.chunk;
     if (chunks >= (1ull << 32)) return fail(ST_ERR_INVALID, "too many rays for one call");
     if (launch_injected()) return fail(ST_ERR_INJECTED, "injected launch failure (trace)");
@@ -887,6 +913,23 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
                         best = sh;
                     }
                 }
+    // the queued winner with list and BVH segments traced in separate steps
+    // (pays where many segments have no list, e.g. rays after two mirrors at
+    // low sun)
+    if (rc == ST_OK && best.queued && !best.lds_hits && grouped_variant_exists(best.block, best.stack, best.waves)) {
+        LaunchShape sh = best;
+        sh.group_bvh = true;
+        double ms = 0;
+        rc = time_shape(sh, &ms);
+        ++trials;
+        if (log_trials)
+            fprintf(stderr, "st_scene_autotune: block %d stack %d waves %d queued 2 grid %d chunk %d: %.3f ms\n",
+                    sh.block, sh.stack, sh.waves, sh.grid, sh.chunk, ms);
+        if (rc == ST_OK && ms < 0.98 * best_ms) {
+            best_ms = ms;
+            best = sh;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     // clear the copies (their sums go to the tuning buffer, discarded)
@@ -899,7 +942,7 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
     out->grid = best.grid;
     out->stack = best.stack;
     out->waves = best.waves;
-    out->queued = best.queued ? 1 : 0;
+    out->queued = best.queued ? (best.group_bvh ? 2 : 1) : 0;
     out->trials = trials;
     out->best_ms = best_ms;
     out->default_ms = def_ms;
@@ -1138,6 +1181,7 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
         for (int k = 0; k < K; ++k) total_rays += d->budgets[k];
         b.shape = pick_shape(total_rays, d->block_size, d->grid_blocks, ST_KERNEL_AUTO, s->topo.depth,
                              s->tuned_valid ? &s->tuned : nullptr, s->n);
+        use_grouped_bvh(b.shape, b.lists);  // per-step first-reflection lists, no pair lists
         std::vector<JobDev> jobs(K);
         std::vector<uint64_t> prefix(K + 1, 0);
         for (int k = 0; k < K; ++k) {

@@ -146,6 +146,7 @@ struct st_scene_s {
     std::vector<uint2> h_bounce_range, h_pair_range;
     std::vector<int32_t> h_pair_owner;
     bool bounce_valid = false, pair_valid = false;
+    double pair_missing = 1.0;  // share of first-reflection list entries without a second-reflection list
     bool lists_stale = true;         // scene state changed since the lists were built
     uint64_t rays_since_change = 0;  // rays traced on the current state without lists
     uint64_t list_min_rays = st::kListMinRays;  // traced rays that repay a rebuild

@@ -84,11 +84,22 @@ struct LaunchShape {
     bool queued = false; // continuing rays parked in LDS, traced in full waves
     bool lds_hits = false;  // hit counts of elements below kLdsHitElements kept per wave in
                             // LDS (launches with a scene that small; queued, block 256, stack 8 or 16)
+    bool group_bvh = false; // queued form tracing list and BVH segments in separate steps
+                            // (use_grouped_bvh; compiled for stack 16, 5 waves/SIMD)
 };
 // Scenes this small count element hits in LDS: their few counters would
 // otherwise take a device-scope atomic from nearly every wave of the launch.
 constexpr int kLdsHitElements = 128;
 bool trace_variant_exists(int block, int stack, int waves, bool queued);
+bool grouped_variant_exists(int block, int stack, int waves);
+// A launch of jobs whose reflected rays partly have candidate lists and partly
+// walk the BVH (first-reflection lists without second-reflection lists, or a
+// noticeable share of second-reflection lists missing) traces the two kinds
+// of segment in separate steps when its shape allows it.
+constexpr double kPairMissingGrouped = 0.01;
+inline void use_grouped_bvh(LaunchShape& sh, bool mixed) {
+    sh.group_bvh = mixed && sh.queued && !sh.lds_hits && grouped_variant_exists(sh.block, sh.stack, sh.waves);
+}
 // waves of this trace variant the device holds at once (0 if unknown)
 int trace_resident_waves(const LaunchShape& shape);
 constexpr bool kDefaultQueued = true;  // form used when the caller and the tuner leave it open

@@ -871,7 +871,7 @@ __device__ __forceinline__ uint32_t queue_tag(uint32_t offset, int depth, int ke
 
 // LHITS > 0: element hit counts of scenes with at most LHITS elements are
 // kept per wave in LDS and added to the output once per chunk.
-template <int BLOCK, int STACK, int MINW, bool QUEUED, int LHITS>
+template <int BLOCK, int STACK, int MINW, int QUEUED, int LHITS>
 __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict__ jobs,
                                                        const uint64_t* __restrict__ chunk_prefix, int32_t n_jobs,
                                                        uint64_t total_chunks, uint32_t* __restrict__ work_counter,
@@ -928,23 +928,42 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                 }
                 qn -= take;
             };
-            // the next segment of every held ray
+            // sends the log when the next step could overfill it
             auto flush_if_full = [&]() {
                 if (len > kLogCap - kLogHeadroom) {
                     log_send(wl, J, len);
                     len = 0;
                 }
             };
+            // the next segment of every held ray. QUEUED == 2: a segment
+            // without a candidate list walks the BVH (tens of nodes) where a
+            // list walk takes a few tests, so a step traces either list
+            // segments or BVH segments; the BVH ones wait (keeping their rays)
+            // until half the wave, or every held ray, needs the BVH.
             auto step = [&]() {
                 flush_if_full();
                 uint32_t ph = kNoEntry, pf = kNoEntry;
-                if (alive) {
-                    const uint64_t ray = ray_of_slot(J, first + off);
-                    int last = depth;
-                    float tmin = kTminBounce;
-                    alive = trace_segment<BLOCK, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
-                                                       next_list(J, depth, key), key, ph, pf);
-                    if (!alive) finish_ray(J, ray, last);
+                if constexpr (QUEUED == 2) {
+                    const SegList L = alive ? next_list(J, depth, key) : no_list();
+                    const bool bvh = alive && L.lb > L.le;
+                    const unsigned long long bm = __ballot(bvh);
+                    if (alive && bvh == (__popcll(bm) >= 32 || bm == __ballot(alive))) {
+                        const uint64_t ray = ray_of_slot(J, first + off);
+                        int last = depth;
+                        float tmin = kTminBounce;
+                        alive = trace_segment<BLOCK, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last, L, key,
+                                                           ph, pf);
+                        if (!alive) finish_ray(J, ray, last);
+                    }
+                } else {
+                    if (alive) {
+                        const uint64_t ray = ray_of_slot(J, first + off);
+                        int last = depth;
+                        float tmin = kTminBounce;
+                        alive = trace_segment<BLOCK, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
+                                                           next_list(J, depth, key), key, ph, pf);
+                        if (!alive) finish_ray(J, ray, last);
+                    }
                 }
                 log_pending(wl, len, J, ph, pf);
             };
@@ -1702,7 +1721,7 @@ hipError_t launch_sun_plane(hipStream_t s, const double* ref_aabb, int32_t n, in
     return hipGetLastError();
 }
 
-template <int BLOCK, int STACK, int MINW, bool QUEUED, int LHITS>
+template <int BLOCK, int STACK, int MINW, int QUEUED, int LHITS>
 static void launch_variant(hipStream_t s, int grid, const JobDev* jobs, const uint64_t* chunk_prefix, int32_t n_jobs,
                            uint64_t total_chunks, uint32_t* work_counter, int chunk, int static_chunks,
                            const JobDev& single) {
@@ -1710,7 +1729,7 @@ static void launch_variant(hipStream_t s, int grid, const JobDev* jobs, const ui
                        n_jobs, total_chunks, work_counter, chunk, static_chunks, single);
 }
 
-template <int BLOCK, int STACK, int MINW, bool QUEUED, int LHITS>
+template <int BLOCK, int STACK, int MINW, int QUEUED, int LHITS>
 static int resident_blocks_per_cu() {
     static std::atomic<int> cached{-1};  // per variant; the library runs on one kind of device
     int nb = cached.load();
@@ -1728,6 +1747,9 @@ static int resident_blocks_per_cu() {
 // Compiled shapes. The queued form keeps 3.5 KB of parked rays per wave in
 // LDS on top of the stack, which caps it at 5 waves per SIMD (stack 16) or 3
 // (stack 32); its register budgets are chosen to match.
+bool grouped_variant_exists(int block, int stack, int waves) {
+    return stack == 16 && waves == 5 && (block == 128 || block == 256);
+}
 bool trace_variant_exists(int block, int stack, int waves, bool queued) {
     if (queued) {
         if (stack == 16) return (block == 128 || block == 256) && (waves == 4 || waves == 5);
@@ -1739,30 +1761,33 @@ bool trace_variant_exists(int block, int stack, int waves, bool queued) {
     return false;
 }
 
+// Q: 0 whole rays, 1 queued, 2 queued with BVH segments grouped
 #define ST_VARIANTS                    \
-    ST_VARIANT(64, 32, 4, false, 0)    \
-    ST_VARIANT(64, 32, 5, false, 0)    \
-    ST_VARIANT(128, 32, 4, false, 0)   \
-    ST_VARIANT(128, 32, 5, false, 0)   \
-    ST_VARIANT(256, 32, 4, false, 0)   \
-    ST_VARIANT(128, 16, 4, false, 0)   \
-    ST_VARIANT(128, 16, 6, false, 0)   \
-    ST_VARIANT(128, 16, 8, false, 0)   \
-    ST_VARIANT(256, 16, 4, false, 0)   \
-    ST_VARIANT(256, 16, 6, false, 0)   \
-    ST_VARIANT(256, 16, 8, false, 0)   \
-    ST_VARIANT(128, 16, 4, true, 0)    \
-    ST_VARIANT(128, 16, 5, true, 0)    \
-    ST_VARIANT(256, 16, 4, true, 0)    \
-    ST_VARIANT(256, 16, 5, true, 0)    \
-    ST_VARIANT(64, 32, 3, true, 0)     \
-    ST_VARIANT(128, 32, 3, true, 0)    \
-    ST_VARIANT(256, 8, 5, true, kLdsHitElements) \
-    ST_VARIANT(256, 16, 5, true, kLdsHitElements)
+    ST_VARIANT(64, 32, 4, 0, 0)        \
+    ST_VARIANT(64, 32, 5, 0, 0)        \
+    ST_VARIANT(128, 32, 4, 0, 0)       \
+    ST_VARIANT(128, 32, 5, 0, 0)       \
+    ST_VARIANT(256, 32, 4, 0, 0)       \
+    ST_VARIANT(128, 16, 4, 0, 0)       \
+    ST_VARIANT(128, 16, 6, 0, 0)       \
+    ST_VARIANT(128, 16, 8, 0, 0)       \
+    ST_VARIANT(256, 16, 4, 0, 0)       \
+    ST_VARIANT(256, 16, 6, 0, 0)       \
+    ST_VARIANT(256, 16, 8, 0, 0)       \
+    ST_VARIANT(128, 16, 4, 1, 0)       \
+    ST_VARIANT(128, 16, 5, 1, 0)       \
+    ST_VARIANT(256, 16, 4, 1, 0)       \
+    ST_VARIANT(256, 16, 5, 1, 0)       \
+    ST_VARIANT(64, 32, 3, 1, 0)        \
+    ST_VARIANT(128, 32, 3, 1, 0)       \
+    ST_VARIANT(128, 16, 5, 2, 0)       \
+    ST_VARIANT(256, 16, 5, 2, 0)       \
+    ST_VARIANT(256, 8, 5, 1, kLdsHitElements) \
+    ST_VARIANT(256, 16, 5, 1, kLdsHitElements)
 
 int trace_resident_waves(const LaunchShape& shape) {
     const int st = shape.stack, w = shape.waves, b = shape.block, h = shape.lds_hits ? kLdsHitElements : 0;
-    const bool r = shape.queued;
+    const int r = shape.queued ? (shape.group_bvh ? 2 : 1) : 0;
 #define ST_VARIANT(B, S, W, Q, H)                                    \
     if (b == B && st == S && w == W && r == Q && h == H)             \
         return resident_blocks_per_cu<B, S, W, Q, H>() * device_cu_count() * (B / 64);
@@ -1781,7 +1806,7 @@ hipError_t launch_trace(hipStream_t s, const JobDev* jobs, const uint64_t* chunk
     if ((uint64_t)grid > max_blocks) grid = (int)(max_blocks > 0 ? max_blocks : 1);
     const int st = shape.stack, w = shape.waves, b = shape.block, c = shape.chunk;
     const int h = shape.lds_hits ? kLdsHitElements : 0;
-    const bool r = shape.queued;
+    const int r = shape.queued ? (shape.group_bvh ? 2 : 1) : 0;
     if (r && (c <= 0 || c > (int)kMaxChunkRays)) return hipErrorInvalidValue;  // parked rays keep an 11-bit offset
     // one chunk per wave without the shared counter, when every wave fits on
     // the device at once (a wave not yet resident would hold its chunk back

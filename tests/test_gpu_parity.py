@@ -324,7 +324,7 @@ def test_candidate_lists_equal_bvh(gpu_ctx, case):
     info = sc.list_info()
     if case.startswith("field") and case != "field_low_sun":
         assert info["grid_items"] > 0 and info["reflection_items"] > 0 and info["pair_items"] > 0, info
-    for kernel in (_abi.KERNEL_QUEUED, _abi.KERNEL_MEGA):
+    for kernel in (_abi.KERNEL_QUEUED, _abi.KERNEL_MEGA, _abi.KERNEL_QUEUED_GROUPED):
         r = sc.trace(n, kernel=kernel, **kw).numpy()
         for k in ("flux", "element_hits", "stats", "records", "record_elements"):
             assert np.array_equal(r[k], ref[k]), (case, kernel, k)
