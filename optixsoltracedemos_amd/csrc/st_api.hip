@@ -379,6 +379,7 @@ static int build_cone_lists(st_scene s, size_t tasks, DevBuf& counts, DevBuf& ra
 
 int scene_bounce_lists(st_scene s) {
     hipStream_t st = s->ctx->stream;
+    ++s->list_builds;
     s->bounce_valid = s->pair_valid = false;
     const double theta = sun_cone_angle(s->sun);
     auto common = [&](int32_t tasks, const int32_t* pitems, const int32_t* powner, uint32_t* c, int32_t* scratch) {
@@ -431,6 +432,19 @@ int scene_bounce_lists(st_scene s) {
                 (unsigned long long)s->bounce_items, (unsigned long long)s->pair_items, (unsigned long long)missing,
                 (unsigned long long)entries);
     return ST_OK;
+}
+
+// Whether a trace's reflected rays mix list segments with BVH ones often enough
+// for the grouped form: first-reflection lists without pair lists, a share of
+// pair lists missing, or pair lists more than kPairsGrouped times as long as
+// the first-reflection lists (many blocked reflections: rays after a second
+// mirror walk the BVH). 10k field: the last ratio is 1.45 / 1.72 / 4.2 / 15.7
+// at 81 / 60 / 20 / 5.5 deg elevation, where the grouped form takes +2 / +2.5 /
+// -21 / -21 % of the plain queued form's time.
+static bool mixed_bvh_segments(st_scene s, const JobDev& J) {
+    if (!J.bounce_range) return false;
+    if (!J.pair_range || s->pair_missing > kPairMissingGrouped) return true;
+    return (double)s->pair_items > kPairsGrouped * (double)s->bounce_items;
 }
 
 int route_flux_copies(DevBuf& buf, JobDev* jobs, int32_t n, uint64_t bins, uint64_t rays, hipStream_t st,
@@ -795,8 +809,10 @@ extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* ou
                                 s->topo.depth, s->tuned_valid ? &s->tuned : nullptr, s->n);
     // grouped when the autotuner chose it for this scene, or when reflected rays
     // mix listed segments with BVH ones (no or too few second-reflection lists)
-    use_grouped_bvh(sh, grouped || sh.group_bvh ||
-                            (J.bounce_range && (!J.pair_range || s->pair_missing > kPairMissingGrouped)));
+    // (the autotuner's choice, when it tuned this scene, stands)
+    const bool tuned = s->tuned_valid && d->block_size <= 0 && d->grid_blocks <= 0 &&
+                       s->tuned_list_build == s->list_builds;
+    use_grouped_bvh(sh, grouped || (tuned ? s->tuned.group_bvh : mixed_bvh_segments(s, J)));
     const uint64_t chunks = (J.ray_count + sh.chunk - 1) / sh.chunk;
     if (chunks >= (1ull << 32)) return fail(ST_ERR_INVALID, "too many rays for one call");
     if (launch_injected()) return fail(ST_ERR_INJECTED, "injected launch failure (trace)");
@@ -938,6 +954,7 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
     if (rc != ST_OK) return rc;
     s->tuned = best;
     s->tuned_valid = true;
+    s->tuned_list_build = s->list_builds;
     out->block = best.block;
     out->grid = best.grid;
     out->stack = best.stack;

@@ -147,6 +147,8 @@ struct st_scene_s {
     std::vector<int32_t> h_pair_owner;
     bool bounce_valid = false, pair_valid = false;
     double pair_missing = 1.0;  // share of first-reflection list entries without a second-reflection list
+    uint64_t list_builds = 0, tuned_list_build = ~0ull;  // the autotuner's choice of grouping holds for
+                                                         // the lists (scene state) it was tuned on
     bool lists_stale = true;         // scene state changed since the lists were built
     uint64_t rays_since_change = 0;  // rays traced on the current state without lists
     uint64_t list_min_rays = st::kListMinRays;  // traced rays that repay a rebuild

@@ -97,6 +97,7 @@ bool grouped_variant_exists(int block, int stack, int waves);
 // noticeable share of second-reflection lists missing) traces the two kinds
 // of segment in separate steps when its shape allows it.
 constexpr double kPairMissingGrouped = 0.01;
+constexpr double kPairsGrouped = 2.0;
 inline void use_grouped_bvh(LaunchShape& sh, bool mixed) {
     sh.group_bvh = mixed && sh.queued && !sh.lds_hits && grouped_variant_exists(sh.block, sh.stack, sh.waves);
 }
