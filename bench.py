@@ -111,8 +111,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--sampling", choices=["stratified", "philox", "halton"], default="stratified",
                     help="sun-plane position sampling (stratified: 64-ray wave-coherent cells)")
-    ap.add_argument("--kernel", choices=["auto", "mega", "queued"], default="auto",
-                    help="trace loop form (auto: the tuned one)")
+    ap.add_argument("--kernel", choices=["auto", "mega", "queued", "grouped"], default="auto",
+                    help="trace loop form (auto: the tuned one; grouped: queued with list and BVH "
+                         "segments in separate steps)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -149,7 +150,8 @@ def main():
     if not args.no_autotune and args.block == 0 and args.grid == 0:
         tuning = sc.autotune(sample_rays=rays, max_depth=w.max_depth)   # tune at the measured size
     stream = torch.cuda.Stream(device=0)
-    kernel = {"auto": _abi.KERNEL_AUTO, "mega": _abi.KERNEL_MEGA, "queued": _abi.KERNEL_QUEUED}[args.kernel]
+    kernel = {"auto": _abi.KERNEL_AUTO, "mega": _abi.KERNEL_MEGA, "queued": _abi.KERNEL_QUEUED,
+              "grouped": _abi.KERNEL_QUEUED_GROUPED}[args.kernel]
     desc = sc.make_desc(rays, seed=args.seed, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny,
                         zero=False, block_size=args.block, grid_blocks=args.grid, stream=stream.cuda_stream,
                         kernel=kernel)
