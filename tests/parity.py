@@ -53,7 +53,7 @@ def compare_directions(g_rec, g_id, o_rec, o_id):
     cross = np.linalg.norm(np.cross(dg, do), axis=-1)
     ang = np.arctan2(cross, np.sum(dg * do, axis=-1))
     ang = np.where(seg, ang, 0.0)
-    return ang.max(axis=1)
+    return ang.max(axis=1) if ang.shape[1] else np.zeros(len(ang))
 
 
 def flux_l1(a, b) -> float:
