@@ -597,27 +597,30 @@ def test_batch_stratified_matches_individual(gpu_ctx):
         assert np.array_equal(r["stats"], out["stats"][k].cpu().numpy())
 
 
-@pytest.mark.parametrize("max_depth", [1, 2, 3])
-def test_depth_cap_matches_oracle(gpu_ctx, oracle_mod, max_depth):
-    """Interaction caps below the field's five: a hit at the cap is counted as
+@pytest.mark.parametrize("name,max_depth", [("field_10k", 1), ("field_10k", 2), ("field_10k", 3),
+                                            ("parabolic_dish", 1), ("parabolic_dish", 2),
+                                            ("three_heliostats", 1), ("three_heliostats", 2)])
+def test_depth_cap_matches_oracle(gpu_ctx, oracle_mod, name, max_depth):
+    """Interaction caps below the scene's own: a hit at the cap is counted as
     truncated and not recorded (max_depth 1 records no mirror hit, 2 no
-    receiver hit behind a mirror). The 10k field with its grid and reflection
-    lists, 1e6 rays of a 1e8-ray stratified job, through every kernel form."""
-    w = scenes.heliostat_field(10_000)
+    receiver hit behind a mirror). 1e6 rays of each workload's job, stratified
+    positions, with the sun-plane grid and reflection lists built, through
+    every kernel form."""
+    w = scenes.WORKLOADS[name]()
     sc = tracer.Scene(gpu_ctx, w.elements)
     sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=_abi.POS_STRATIFIED)
     sc.build_lists()
     os_ = oracle_mod.Scene(w.elements)
     os_.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=_abi.POS_STRATIFIED)
     n = 1_000_000
-    kw = dict(max_depth=max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny, ray_begin=3_000_000, ray_count=n,
-              records=True, seed=6)
+    kw = dict(max_depth=max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny, ray_begin=min(3_000_000, w.num_rays - n),
+              ray_count=n, records=True, seed=6)
     o = os_.trace(w.num_rays, threads=16, **kw)
     assert o["stats"][_abi.STAT_NAMES.index("truncated")] > 0
     if max_depth == 1:
         assert o["stats"][_abi.STAT_NAMES.index("mirror_hits")] == 0
     for kernel in (_abi.KERNEL_MEGA, _abi.KERNEL_QUEUED, _abi.KERNEL_QUEUED_GROUPED):
-        print(f"max_depth {max_depth} kernel {kernel}", flush=True)
+        print(f"{name} max_depth {max_depth} kernel {kernel}", flush=True)
         g = sc.trace(w.num_rays, kernel=kernel, **kw).numpy()
         check_parity(w, g, o, n, capped=True)
     sc.close()
