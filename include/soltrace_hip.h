@@ -180,7 +180,14 @@ typedef struct st_trace_desc {
 /* Device output buffers, caller-owned (any of them may be NULL).
  * Accumulated with atomics unless ST_TRACE_ZERO_OUTPUTS is set. */
 typedef struct st_outputs {
-    int64_t* flux;            /* [flux_ny][flux_nx] hit counts                    */
+    int64_t* flux;            /* [flux_ny][flux_nx] hit counts. Flat receiver: x runs
+                                 along -x_b and y along y_b of the element frame (the
+                                 parallelogram v1 = -W x_b, v2 = H y_b of
+                                 lib/element.cpp:136-139 that the flux scripts bin),
+                                 ix = floor((W/2 - x) nx / W). Cylinder: x is
+                                 theta = atan2(-z_local, x_local) mod 2 pi, y along
+                                 the axis (scripts/post_processing/
+                                 plot_flux_map_gpu.py:185-206)                     */
     uint64_t* element_hits;   /* [n_elements] recorded interactions per element   */
     uint64_t* stats;          /* [ST_STAT_COUNT]                                   */
     float* records;           /* [ray_count][max_depth][4] = (tag,x,y,z), tag 0 sun,

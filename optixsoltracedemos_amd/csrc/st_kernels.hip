@@ -743,7 +743,9 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
                     if (th < 0.f) th += 6.283185307179586f;
                     fx = th * (float)J.flux_nx * 0.15915494309189535f;
                 } else {
-                    fx = (lx + q0.w) * ((float)J.flux_nx * 0.5f * __builtin_amdgcn_rcpf(q0.w));
+                    // the map's x runs along -x_b: the scripts take the receiver as the
+                    // parallelogram v1 = -W x_b, v2 = H y_b of lib/element.cpp:136-139
+                    fx = (q0.w - lx) * ((float)J.flux_nx * 0.5f * __builtin_amdgcn_rcpf(q0.w));
                 }
                 fy = (ly + q1.w) * by;
                 int ix = (int)floorf(fx), iy = (int)floorf(fy);

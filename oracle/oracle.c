@@ -18,6 +18,15 @@
 #define OR_MAX_STACK 128
 #define OR_STATS 16
 
+/* Planted mistakes (tests only): the pinning tests switch one on to show that
+ * their check fails on it, then switch it off. 0 = none; see or_debug_plant(). */
+static int g_plant = 0;
+int or_debug_plant(int code) {
+    int prev = g_plant;
+    g_plant = code;
+    return prev;
+}
+
 /* ------------------------------------------------------------------------ */
 /* small vector helpers                                                       */
 static inline void v3set(double* r, double x, double y, double z) { r[0] = x; r[1] = y; r[2] = z; }
@@ -468,6 +477,48 @@ void or_reflect(const double d[3], const double n[3], double out[3]) {
     for (int i = 0; i < 3; ++i) out[i] = d[i] - k * n[i];
 }
 
+/* Outgoing direction of a mirror interaction (cuda/materials.cu:26-84, with
+ * SolTrace's error model added): the surface normal n is turned to face the
+ * incoming ray d (faceforward, :38), tilted by the slope error (sigma_slope,
+ * draws xs_r, xs_p), d is reflected about it (:51), and the reflected ray is
+ * tilted by the specular error (sigma_spec, draws xp_r, xp_p). */
+void or_mirror_direction(const double d[3], const double n[3], double sigma_slope, double sigma_spec,
+                         double xs_r, double xs_p, double xp_r, double xp_p, double out[3]) {
+    double nf[3] = {n[0], n[1], n[2]};
+    if (dot3(d, nf) > 0) { nf[0] = -nf[0]; nf[1] = -nf[1]; nf[2] = -nf[2]; }
+    double rd[3], tmp[3];
+    if (sigma_slope > 0) {
+        if (g_plant == 5) {
+            /* planted: slope error applied to the reflected ray */
+            or_reflect(d, nf, rd);
+            normalize3(rd);
+            or_perturb(rd, sigma_slope, xs_r, xs_p, tmp);
+            memcpy(rd, tmp, sizeof(tmp));
+        } else {
+            or_perturb(nf, sigma_slope, xs_r, xs_p, tmp);
+            memcpy(nf, tmp, sizeof(tmp));
+            or_reflect(d, nf, rd);
+            normalize3(rd);
+        }
+    } else {
+        or_reflect(d, nf, rd);
+        normalize3(rd);
+    }
+    if (sigma_spec > 0) {
+        if (g_plant == 6) {
+            /* planted: specular error applied to the normal before reflection */
+            double n2[3];
+            or_perturb(nf, sigma_spec, xp_r, xp_p, n2);
+            or_reflect(d, n2, rd);
+            normalize3(rd);
+        } else {
+            or_perturb(rd, sigma_spec, xp_r, xp_p, tmp);
+            memcpy(rd, tmp, sizeof(tmp));
+        }
+    }
+    memcpy(out, rd, sizeof(rd));
+}
+
 /* Sun ray: __raygen__sun_source, cuda/sun.cu:94-134. Position on the sun
  * parallelogram v0 + a (v1-v0) + b (v3-v0) (:28-38) with (a,b) = Halton(2,3)
  * of the ray index or Philox uniforms; direction uniform over the cap of
@@ -744,19 +795,35 @@ typedef struct or_worker {
 } or_worker;
 
 /* Flux bin of a receiver hit, scripts/post_processing/flux_map_calculator.py:95-179
- * (flat: floor((x + W/2) nx / W); cylinder: floor(theta nx / 2 pi),
- * floor((z + H/2) ny / H); indices clipped to the grid). theta is taken in the
- * cylinder frame, atan2(-z_local, x_local) mod 2 pi, which equals the script's
- * global atan2(y, x) for its tower convention base_x = (1,0,0), base_z = (0,-1,0). */
+ * and plot_flux_map_gpu.py:150-206 (indices clipped to the grid, map [ny][nx] as
+ * flux_2D[raybins_y, raybins_x] at plot_flux_map_gpu.py:263).
+ *  - Flat: the scripts describe the receiver as the parallelogram (v1, v2,
+ *    anchor) that lib/element.cpp:136-139 makes of an element: v1 = -W x_b,
+ *    v2 = H y_b, so the map's x runs along -x_b:
+ *    ix = floor((W/2 - x_local) nx / W), iy = floor((y_local + H/2) ny / H).
+ *    (plot_flux_map_gpu.py:63-66 hard-codes exactly this parallelogram for the
+ *    three-heliostat receiver at (0,0,10) aimed at (0,5,0).)
+ *  - Cylinder: theta = atan2(-z_local, x_local) mod 2 pi, which equals the
+ *    scripts' global atan2(y - c_y, x - c_x) for their tower frame base_x =
+ *    (1,0,0), base_z = (0,-1,0) (plot_flux_map_gpu.py:125-130, 185, 201);
+ *    z along base_z x base_x; ix = floor(theta nx / 2 pi), iy = floor((z + H/2) ny / H). */
 static int flux_bin(const or_prim* p, const or_hit* h, int nx, int ny) {
     double fx, fy;
     if (p->surface == 3) {
-        double th = atan2(-h->local[2], h->local[0]);
+        double th = g_plant == 1 ? atan2(h->local[2], h->local[0]) : atan2(-h->local[2], h->local[0]);
+        if (g_plant == 2) th += OR_PI;
         if (th < 0) th += 2 * OR_PI;
+        if (th >= 2 * OR_PI) th -= 2 * OR_PI;
         fx = th * nx / (2 * OR_PI);
         fy = (h->local[1] + p->hh) * ny / (2 * p->hh);
-    } else {
+    } else if (g_plant == 3) {
+        fx = (p->hh - h->local[1]) * nx / (2 * p->hh);
+        fy = (h->local[0] + p->hw) * ny / (2 * p->hw);
+    } else if (g_plant == 4) {
         fx = (h->local[0] + p->hw) * nx / (2 * p->hw);
+        fy = (h->local[1] + p->hh) * ny / (2 * p->hh);
+    } else {
+        fx = (p->hw - h->local[0]) * nx / (2 * p->hw);
         fy = (h->local[1] + p->hh) * ny / (2 * p->hh);
     }
     long ix = (long)floor(fx), iy = (long)floor(fy);
@@ -829,23 +896,10 @@ static void trace_ray(or_worker* w, uint64_t local) {
         double ue[4] = {0, 0, 0, 0};
         if (rho < 1.0 || p->slope > 0 || p->spec > 0) draw4(ray, (uint32_t)(2 * nd - 1), dsc->job_id, dsc->seed, ue);
         if (rho < 1.0 && ue[0] >= rho) { w->stats[5]++; return; }
-        /* faceforward(n, -d, n) (:38): n facing the incoming ray */
-        double nf[3] = {h.n[0], h.n[1], h.n[2]};
-        if (dot3(d, nf) > 0) { nf[0] = -nf[0]; nf[1] = -nf[1]; nf[2] = -nf[2]; }
-        if (p->slope > 0) {
-            double tmp[3];
-            or_perturb(nf, p->slope, ue[1], ue[2], tmp);
-            memcpy(nf, tmp, sizeof(tmp));
-        }
+        double us[4] = {0, 0, 0, 0};
+        if (p->spec > 0) draw4(ray, (uint32_t)(2 * nd), dsc->job_id, dsc->seed, us);
         double rd[3];
-        or_reflect(d, nf, rd);
-        normalize3(rd);
-        if (p->spec > 0) {
-            double us[4], tmp[3];
-            draw4(ray, (uint32_t)(2 * nd), dsc->job_id, dsc->seed, us);
-            or_perturb(rd, p->spec, ue[3], us[0], tmp);
-            memcpy(rd, tmp, sizeof(tmp));
-        }
+        or_mirror_direction(d, h.n, p->slope, p->spec, ue[1], ue[2], ue[3], us[0], rd);
         memcpy(o, h.p, sizeof(o));
         memcpy(d, rd, sizeof(d));
         tmin = 1e-2; /* cuda/materials.cu:70 */

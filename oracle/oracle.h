@@ -121,6 +121,15 @@ void or_sample_sun(const or_scene* s, uint64_t ray, uint64_t seed, uint32_t job,
                    double dir[3]);
 void or_perturb(const double a[3], double sigma, double xi_r, double xi_phi, double out[3]);
 void or_reflect(const double d[3], const double n[3], double out[3]);
+/* mirror interaction: faceforward, slope error on the normal, reflect, specular
+ * error on the reflected ray (the composition the trace uses) */
+void or_mirror_direction(const double d[3], const double n[3], double sigma_slope, double sigma_spec,
+                         double xs_r, double xs_p, double xp_r, double xp_p, double out[3]);
+/* tests only: switch on planted mistake `code` (0 = none), returns the previous
+ * code. 1 cylinder theta mirrored, 2 cylinder theta + pi, 3 flat map axes
+ * swapped, 4 flat map x along +x_b, 5 slope error on the reflected ray,
+ * 6 specular error on the normal. */
+int or_debug_plant(int code);
 
 /* the full trace: rays [ray_begin, ray_begin+ray_count) with `threads` workers */
 int or_trace(const or_scene* s, const or_trace_desc* desc, const or_outputs* out, int32_t threads);

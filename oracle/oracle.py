@@ -86,6 +86,11 @@ def lib():
         L.or_perturb.restype = None
         L.or_reflect.argtypes = [d3, d3, d3]
         L.or_reflect.restype = None
+        L.or_mirror_direction.argtypes = [d3, d3, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, d3]
+        L.or_mirror_direction.restype = None
+        L.or_debug_plant.argtypes = [C.c_int]
+        L.or_debug_plant.restype = C.c_int
         L.or_trace.argtypes = [C.c_void_p, C.POINTER(OrTraceDesc), C.POINTER(OrOutputs), C.c_int32]
         _lib = L
     return _lib
@@ -220,3 +225,24 @@ def reflect(d, n):
     out = (C.c_double * 3)()
     lib().or_reflect(_d3(d), _d3(n), out)
     return np.array(out)
+
+
+def mirror_direction(d, n, sigma_slope, sigma_spec, xs_r=0.5, xs_p=0.5, xp_r=0.5, xp_p=0.5):
+    out = (C.c_double * 3)()
+    lib().or_mirror_direction(_d3(d), _d3(n), sigma_slope, sigma_spec, xs_r, xs_p, xp_r, xp_p, out)
+    return np.array(out)
+
+
+class planted:
+    """Context manager: trace with planted mistake `code` switched on (tests only)."""
+
+    def __init__(self, code: int):
+        self.code = code
+
+    def __enter__(self):
+        self.prev = lib().or_debug_plant(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_debug_plant(self.prev)
+        return False
