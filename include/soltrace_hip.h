@@ -268,8 +268,29 @@ typedef struct st_tune_result {
 int st_scene_autotune(st_scene scene, uint64_t sample_rays, int32_t max_depth, st_tune_result* out);
 
 /* ---- tracing ----------------------------------------------------------- */
-/* Trace on desc->stream (or the scene stream). Synchronous unless stream != NULL. */
+/* Trace on desc->stream (or the scene stream). Synchronous unless stream != NULL.
+ * Replaces optixLaunch in SolTraceSystem::run() (lib/soltrace_system.cpp:145-170)
+ * and the offline binning of scripts/post_processing/flux_map_calculator.py:95-179.
+ * desc->kernel: an explicit form runs as asked (ST_KERNEL_QUEUED_GROUPED fails
+ * with ST_ERR_INVALID when no grouped variant fits the shape); ST_KERNEL_AUTO
+ * takes the tuned form or the default and picks the grouped form where
+ * reflected rays mix listed and BVH segments. Position sampling ST_POS_HALTON
+ * needs ray_begin + count <= 2^32 (the radical inverse takes a 32-bit index,
+ * as the reference's unsigned launch index, cuda/sun.cu:94-134).
+ * Traces of one scene on different streams are ordered (each waits for the
+ * scene's previous trace): they share the scene's chunk counter and flux copies. */
 int st_trace(st_scene scene, const st_trace_desc* desc, const st_outputs* out);
+
+/* The trace-kernel variant the scene's last st_trace launched (new; tests and
+ * profiling). */
+typedef struct st_launch_info {
+    int32_t kernel;          /* ST_KERNEL_MEGA / _QUEUED / _QUEUED_GROUPED launched */
+    int32_t block_size, grid_blocks, stack, waves_per_simd, chunk_rays;
+    int32_t lds_hits;        /* element hits counted per wave in LDS              */
+    int32_t flux_copies;     /* copies the flux map was accumulated in (1 = none) */
+    uint64_t chunks;
+} st_launch_info;
+int st_scene_last_launch(st_scene scene, st_launch_info* info);
 /* FP64 normalisation: flux_wm2[i] = flux[i] * dni * area / num_rays / bin_area. */
 int st_flux_to_watts(st_scene scene, const st_trace_desc* desc, const int64_t* flux_dev,
                      double* flux_wm2_dev, double* power_per_ray);

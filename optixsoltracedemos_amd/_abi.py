@@ -147,6 +147,12 @@ class TuneResult(C.Structure):
                 ("trials", C.c_int32), ("queued", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double)]
 
 
+class LaunchInfo(C.Structure):
+    _fields_ = [("kernel", C.c_int32), ("block_size", C.c_int32), ("grid_blocks", C.c_int32), ("stack", C.c_int32),
+                ("waves_per_simd", C.c_int32), ("chunk_rays", C.c_int32), ("lds_hits", C.c_int32),
+                ("flux_copies", C.c_int32), ("chunks", C.c_uint64)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [
         ("hbm_budget_bytes", C.c_uint64),
@@ -223,6 +229,7 @@ SIGNATURES = {
     "st_scene_build_lists": (C.c_int, [P]),
     "st_scene_list_info": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "st_trace": (C.c_int, [P, C.POINTER(TraceDesc), C.POINTER(Outputs)]),
+    "st_scene_last_launch": (C.c_int, [P, C.POINTER(LaunchInfo)]),
     "st_flux_to_watts": (C.c_int, [P, C.POINTER(TraceDesc), P, P, C.POINTER(C.c_double)]),
     "st_trace_batch": (C.c_int, [P, C.POINTER(BatchDesc), P, P, P]),
     "st_server_create": (C.c_int, [P, C.POINTER(ServerConfig), C.POINTER(P)]),

@@ -82,12 +82,15 @@ LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kerne
     const int cus = device_cu_count();
     // ST_KERNEL_MEGA: whole ray per lane; ST_KERNEL_QUEUED: continuing rays
     // regrouped in full waves; AUTO keeps the tuned (or default) form
+    // an explicit grouped form is a queued form that must keep a grouped variant
+    const bool want_grouped = kernel == ST_KERNEL_QUEUED_GROUPED;
+    const bool want_queued = kernel == ST_KERNEL_QUEUED || want_grouped;
     const bool use_tuned = tuned && block <= 0 && grid <= 0 &&
-                           (kernel == ST_KERNEL_AUTO || (kernel == ST_KERNEL_QUEUED) == tuned->queued);
+                           (kernel == ST_KERNEL_AUTO || want_queued == tuned->queued);
     if (use_tuned) {
         sh = *tuned;
     } else {
-        sh.queued = kernel == ST_KERNEL_AUTO ? kDefaultQueued : kernel == ST_KERNEL_QUEUED;
+        sh.queued = kernel == ST_KERNEL_AUTO ? kDefaultQueued : want_queued;
         sh.block = block > 0 ? block : 256;
         sh.grid = grid > 0 ? grid : cus * 8;
         sh.stack = bvh_depth <= 16 && sh.block >= 128 ? 16 : 32;
@@ -99,7 +102,7 @@ LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kerne
         sh.waves = sh.queued ? 3 : 4;
         if (!trace_variant_exists(sh.block, sh.stack, sh.waves, sh.queued)) sh.block = 128;
     }
-    if (sh.queued && min_elements <= kLdsHitElements && bvh_depth <= 16 && block <= 0) {
+    if (sh.queued && !want_grouped && min_elements <= kLdsHitElements && bvh_depth <= 16 && block <= 0) {
         sh.block = 256;
         sh.stack = bvh_depth <= 8 ? 8 : 16;
         sh.waves = 5;
@@ -181,6 +184,35 @@ void context_release(st_context ctx) {
 static bool capturing(hipStream_t s) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+int order_scene_traces(st_scene s, hipStream_t st) {
+    if (!s->trace_done || s->trace_stream == st || capturing(st)) return ST_OK;
+    HIP_TRY(hipStreamWaitEvent(st, s->trace_done, 0), "wait for the scene's last trace");
+    return ST_OK;
+}
+
+int mark_scene_trace(st_scene s, hipStream_t st) {
+    if (capturing(st)) return ST_OK;
+    if (!s->trace_done) HIP_TRY(hipEventCreateWithFlags(&s->trace_done, hipEventDisableTiming), "trace event");
+    HIP_TRY(hipEventRecord(s->trace_done, st), "trace event");
+    s->trace_stream = st;
+    return ST_OK;
+}
+
+st_launch_info launch_info(const LaunchShape& sh, uint64_t chunks, uint32_t copies) {
+    st_launch_info li;
+    memset(&li, 0, sizeof(li));
+    li.kernel = !sh.queued ? ST_KERNEL_MEGA : sh.group_bvh ? ST_KERNEL_QUEUED_GROUPED : ST_KERNEL_QUEUED;
+    li.block_size = sh.block;
+    li.grid_blocks = sh.grid;
+    li.stack = sh.stack;
+    li.waves_per_simd = sh.waves;
+    li.chunk_rays = sh.chunk;
+    li.lds_hits = sh.lds_hits ? 1 : 0;
+    li.flux_copies = (int32_t)copies;
+    li.chunks = chunks;
+    return li;
 }
 
 int order_before_trace(st_context ctx, hipStream_t user) {
@@ -567,6 +599,10 @@ st_scene_s::~st_scene_s() {
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
     }
+    if (trace_done) {
+        (void)hipEventSynchronize(trace_done);
+        (void)hipEventDestroy(trace_done);
+    }
     if (batch_graph_exec) (void)hipGraphExecDestroy(batch_graph_exec);
 }
 
@@ -774,6 +810,8 @@ int validate_trace(st_scene s, const st_trace_desc* d, const st_outputs* out) {
     if (d->ray_begin >= d->num_rays) return fail(ST_ERR_INVALID, "ray_begin beyond num_rays");
     uint64_t count = d->ray_count ? d->ray_count : d->num_rays - d->ray_begin;
     if (d->ray_begin + count > d->num_rays) return fail(ST_ERR_INVALID, "ray range exceeds num_rays");
+    if (s->sun.position_sampling == ST_POS_HALTON && d->ray_begin + count > kHaltonMaxRays)
+        return fail(ST_ERR_INVALID, "Halton positions take a 32-bit ray index: ray_begin + count must be <= 2^32");
     if (d->max_depth < 1 || d->max_depth > kMaxDepthLimit)
         return fail(ST_ERR_INVALID, "max_depth %d outside [1, %d]", d->max_depth, kMaxDepthLimit);
     if (d->block_size != 0 && d->block_size != 64 && d->block_size != 128 && d->block_size != 256)
@@ -804,24 +842,44 @@ extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* ou
         if (out->element_hits) HIP_TRY(launch_fill_u64(st, out->element_hits, (uint64_t)s->n, 0), "zero hits");
         if (out->stats) HIP_TRY(launch_fill_u64(st, out->stats, ST_STAT_COUNT, 0), "zero stats");
     }
-    const bool grouped = d->kernel == ST_KERNEL_QUEUED_GROUPED;
-    LaunchShape sh = pick_shape(J.ray_count, d->block_size, d->grid_blocks, grouped ? ST_KERNEL_QUEUED : d->kernel,
-                                s->topo.depth, s->tuned_valid ? &s->tuned : nullptr, s->n);
-    // grouped when the autotuner chose it for this scene, or when reflected rays
-    // mix listed segments with BVH ones (no or too few second-reflection lists)
-    // (the autotuner's choice, when it tuned this scene, stands)
-    const bool tuned = s->tuned_valid && d->block_size <= 0 && d->grid_blocks <= 0 &&
-                       s->tuned_list_build == s->list_builds;
-    use_grouped_bvh(sh, grouped || (tuned ? s->tuned.group_bvh : mixed_bvh_segments(s, J)));
+    LaunchShape sh = pick_shape(J.ray_count, d->block_size, d->grid_blocks, d->kernel, s->topo.depth,
+                                s->tuned_valid ? &s->tuned : nullptr, s->n);
+    // An explicit form runs as asked. Under ST_KERNEL_AUTO the grouped form is
+    // used when the autotuner chose it for this scene (its choice stands for
+    // the lists it was tuned on), or when reflected rays mix listed segments
+    // with BVH ones (no or too few second-reflection lists).
+    bool group = false;
+    if (d->kernel == ST_KERNEL_QUEUED_GROUPED) {
+        group = true;
+    } else if (d->kernel == ST_KERNEL_AUTO) {
+        const bool tuned = s->tuned_valid && d->block_size <= 0 && d->grid_blocks <= 0 &&
+                           s->tuned_list_build == s->list_builds;
+        group = tuned ? s->tuned.group_bvh : mixed_bvh_segments(s, J);
+    }
+    use_grouped_bvh(sh, group);
+    if (d->kernel == ST_KERNEL_QUEUED_GROUPED && !sh.group_bvh)
+        return fail(ST_ERR_INVALID, "no grouped trace variant for block %d, stack %d, %d waves/SIMD", sh.block,
+                    sh.stack, sh.waves);
     const uint64_t chunks = (J.ray_count + sh.chunk - 1) / sh.chunk;
     if (chunks >= (1ull << 32)) return fail(ST_ERR_INVALID, "too many rays for one call");
     if (launch_injected()) return fail(ST_ERR_INJECTED, "injected launch failure (trace)");
+    // the chunk counter and the flux copies belong to the scene: a trace on
+    // another stream than the scene's last one waits for that one's trace
+    ST_TRY(order_scene_traces(s, st));
     int64_t* const flux_out = J.flux;
     uint32_t copies = 1;
     if (J.flux) ST_TRY(route_flux_copies(s->d_flux_copies, &J, 1, J.flux_bins, J.ray_count, st, &copies));
     HIP_TRY(launch_trace(st, nullptr, nullptr, 1, chunks, (uint32_t*)s->d_counter.ptr, sh, J), "trace kernel");
     if (copies > 1) HIP_TRY(launch_flux_gather(st, J.flux, copies, J.flux_bins, 1, flux_out), "flux gather");
+    ST_TRY(mark_scene_trace(s, st));
+    s->last_launch = launch_info(sh, chunks, copies);
     if (!d->stream) HIP_TRY(hipStreamSynchronize(st), "trace sync");
+    return ST_OK;
+}
+
+extern "C" int st_scene_last_launch(st_scene s, st_launch_info* info) {
+    if (!s || !info) return fail(ST_ERR_INVALID, "null argument");
+    *info = s->last_launch;
     return ST_OK;
 }
 
@@ -1139,6 +1197,8 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
         const double* v = d->sun_vectors + 3 * k;
         if (!(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] > 0)) return fail(ST_ERR_INVALID, "step %d: zero sun vector", k);
         if (d->budgets[k] == 0) return fail(ST_ERR_INVALID, "step %d: zero ray budget", k);
+        if (d->sun.position_sampling == ST_POS_HALTON && d->budgets[k] > kHaltonMaxRays)
+            return fail(ST_ERR_INVALID, "step %d: Halton positions take a 32-bit ray index (budget > 2^32)", k);
     }
     int32_t fslot = -1;
     ST_TRY(resolve_flux_element(s, d->flux_element, d->flux_nx, d->flux_ny, &fslot));

@@ -162,6 +162,12 @@ struct st_scene_s {
     hipGraphExec_t batch_graph_exec = nullptr;
     bool tuned_valid = false;
     st::LaunchShape tuned;
+    // the scene's last st_trace: its stream and an event after it (traces on
+    // another stream wait for it: they share d_counter and d_flux_copies), and
+    // the variant it launched (st_scene_last_launch)
+    hipEvent_t trace_done = nullptr;
+    hipStream_t trace_stream = nullptr;
+    st_launch_info last_launch{};
     ~st_scene_s();
 };
 
@@ -182,4 +188,7 @@ int resolve_flux_element(st_scene s, int32_t requested, int32_t nx, int32_t ny, 
 int fill_job(st_scene s, const st_trace_desc* d, const st_outputs* out, const void* prims, const void* nodes,
              const void* sun, JobDev* J);
 int validate_trace(st_scene s, const st_trace_desc* d, const st_outputs* out);
+int order_scene_traces(st_scene s, hipStream_t st);
+int mark_scene_trace(st_scene s, hipStream_t st);
+st_launch_info launch_info(const LaunchShape& sh, uint64_t chunks, uint32_t copies);
 }  // namespace st

@@ -53,6 +53,9 @@ struct JobDev {
 // is traced block by block of kHaltonBlock aligned indices, 64 rays of one such
 // class per wave, when its indices (the sequence's 32-bit index) stay below 2^32.
 constexpr uint32_t kHaltonCells = 256u * 243u, kHaltonBlock = 64u * kHaltonCells;
+// Halton positions take the 32-bit ray index (the reference's unsigned launch
+// index, cuda/sun.cu:94-134): jobs whose ray range passes 2^32 are rejected
+constexpr uint64_t kHaltonMaxRays = 1ull << 32;
 inline void set_trace_order(JobDev* J, int32_t position_sampling) {
     J->order_head = 0;
     J->order_blocks = 0;

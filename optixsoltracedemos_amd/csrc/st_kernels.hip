@@ -702,7 +702,13 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
     if (listed) closest_in_list(J, L.items, L.lb, L.le, L.headed, o, d, tmin, t, slot, cap, bk, wc);
     else traverse<STRIDE>(J.nodes, J.prims, J.meta, o, d, tmin, t, slot, cap, stk,
                           (J.flags & ST_TRACE_COUNT_WORK) ? wc : nullptr);
-    if (slot < 0) return false;  // __miss__ms, cuda/materials.cu:236-253 (counted at the chunk's end)
+    if (slot < 0) {
+        // __miss__ms, cuda/materials.cu:236-253: derived at the chunk's end, and
+        // counted here too when work is counted (the flush then keeps the count,
+        // so a ST_TRACE_COUNT_WORK run checks the derivation)
+        if (J.flags & ST_TRACE_COUNT_WORK) count(wc, ST_STAT_ESCAPED);
+        return false;
+    }
     const int nd = depth + 1;
     const uint32_t m = ld1(J.meta + slot);
     const uint32_t kind = meta_kind(m), elem = meta_elem(m);
@@ -1057,7 +1063,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
         uint32_t v = lane < kNumCounters ? wc[lane] : 0u;
         if (lane == ST_STAT_RAYS) v = rays;
         if (lane == ST_STAT_SEGMENTS) v = rays + wc[ST_STAT_MIRROR_HITS] - wc[ST_STAT_ABSORBED];
-        if (lane == ST_STAT_ESCAPED)
+        if (lane == ST_STAT_ESCAPED && !(J.flags & ST_TRACE_COUNT_WORK))
             v = rays - wc[ST_STAT_ABSORBED] - wc[ST_STAT_BACK_FACE] - wc[ST_STAT_TRUNCATED] - wc[ST_STAT_RECEIVER_HITS];
         if (lane < kNumCounters) wc[lane] = 0;
         if (v) {

@@ -700,6 +700,8 @@ extern "C" int st_server_submit(st_server srv, const st_job_desc* job, int64_t* 
     if (!(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] > 0)) return fail(ST_ERR_INVALID, "zero sun vector");
     if (job->sun.shape != ST_SUN_PILLBOX && job->sun.shape != ST_SUN_GAUSSIAN) return fail(ST_ERR_INVALID, "bad sun shape");
     if (!(job->sun.param >= 0) || job->sun.param > 0.2) return fail(ST_ERR_INVALID, "bad sun angle");
+    if (job->sun.position_sampling == ST_POS_HALTON && job->num_rays > kHaltonMaxRays)
+        return fail(ST_ERR_INVALID, "Halton positions take a 32-bit ray index (num_rays > 2^32)");
     st_scene sc = srv->scenes[job->scene_slot];
     bool has_receiver = false;
     for (const auto& e : sc->host_elems) has_receiver |= e.role == ST_ROLE_RECEIVER;

@@ -187,6 +187,12 @@ class Scene:
         self.trace_into(desc, res)
         return res
 
+    def last_launch(self) -> dict:
+        """The trace-kernel variant the scene's last trace launched."""
+        li = _abi.LaunchInfo()
+        call("st_scene_last_launch", self.h, C.byref(li))
+        return {k: getattr(li, k) for k, _ in _abi.LaunchInfo._fields_}
+
     def flux_to_watts(self, desc: TraceDesc, flux):
         torch = _torch()
         out = torch.zeros(flux.shape, dtype=torch.float64, device=flux.device)

@@ -917,6 +917,8 @@ int or_trace(const or_scene* s, const or_trace_desc* desc, const or_outputs* out
     if (!s || !desc || !out || !s->sun_set) return -1;
     if (desc->max_depth < 1 || desc->max_depth > 32) return -1;
     uint64_t count = desc->ray_count ? desc->ray_count : desc->num_rays - desc->ray_begin;
+    /* Halton positions take the 32-bit launch index of cuda/sun.cu:94-134 */
+    if (s->sun.position_sampling == 1 && desc->ray_begin + count > (1ull << 32)) return -3;
     int32_t fe = desc->flux_element;
     if (fe < 0) {
         for (int i = s->n - 1; i >= 0; --i)

@@ -32,7 +32,7 @@ def test_every_declared_symbol_is_exported_and_bound():
 STRUCTS = {"Element": "st_element_f64", "SunDesc": "st_sun_desc", "SunPlane": "st_sun_plane",
            "TraceDesc": "st_trace_desc", "Outputs": "st_outputs", "BatchDesc": "st_batch_desc",
            "ServerConfig": "st_server_config", "JobDesc": "st_job_desc", "JobResult": "st_job_result",
-           "ServerStats": "st_server_stats", "TuneResult": "st_tune_result"}
+           "ServerStats": "st_server_stats", "TuneResult": "st_tune_result", "LaunchInfo": "st_launch_info"}
 
 
 def test_abi_version_and_struct_layouts(tmp_path):

@@ -256,3 +256,17 @@ def test_stratified_positions_uniform(oracle_mod):
     b = s.trace(400_000, max_depth=5, flux_nx=8, flux_ny=8, threads=8)
     ha, hb = int(a["stats"][3]), int(b["stats"][3])
     assert abs(ha - hb) < 5 * math.sqrt(ha + hb)
+
+
+def test_halton_range_past_2_32_is_rejected(oracle_mod):
+    """Halton positions take the 32-bit launch index of cuda/sun.cu:94-134: a ray
+    range that passes 2^32 is an error, one that ends at 2^32 is traced."""
+    w = scenes.three_heliostats()
+    s = oracle_mod.Scene(w.elements)
+    s.set_sun(w.sun_vector, w.sun_param, position_sampling=POS_HALTON)
+    with pytest.raises(RuntimeError):
+        s.trace((1 << 32) + 64, ray_begin=(1 << 32) - 64, ray_count=128, threads=1)
+    r = s.trace(1 << 32, ray_begin=(1 << 32) - 128, ray_count=128, threads=1)
+    assert r["stats"][0] == 128
+    s.set_sun(w.sun_vector, w.sun_param)   # Philox positions take the 64-bit index
+    assert s.trace((1 << 32) + 64, ray_begin=(1 << 32) - 64, ray_count=128, threads=1)["stats"][0] == 128
