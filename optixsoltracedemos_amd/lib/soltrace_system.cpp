@@ -14,6 +14,7 @@
 
 #include "data_manager.h"
 #include "element.h"
+#include "stinput.h"
 
 namespace {
 void check(int rc, const char* what) {
@@ -299,7 +300,7 @@ void SolTraceSystem::write_csv(const std::string& filename, bool with_elements) 
         check(st_buffer_download(m_ctx, ids.data(), m_record_elements, n * sizeof(uint32_t)), "download elements");
     std::ofstream out(filename);
     if (!out.is_open()) {
-        std::cerr << "Error: Could not open the file " << filename << " for writing." << std::endl;
+        std::cerr << "write_output: cannot create " << filename << std::endl;
         return;
     }
     out << "number,stage,loc_x,loc_y,loc_z" << (with_elements ? ",element" : "") << "\n";
@@ -323,203 +324,99 @@ void SolTraceSystem::write_csv(const std::string& filename, bool with_elements) 
         out << "\n";
         ++stage;
     }
-    if (m_verbose) std::cout << "Data successfully written to " << filename << std::endl;
+    if (m_verbose) std::cout << "write_output: " << ray - (stage == 0 ? 1 : 0) << " rays written to " << filename << std::endl;
 }
 
 void SolTraceSystem::write_output(const std::string& filename) { write_csv(filename, false); }
 void SolTraceSystem::write_output_with_elements(const std::string& filename) { write_csv(filename, true); }
 
 // ---------------------------------------------------------------------------
-// STINPUT reader (format of data/stinput/*.stinput, parsed at
-// lib/soltrace_system.cpp:449-846)
-void SolTraceSystem::read_line(char* buf, int len, FILE* fp) {
-    buf[0] = 0;
-    if (!fgets(buf, len, fp)) {
-        buf[0] = 0;
-        return;
-    }
-    size_t n = strlen(buf);
-    while (n > 0 && (buf[n - 1] == '\n' || buf[n - 1] == '\r')) buf[--n] = 0;
-}
-
-std::vector<std::string> SolTraceSystem::split(const std::string& str, const std::string& delim, bool ret_empty,
-                                               bool ret_delim) {
-    std::vector<std::string> out;
-    std::string::size_type pos = 0;
-    while (pos < str.length()) {
-        std::string::size_type next = str.find_first_of(delim, pos);
-        std::string token = next == std::string::npos ? str.substr(pos) : str.substr(pos, next - pos);
-        const char d = next == std::string::npos ? 0 : str[next];
-        pos = next == std::string::npos ? str.length() : next + 1;
-        if (token.empty() && !ret_empty) continue;
-        out.push_back(token);
-        if (ret_delim && d && pos < str.length()) out.push_back(std::string(1, d));
-    }
-    return out;
-}
-
-bool SolTraceSystem::read_sun(FILE* fp) {
-    char buf[1024];
-    int ptsrc = 0, count = 0;
-    char shape = 'g';
-    double sigma = 0, halfwidth = 0;
-    read_line(buf, 1023, fp);
-    if (sscanf(buf, "SUN\tPTSRC\t%d\tSHAPE\t%c\tSIGMA\t%lg\tHALFWIDTH\t%lg", &ptsrc, &shape, &sigma, &halfwidth) < 3) {
-        if (m_verbose) printf("bad SUN line: %s\n", buf);
-        return false;
-    }
-    shape = (char)tolower(shape);
-    // the reference treats SIGMA (mrad) as the pillbox half-angle for any shape
-    set_sun_angle(sigma * 0.001);
-    if (m_use_stinput_optics) set_sun_shape(shape == 'g' ? ST_SUN_GAUSSIAN : ST_SUN_PILLBOX);
-    double x = 0, y = 0, z = 0, lat = 0, day = 0, hour = 0;
-    int use_ldh = 0;
-    read_line(buf, 1023, fp);
-    if (sscanf(buf, "XYZ\t%lg\t%lg\t%lg\tUSELDH\t%d\tLDH\t%lg\t%lg\t%lg", &x, &y, &z, &use_ldh, &lat, &day, &hour) < 3) {
-        if (m_verbose) printf("bad XYZ line: %s\n", buf);
-        return false;
-    }
-    set_sun_vector(Vector3d(x, y, z));
-    read_line(buf, 1023, fp);
-    sscanf(buf, "USER SHAPE DATA\t%d", &count);
-    for (int i = 0; i < count; ++i) read_line(buf, 1023, fp);  // user sun shape table: not used
-    return true;
-}
-
-bool SolTraceSystem::read_optic_surface(FILE* fp, bool front, const std::string& name) {
-    char buf[1024];
-    read_line(buf, 1023, fp);
-    std::vector<std::string> parts = split(std::string(buf), "\t", true, false);
-    if (parts.size() < 15) {
-        printf("too few tokens for optical surface: %zu\n", parts.size());
-        printf("\t>> %s\n", buf);
-        return false;
-    }
-    if (front) m_optics.push_back({name, atof(parts[5].c_str()), atof(parts[7].c_str()), atof(parts[8].c_str())});
-    int refl_points = 0, trans_points = 0;
-    bool refl_table = false, trans_table = false;
-    if (parts.size() >= 17) {
-        refl_table = atoi(parts[15].c_str()) > 0;
-        refl_points = atoi(parts[16].c_str());
-        if (parts.size() >= 19) {
-            trans_table = atoi(parts[17].c_str()) > 0;
-            trans_points = atoi(parts[18].c_str());
-        }
-    }
-    if (refl_table)
-        for (int i = 0; i < refl_points; ++i) read_line(buf, 1023, fp);
-    if (trans_table)
-        for (int i = 0; i < trans_points; ++i) read_line(buf, 1023, fp);
-    return true;
-}
-
-bool SolTraceSystem::read_optic(FILE* fp) {
-    char buf[1024];
-    read_line(buf, 1023, fp);
-    if (strncmp(buf, "OPTICAL PAIR", 12) != 0) return false;
-    std::string name = buf + 12;
-    while (!name.empty() && (name[0] == '\t' || name[0] == ' ')) name.erase(0, 1);
-    return read_optic_surface(fp, true, name) && read_optic_surface(fp, false, name);
-}
-
-bool SolTraceSystem::read_element(FILE* fp) {
-    char buf[1024];
-    read_line(buf, 1023, fp);
-    std::vector<std::string> tok = split(buf, "\t", true, false);
-    if (tok.size() < 29) {
-        printf("too few tokens for element: %d\n", (int)tok.size());
-        printf("\t>> %s\n", buf);
-        return false;
-    }
-    const char ap = tok[8].empty() ? 0 : tok[8][0];
-    const char sf = tok[17].empty() ? 0 : tok[17][0];
-    if (ap == 'c' && sf == 'f') return true;  // cylinder end cap: skipped (:689-693)
-    auto e = std::make_shared<Element>();
-    Vector3d origin(atof(tok[1].c_str()), atof(tok[2].c_str()), atof(tok[3].c_str()));
-    const bool cyl = ap == 'l' && sf == 't';
-    if (cyl) origin[1] += 1.0 / atof(tok[18].c_str());  // tok[18] = 1 / radius
-    e->set_origin(origin);
-    e->set_aim_point(Vector3d(atof(tok[4].c_str()), atof(tok[5].c_str()), atof(tok[6].c_str())));
-    e->set_zrot(atof(tok[7].c_str()));
-    if (ap == 'r') {
-        e->set_aperture(std::make_shared<ApertureRectangle>(atof(tok[9].c_str()), atof(tok[10].c_str())));
-    } else if (cyl) {
-        e->set_aperture(std::make_shared<ApertureRectangle>(2.0 / atof(tok[18].c_str()), atof(tok[11].c_str())));
-    } else {
-        printf("Aperture type not implemented: %s\n", tok[8].c_str());
-        return false;
-    }
-    if (sf == 'p') {
-        auto s = std::make_shared<SurfaceParabolic>();
-        s->set_curvature(atof(tok[18].c_str()), atof(tok[19].c_str()));
-        e->set_surface(s);
-    } else if (cyl) {
-        auto s = std::make_shared<SurfaceCylinder>();
-        s->set_radius(1.0 / atof(tok[18].c_str()));
-        s->set_half_height(atof(tok[11].c_str()) / 2);
-        e->set_surface(s);
-    } else if (sf == 'f') {
-        e->set_surface(std::make_shared<SurfaceFlat>());
-    } else {
-        printf("Surface type not implemented: %s\n", tok[17].c_str());
-        return false;
-    }
-    add_element(e);
-    m_element_optic.back() = tok[27];
-    return true;
-}
-
-bool SolTraceSystem::read_stage(FILE* fp) {
-    char buf[1024];
-    read_line(buf, 1023, fp);
-    int virt = 0, multi = 1, count = 0, tr = 0;
-    double x, y, z, ax, ay, az, zrot;
-    if (sscanf(buf, "STAGE\tXYZ\t%lg\t%lg\t%lg\tAIM\t%lg\t%lg\t%lg\tZROT\t%lg\tVIRTUAL\t%d\tMULTIHIT\t%d\tELEMENTS\t%d\tTRACETHROUGH\t%d",
-               &x, &y, &z, &ax, &ay, &az, &zrot, &virt, &multi, &count, &tr) < 10)
-        return false;
-    read_line(buf, 1023, fp);  // stage name
-    for (int i = 0; i < count; ++i)
-        if (!read_element(fp)) {
-            printf("error in element %d\n", i);
-            return false;
-        }
-    return true;
-}
-
-bool SolTraceSystem::read_system(FILE* fp) {
-    char buf[1024];
-    int c = fgetc(fp);
-    if (c != '#') {
-        if (c != EOF) ungetc(c, fp);
-        printf("input file must start with '#'\n");
-        return false;
-    }
-    int vmaj = 0, vmin = 0, vmic = 0;
-    read_line(buf, 1023, fp);
-    sscanf(buf, " SOLTRACE VERSION %d.%d.%d INPUT FILE", &vmaj, &vmin, &vmic);
-    if (m_verbose) printf("loading input file version %d.%d.%d\n", vmaj, vmin, vmic);
-    if (!read_sun(fp)) return false;
-    int count = 0;
-    read_line(buf, 1023, fp);
-    sscanf(buf, "OPTICS LIST COUNT\t%d", &count);
-    for (int i = 0; i < count; ++i)
-        if (!read_optic(fp)) return false;
-    count = 0;
-    read_line(buf, 1023, fp);
-    sscanf(buf, "STAGE LIST COUNT\t%d", &count);
-    for (int i = 0; i < count; ++i)
-        if (!read_stage(fp)) return false;
-    return true;
-}
-
+// STINPUT: the file is parsed into a stinput::Document (stinput.cpp) and then
+// turned into elements here, with the reference's reading of the format
+// (lib/soltrace_system.cpp:449-846):
+//  * SIGMA (mrad) becomes the sun's half-angle whatever SHAPE says (:513); the
+//    shape itself is used only with set_use_stinput_optics(true);
+//  * stage frames are taken as the global frame (element coordinates are global);
+//  * a rectangle ('r') aperture gives width, height = parameters 0, 1; a
+//    single-axis-curvature section ('l') with a cylindrical ('t') surface is a
+//    cylinder of radius 1 / surface parameter 0 and length aperture parameter 2,
+//    centred radius further along y than its row's origin; a circular ('c') flat
+//    row is a cylinder end cap and is skipped;
+//  * surfaces: 'f' flat, 'p' parabolic with curvatures = surface parameters 0, 1;
+//  * the optic named in the row is kept per element; its front side's
+//    reflectivity, RMS slope and RMS specularity (mrad) reach the trace with
+//    set_use_stinput_optics(true) (the reference parses and drops them, :576-590).
 bool SolTraceSystem::read_st_input(const char* filename) {
-    FILE* fp = fopen(filename, "r");
-    if (!fp) {
-        if (m_verbose) printf("failed to open system input file\n");
+    stinput::Document doc;
+    std::string err;
+    if (!filename || !stinput::parse_file(filename, doc, err)) {
+        std::cerr << "read_st_input: " << (filename ? filename : "(null)") << ": " << err << std::endl;
         return false;
     }
-    const bool ok = read_system(fp);
-    if (!ok && m_verbose) printf("error in system input file.\n");
-    fclose(fp);
-    return ok;
+    return apply_stinput(doc);
+}
+
+bool SolTraceSystem::apply_stinput(const stinput::Document& doc) {
+    std::vector<std::shared_ptr<Element>> elems;
+    std::vector<std::string> optic_names;
+    for (const stinput::Stage& st : doc.stages) {
+        for (const stinput::ElementRow& r : st.elements) {
+            const bool cylinder = r.aperture == 'l' && r.surface == 't';
+            if (r.aperture == 'c' && r.surface == 'f') continue;  // cylinder end cap
+            auto e = std::make_shared<Element>();
+            Vector3d origin(r.origin[0], r.origin[1], r.origin[2]);
+            if (cylinder) {
+                if (!(r.surface_params[0] > 0)) {
+                    std::cerr << "read_st_input: line " << r.line << ": cylinder curvature must be positive" << std::endl;
+                    return false;
+                }
+                origin[1] += 1.0 / r.surface_params[0];
+            }
+            e->set_origin(origin);
+            e->set_aim_point(Vector3d(r.aim[0], r.aim[1], r.aim[2]));
+            e->set_zrot(r.zrot_deg);
+            if (r.aperture == 'r') {
+                e->set_aperture(std::make_shared<ApertureRectangle>(r.aperture_params[0], r.aperture_params[1]));
+            } else if (cylinder) {
+                e->set_aperture(std::make_shared<ApertureRectangle>(2.0 / r.surface_params[0], r.aperture_params[2]));
+            } else {
+                std::cerr << "read_st_input: line " << r.line << ": aperture code '" << r.aperture
+                          << "' is not supported (rectangle 'r', or 'l' with a cylindrical surface)" << std::endl;
+                return false;
+            }
+            if (r.surface == 'f') {
+                e->set_surface(std::make_shared<SurfaceFlat>());
+            } else if (r.surface == 'p') {
+                auto s = std::make_shared<SurfaceParabolic>();
+                s->set_curvature(r.surface_params[0], r.surface_params[1]);
+                e->set_surface(s);
+            } else if (cylinder) {
+                auto s = std::make_shared<SurfaceCylinder>();
+                s->set_radius(1.0 / r.surface_params[0]);
+                s->set_half_height(0.5 * r.aperture_params[2]);
+                e->set_surface(s);
+            } else {
+                std::cerr << "read_st_input: line " << r.line << ": surface code '" << r.surface
+                          << "' is not supported (flat 'f', parabolic 'p', cylindrical 't')" << std::endl;
+                return false;
+            }
+            elems.push_back(e);
+            optic_names.push_back(r.optic);
+        }
+    }
+    // the rows are all valid: only now does the system change
+    set_sun_angle(doc.sun.sigma_mrad * 1e-3);
+    if (m_use_stinput_optics) set_sun_shape(doc.sun.shape == 'g' ? ST_SUN_GAUSSIAN : ST_SUN_PILLBOX);
+    set_sun_vector(Vector3d(doc.sun.xyz[0], doc.sun.xyz[1], doc.sun.xyz[2]));
+    for (const stinput::OpticPair& p : doc.optics)
+        m_optics.push_back({p.name, p.front.reflectivity, p.front.rms_slope_mrad, p.front.rms_specularity_mrad});
+    for (size_t i = 0; i < elems.size(); ++i) {
+        add_element(elems[i]);
+        m_element_optic.back() = optic_names[i];
+    }
+    if (m_verbose)
+        std::cout << "read_st_input: STINPUT " << doc.version[0] << "." << doc.version[1] << "." << doc.version[2]
+                  << ", " << doc.stages.size() << " stages, " << elems.size() << " elements, " << doc.optics.size()
+                  << " optics" << std::endl;
+    return true;
 }

@@ -15,6 +15,9 @@
 
 class dataManager;
 class Element;
+namespace stinput {
+struct Document;
+}
 
 class SolTraceSystem {
 public:
@@ -61,19 +64,14 @@ public:
     int get_sun_shape() const { return m_sun_shape; }
     st_scene get_scene() const { return m_scene; }
     const std::vector<std::shared_ptr<Element>>& get_elements() const { return m_element_list; }
+    // the FP64 element records initialize() hands to the tracer (optics included)
+    std::vector<st_element_f64> get_element_records() const { return collect_elements(); }
 
 private:
     std::vector<st_element_f64> collect_elements() const;
     void write_csv(const std::string& filename, bool with_elements);
 
-    bool read_system(FILE* fp);
-    bool read_stage(FILE* fp);
-    bool read_element(FILE* fp);
-    bool read_optic(FILE* fp);
-    bool read_optic_surface(FILE* fp, bool front, const std::string& name);
-    bool read_sun(FILE* fp);
-    void read_line(char* buf, int len, FILE* fp);
-    std::vector<std::string> split(const std::string& str, const std::string& delim, bool ret_empty, bool ret_delim);
+    bool apply_stinput(const stinput::Document& doc);
 
     std::shared_ptr<dataManager> data_manager;
     int m_num_sunpoints;

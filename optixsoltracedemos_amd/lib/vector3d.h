@@ -7,6 +7,7 @@
 #include <initializer_list>
 #include <iostream>
 #include <stdexcept>
+#include <string>
 
 class Vector3d {
 public:
@@ -51,7 +52,9 @@ public:
               double a33)
         : m_{a11, a12, a13, a21, a22, a23, a31, a32, a33} {}
     Matrix33d(std::initializer_list<double> init) {
-        if (init.size() != 9) throw std::runtime_error("must have exactly 9 elements.");
+        if (init.size() != 9)
+            throw std::runtime_error("Matrix33d: an initializer list of " + std::to_string(init.size()) +
+                                     " values (row-major 3x3 needs 9)");
         int i = 0;
         for (double x : init) m_[i++] = x;
     }

@@ -69,7 +69,11 @@ def test_three_heliostats_demo_vs_oracle(oracle_mod, tmp_path):
     assert skipped <= 1 and k > 1000
 
 
-def parse_fixture_elements():
+# OPTICAL rows of the fixture (front side): reflectivity, RMS slope, RMS specularity (mrad)
+FIXTURE_OPTICS = {"Reflector": (1.0, 0.95, 0.2), "Absorber": (0.0, 0.95, 0.2)}
+
+
+def parse_fixture_elements(optics=False):
     rows = [ln.rstrip("\n").split("\t") for ln in open(FIXTURE)]
     rows = [t for t in rows if len(t) >= 29 and t[0] == "1"]
     e = scenes.make_elements(len(rows))
@@ -82,13 +86,16 @@ def parse_fixture_elements():
             e[i]["surface"] = _abi.SURFACE_PARABOLIC
             e[i]["curv_x"], e[i]["curv_y"] = float(t[18]), float(t[19])
         e[i]["role"] = _abi.ROLE_RECEIVER if i == len(rows) - 1 else _abi.ROLE_MIRROR
+        if optics:
+            rho, slope, spec = FIXTURE_OPTICS[t[27]]
+            e[i]["reflectivity"], e[i]["slope_error"], e[i]["spec_error"] = rho, slope * 1e-3, spec * 1e-3
     return e
 
 
 def test_stinput_demo_vs_oracle(oracle_mod):
     n = 300_000
     text = run("demo_stinput", FIXTURE, n, "-", 1)
-    st = kv(text.strip().splitlines()[-1])
+    st = kv([ln for ln in text.splitlines() if ln.startswith("elements")][0])
     assert int(st["elements"]) == 4
     e = parse_fixture_elements()
     s = oracle_mod.Scene(e)
@@ -96,6 +103,43 @@ def test_stinput_demo_vs_oracle(oracle_mod):
     o = s.trace(n, max_depth=5)
     assert abs(int(st["receiver_hits"]) - int(o["stats"][3])) <= 3
     assert int(st["receiver_hits"]) > 0
+
+
+def _stinput_run(tmp_path, n, optics):
+    flux = tmp_path / f"flux_{int(optics)}.txt"
+    args = [FIXTURE, n, "-", 1] + (["optics"] if optics else []) + ["seed=0", f"flux=32x24:{flux}"]
+    text = run("demo_stinput", *args)
+    lines = text.splitlines()
+    st = kv([ln for ln in lines if ln.startswith("elements")][0])
+    hits = np.array([int(x) for x in [ln for ln in lines if ln.startswith("element_hits")][0].split(",")[1:]])
+    return st, hits, np.loadtxt(flux, dtype=np.int64)
+
+
+def test_stinput_optics_reach_the_trace(oracle_mod, tmp_path):
+    """Toy STINPUT with set_use_stinput_optics(true): the file's reflectivity, RMS
+    slope and RMS specularity reach the kernels. The drop-in's flux map and
+    per-element counts equal an oracle trace with the same optics over the same
+    rays (Halton positions, seed 0), within the per-ray bar's bin flips; the same
+    file without optics gives a clearly different map."""
+    n = 1_000_000
+    st, hits, flux = _stinput_run(tmp_path, n, True)
+    e = parse_fixture_elements(optics=True)
+    s = oracle_mod.Scene(e)
+    s.set_sun([0, 0, 100], 4.65e-3, position_sampling=_abi.POS_HALTON)
+    o = s.trace(n, max_depth=5, flux_nx=32, flux_ny=24, seed=0)
+    assert flux.shape == (24, 32) and flux.sum() > 10_000
+    assert int(st["receiver_hits"]) == pytest.approx(int(o["stats"][3]), abs=5)
+    assert np.abs(hits - o["element_hits"].astype(np.int64)).sum() <= 10
+    l1 = np.abs(flux - o["flux"]).sum() / o["flux"].sum()
+    assert l1 <= 2e-3, l1
+    # without the file's optics: no slope or specular error, so a different map
+    _, _, flux0 = _stinput_run(tmp_path, n, False)
+    e0 = parse_fixture_elements(optics=False)
+    s0 = oracle_mod.Scene(e0)
+    s0.set_sun([0, 0, 100], 4.65e-3, position_sampling=_abi.POS_HALTON)
+    o0 = s0.trace(n, max_depth=5, flux_nx=32, flux_ny=24, seed=0)
+    assert np.abs(flux0 - o0["flux"]).sum() / o0["flux"].sum() <= 2e-3
+    assert np.abs(flux - flux0).sum() / flux0.sum() > 0.02     # oracle: 0.046, ten times the parity bar
 
 
 def test_dynamic_scene_demo_vs_oracle(oracle_mod):
