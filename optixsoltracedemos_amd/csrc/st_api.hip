@@ -827,6 +827,34 @@ int validate_trace(st_scene s, const st_trace_desc* d, const st_outputs* out) {
 }
 }  // namespace st
 
+namespace st {
+// Launch shape of one st_trace / st_trace_noalloc call. An explicit form runs
+// as asked. Under ST_KERNEL_AUTO the grouped form is used when the autotuner
+// chose it for this scene (its choice stands for the lists it was tuned on),
+// or when reflected rays mix listed segments with BVH ones (no or too few
+// second-reflection lists).
+int trace_shape(st_scene s, const st_trace_desc* d, const JobDev& J, LaunchShape* out, uint64_t* chunks) {
+    LaunchShape sh = pick_shape(J.ray_count, d->block_size, d->grid_blocks, d->kernel, s->topo.depth,
+                                s->tuned_valid ? &s->tuned : nullptr, s->n);
+    bool group = false;
+    if (d->kernel == ST_KERNEL_QUEUED_GROUPED) {
+        group = true;
+    } else if (d->kernel == ST_KERNEL_AUTO) {
+        const bool tuned = s->tuned_valid && d->block_size <= 0 && d->grid_blocks <= 0 &&
+                           s->tuned_list_build == s->list_builds;
+        group = tuned ? s->tuned.group_bvh : mixed_bvh_segments(s, J);
+    }
+    use_grouped_bvh(sh, group);
+    if (d->kernel == ST_KERNEL_QUEUED_GROUPED && !sh.group_bvh)
+        return fail(ST_ERR_INVALID, "no grouped trace variant for block %d, stack %d, %d waves/SIMD", sh.block,
+                    sh.stack, sh.waves);
+    *chunks = (J.ray_count + sh.chunk - 1) / sh.chunk;
+    if (*chunks >= (1ull << 32)) return fail(ST_ERR_INVALID, "too many rays for one call");
+    *out = sh;
+    return ST_OK;
+}
+}  // namespace st
+
 extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* out) {
     ST_TRY(validate_trace(s, d, out));
     HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
@@ -842,26 +870,9 @@ extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* ou
         if (out->element_hits) HIP_TRY(launch_fill_u64(st, out->element_hits, (uint64_t)s->n, 0), "zero hits");
         if (out->stats) HIP_TRY(launch_fill_u64(st, out->stats, ST_STAT_COUNT, 0), "zero stats");
     }
-    LaunchShape sh = pick_shape(J.ray_count, d->block_size, d->grid_blocks, d->kernel, s->topo.depth,
-                                s->tuned_valid ? &s->tuned : nullptr, s->n);
-    // An explicit form runs as asked. Under ST_KERNEL_AUTO the grouped form is
-    // used when the autotuner chose it for this scene (its choice stands for
-    // the lists it was tuned on), or when reflected rays mix listed segments
-    // with BVH ones (no or too few second-reflection lists).
-    bool group = false;
-    if (d->kernel == ST_KERNEL_QUEUED_GROUPED) {
-        group = true;
-    } else if (d->kernel == ST_KERNEL_AUTO) {
-        const bool tuned = s->tuned_valid && d->block_size <= 0 && d->grid_blocks <= 0 &&
-                           s->tuned_list_build == s->list_builds;
-        group = tuned ? s->tuned.group_bvh : mixed_bvh_segments(s, J);
-    }
-    use_grouped_bvh(sh, group);
-    if (d->kernel == ST_KERNEL_QUEUED_GROUPED && !sh.group_bvh)
-        return fail(ST_ERR_INVALID, "no grouped trace variant for block %d, stack %d, %d waves/SIMD", sh.block,
-                    sh.stack, sh.waves);
-    const uint64_t chunks = (J.ray_count + sh.chunk - 1) / sh.chunk;
-    if (chunks >= (1ull << 32)) return fail(ST_ERR_INVALID, "too many rays for one call");
+    LaunchShape sh;
+    uint64_t chunks = 0;
+    ST_TRY(trace_shape(s, d, J, &sh, &chunks));
     if (launch_injected()) return fail(ST_ERR_INJECTED, "injected launch failure (trace)");
     // the chunk counter and the flux copies belong to the scene: a trace on
     // another stream than the scene's last one waits for that one's trace
