@@ -281,6 +281,19 @@ int st_scene_autotune(st_scene scene, uint64_t sample_rays, int32_t max_depth, s
  * scene's previous trace): they share the scene's chunk counter and flux copies. */
 int st_trace(st_scene scene, const st_trace_desc* desc, const st_outputs* out);
 
+/* st_trace without device allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory - code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc. Same arguments, results
+ * and errors as st_trace, but it calls no hipMalloc/hipFree: candidate lists are
+ * not built on demand (those the scene already holds are used; results are the
+ * same either way), and the chunk counter and the copies of a small flux map
+ * live in a static device scratch of 2 MiB + 256 B per GPU (held until the
+ * process ends; every flux-copy set st_trace would use fits, larger ones are
+ * rejected with ST_ERR_INVALID). The scratch is cleared by kernels at each call
+ * and keeps the last call's contents; calls that overlap on one GPU (several
+ * streams or threads) share it and race - serialise them. */
+int st_trace_noalloc(st_scene scene, const st_trace_desc* desc, const st_outputs* out);
+
 /* The trace-kernel variant the scene's last st_trace launched (new; tests and
  * profiling). */
 typedef struct st_launch_info {

@@ -230,6 +230,7 @@ SIGNATURES = {
     "st_scene_list_info": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "st_trace": (C.c_int, [P, C.POINTER(TraceDesc), C.POINTER(Outputs)]),
     "st_scene_last_launch": (C.c_int, [P, C.POINTER(LaunchInfo)]),
+    "st_trace_noalloc": (C.c_int, [P, C.POINTER(TraceDesc), C.POINTER(Outputs)]),
     "st_flux_to_watts": (C.c_int, [P, C.POINTER(TraceDesc), P, P, C.POINTER(C.c_double)]),
     "st_trace_batch": (C.c_int, [P, C.POINTER(BatchDesc), P, P, P]),
     "st_server_create": (C.c_int, [P, C.POINTER(ServerConfig), C.POINTER(P)]),

@@ -152,6 +152,40 @@ extern "C" int st_device_count(int* count) {
     return ST_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Device scratch of st_trace_noalloc: a file-scope array, one copy per GPU for
+// the life of the process. [0, kScratchCounterBytes) holds the trace's chunk
+// counter, the rest the copies of a small flux map. Its device address is taken
+// once per device (st_context_create, device current) and kept in
+// g_scratch_ptr; host code passes that address, never the array's name.
+namespace st {
+constexpr size_t kScratchCounterBytes = 256;
+constexpr size_t kScratchFluxBytes = 2u << 20;  // every map the flux-copy rule applies to fits
+constexpr size_t kScratchBytes = kScratchCounterBytes + kScratchFluxBytes;
+constexpr int kScratchDevices = 64;
+}  // namespace st
+static __device__ __align__(256) unsigned char scratch[st::kScratchBytes];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[st::kScratchDevices] = {};
+
+namespace st {
+static int scratch_setup(int device) {
+    if (device < 0 || device >= kScratchDevices) return fail(ST_ERR_INVALID, "device %d beyond the scratch table", device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return ST_OK;
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(ST_ERR_DEVICE, "no device address for the trace scratch");
+    g_scratch_ptr[device] = p;
+    return ST_OK;
+}
+static void* scratch_of(int device) {
+    if (device < 0 || device >= kScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+}  // namespace st
+
 extern "C" int st_context_create(int device, st_context* out) {
     if (!out) return fail(ST_ERR_INVALID, "null output handle");
     *out = nullptr;
@@ -159,6 +193,7 @@ extern "C" int st_context_create(int device, st_context* out) {
     ST_TRY(st_device_count(&n));
     if (device < 0 || device >= n) return fail(ST_ERR_INVALID, "device %d out of range (%d devices)", device, n);
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    ST_TRY(scratch_setup(device));
     auto ctx = std::make_unique<st_context_s>();
     ctx->device = device;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -883,6 +918,52 @@ extern "C" int st_trace(st_scene s, const st_trace_desc* d, const st_outputs* ou
     HIP_TRY(launch_trace(st, nullptr, nullptr, 1, chunks, (uint32_t*)s->d_counter.ptr, sh, J), "trace kernel");
     if (copies > 1) HIP_TRY(launch_flux_gather(st, J.flux, copies, J.flux_bins, 1, flux_out), "flux gather");
     ST_TRY(mark_scene_trace(s, st));
+    s->last_launch = launch_info(sh, chunks, copies);
+    if (!d->stream) HIP_TRY(hipStreamSynchronize(st), "trace sync");
+    return ST_OK;
+}
+
+// st_trace without device allocation: no lazy candidate-list build (lists the
+// scene already holds are used), the chunk counter and the flux copies in the
+// device's scratch instead of the scene's buffers. Both are cleared by kernels
+// at each call (graph-safe); calls overlapping on one GPU share the scratch
+// and race.
+extern "C" int st_trace_noalloc(st_scene s, const st_trace_desc* d, const st_outputs* out) {
+    ST_TRY(validate_trace(s, d, out));
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    char* const scr = (char*)scratch_of(s->ctx->device);
+    if (!scr) return fail(ST_ERR_STATE, "no trace scratch on device %d (context not created)", s->ctx->device);
+    JobDev J;
+    ST_TRY(fill_job(s, d, out, s->d_prims.ptr, s->d_nodes.ptr, s->d_sun.ptr, &J));
+    set_trace_order(&J, s->sun.position_sampling);
+    hipStream_t st = d->stream ? (hipStream_t)d->stream : s->ctx->stream;
+    ST_TRY(order_before_trace(s->ctx, st));
+    LaunchShape sh;
+    uint64_t chunks = 0;
+    ST_TRY(trace_shape(s, d, J, &sh, &chunks));
+    // flux copies as st_trace takes them, halved while they exceed the scratch
+    uint32_t copies = J.flux ? flux_copies(J.flux_bins, J.ray_count) : 1;
+    while (copies > 1 && (uint64_t)copies * J.flux_bins * sizeof(int64_t) > kScratchFluxBytes) copies >>= 1;
+    if (copies > 1 && (uint64_t)copies * J.flux_bins * sizeof(int64_t) > kScratchFluxBytes)
+        return fail(ST_ERR_INVALID, "flux copies of %llu bytes exceed the %zu-byte trace scratch",
+                    (unsigned long long)((uint64_t)copies * J.flux_bins * sizeof(int64_t)), kScratchFluxBytes);
+    if (launch_injected()) return fail(ST_ERR_INJECTED, "injected launch failure (trace)");
+    if ((d->flags & ST_TRACE_ZERO_OUTPUTS) && out) {
+        if (J.flux) HIP_TRY(launch_fill_u64(st, J.flux, (uint64_t)d->flux_nx * d->flux_ny, 0), "zero flux");
+        if (out->element_hits) HIP_TRY(launch_fill_u64(st, out->element_hits, (uint64_t)s->n, 0), "zero hits");
+        if (out->stats) HIP_TRY(launch_fill_u64(st, out->stats, ST_STAT_COUNT, 0), "zero stats");
+    }
+    uint32_t* const counter = (uint32_t*)scr;
+    HIP_TRY(launch_fill_u64(st, counter, kScratchCounterBytes / sizeof(uint64_t), 0), "clear chunk counter");
+    int64_t* const flux_out = J.flux;
+    if (copies > 1) {
+        int64_t* const region = (int64_t*)(scr + kScratchCounterBytes);
+        HIP_TRY(launch_fill_u64(st, region, (uint64_t)copies * J.flux_bins, 0), "clear flux copies");
+        J.flux = region;
+        J.flux_copy_mask = copies - 1;
+    }
+    HIP_TRY(launch_trace(st, nullptr, nullptr, 1, chunks, counter, sh, J), "trace kernel");
+    if (copies > 1) HIP_TRY(launch_flux_gather(st, J.flux, copies, J.flux_bins, 1, flux_out), "flux gather");
     s->last_launch = launch_info(sh, chunks, copies);
     if (!d->stream) HIP_TRY(hipStreamSynchronize(st), "trace sync");
     return ST_OK;

@@ -191,4 +191,5 @@ int validate_trace(st_scene s, const st_trace_desc* d, const st_outputs* out);
 int order_scene_traces(st_scene s, hipStream_t st);
 int mark_scene_trace(st_scene s, hipStream_t st);
 st_launch_info launch_info(const LaunchShape& sh, uint64_t chunks, uint32_t copies);
+int trace_shape(st_scene s, const st_trace_desc* d, const JobDev& J, LaunchShape* out, uint64_t* chunks);
 }  // namespace st

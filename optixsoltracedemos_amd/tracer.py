@@ -181,6 +181,11 @@ class Scene:
         o = self.outputs_struct(res)
         call("st_trace", self.h, C.byref(desc), C.byref(o))
 
+    def trace_noalloc_into(self, desc: TraceDesc, res: TraceResult):
+        """st_trace_noalloc: the same trace with no device allocation in the call."""
+        o = self.outputs_struct(res)
+        call("st_trace_noalloc", self.h, C.byref(desc), C.byref(o))
+
     def trace(self, num_rays, **kw) -> TraceResult:
         desc = self.make_desc(num_rays, **kw)
         res = self.alloc_outputs(desc)
