@@ -213,6 +213,42 @@ ST_INLINE U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
 // 23-bit uniform ((x >> 9) + 0.5) / 2^23 in (0,1); u and 1 - u are exact in FP32
 ST_INLINE float u01(uint32_t x) { return ((float)(x >> 9) + 0.5f) * (1.0f / 8388608.0f); }
 
+// FP32 reciprocal: the hardware v_rcp_f32 on the device (about 1 ulp), a
+// division on the host
+ST_INLINE float rcp_f32(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
+// atan2(y, x) in (-pi, pi] for the cylindrical flux bin. Octant reduction to
+// a = min(|x|,|y|) / max(|x|,|y|) in [0, 1], then the polynomial of
+// Abramowitz & Stegun, Handbook of Mathematical Functions, eq. 4.4.49 (from
+// C. Hastings, Approximations for Digital Computers, 1955):
+//   atan(a) = a (1 + a2 a^2 + a4 a^4 + ... + a16 a^16),  |error| <= 2e-8 on [0, 1],
+// so the FP32 result errs by its own rounding, a few 1e-8 rad (checked on
+// the host against the libm atan2 by tests/cpp/atan_check.cpp). atan2(0, 0) = 0.
+ST_INLINE float atan2_poly(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float a = mx > 0.f ? mn * rcp_f32(mx) : 0.f;
+    const float s = a * a;
+    float r = fmaf(s, 0.0028662257f, -0.0161657367f);
+    r = fmaf(s, r, 0.0429096138f);
+    r = fmaf(s, r, -0.0752896400f);
+    r = fmaf(s, r, 0.1065626393f);
+    r = fmaf(s, r, -0.1420889944f);
+    r = fmaf(s, r, 0.1999355085f);
+    r = fmaf(s, r, -0.3333314528f);
+    r = fmaf(s, r, 1.0f);
+    r *= a;
+    if (ay > ax) r = 1.5707963267948966f - r;
+    if (x < 0.f) r = 3.141592653589793f - r;
+    return copysignf(r, y);
+}
+
 // ---------------------------------------------------------------------------
 // The conservative tests that build candidate lists: the sun-plane grid's
 // footprints, the reflection cone against boxes and the flat-mirror sweep. They

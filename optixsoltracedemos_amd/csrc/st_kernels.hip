@@ -116,27 +116,6 @@ __device__ __forceinline__ V3 perturb(V3 a, float sigma, float xr, float xp) {
     return normalize(fma3(ct, a, st * fma3(cp, u, sp * v)));
 }
 
-// atan2(y, x) in (-pi, pi]: octant reduction and a degree-15 odd polynomial
-// fitted to atan on [0, 1] (error ~1.3e-7 rad in FP32, at the level of the
-// rounding of the result); atan2(0, 0) = 0
-__device__ __forceinline__ float fast_atan2(float y, float x) {
-    const float ax = fabsf(x), ay = fabsf(y);
-    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
-    const float a = mx > 0.f ? mn * __builtin_amdgcn_rcpf(mx) : 0.f;
-    const float s = a * a;
-    float r = fmaf(s, -0.0039626276f, 0.021518581f);
-    r = fmaf(s, r, -0.055396538f);
-    r = fmaf(s, r, 0.096028775f);
-    r = fmaf(s, r, -0.13892591f);
-    r = fmaf(s, r, 0.19943213f);
-    r = fmaf(s, r, -0.33329552f);
-    r = fmaf(s, r, 0.99999923f);
-    r *= a;
-    if (ay > ax) r = 1.5707963267948966f - r;
-    if (x < 0.f) r = 3.141592653589793f - r;
-    return copysignf(r, y);
-}
-
 __device__ __forceinline__ float halton2(uint32_t i) {
     return (float)((double)__brev(i) * 2.3283064365386962890625e-10);
 }
@@ -745,7 +724,7 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
                 const float by = (float)J.flux_ny * 0.5f * __builtin_amdgcn_rcpf(q1.w);
                 if (kind == kKindCylinder) {
                     const float lz = fmaf(t, dot(d, v3(q3)), dot(r, v3(q3)));
-                    float th = fast_atan2(-lz, lx);
+                    float th = atan2_poly(-lz, lx);
                     if (th < 0.f) th += 6.283185307179586f;
                     fx = th * (float)J.flux_nx * 0.15915494309189535f;
                 } else {

@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# ST_ORACLE_LIB selects another build of the same source (the sanitizer test's)
+LIB = os.environ.get("ST_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 STATS = 16
 
