@@ -39,6 +39,34 @@ OPS_CYLINDER_TEST = 70  # wall quadratic about closest approach + two caps
 OPS_MIRROR_HIT = 172    # Philox (60), faceforward, slope-error cone (81), reflection (14), counts
 OPS_RECEIVER_HIT = 70   # local coordinates, atan2 bin, aggregated flux add
 HBM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_field10k_hbm_traffic.txt")
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, published
+RECORD_BYTES_PER_SLOT = 16 + 4   # float4 (tag, x, y, z) + u32 element id per interaction slot
+
+
+def cpu_model() -> str:
+    """The host CPU's model string (what lscpu prints as "Model name")."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def time_launches(sc, desc, res, stream, n):
+    """Mean kernel time (ms, HIP events on the launch stream) of n launches after one warm-up."""
+    import numpy as np
+    import torch
+    sc.trace_into(desc, res)
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+    for i in range(n):
+        starts[i].record(stream)
+        sc.trace_into(desc, res)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    return float(np.mean([a.elapsed_time(b) for a, b in zip(starts, ends)]))
 
 
 def ops_per_ray(stats) -> float:
@@ -75,7 +103,7 @@ def cpu_baseline(workload, sample: int, pos: int):
     rec = {"value": sample / dt, "unit": "rays/s", "cores": threads, "kind": "port",
            "sample": f"{whole} {sample:.0e} rays of the same {workload.num_rays:.0e}-ray field workload, "
                      f"FP64 scalar oracle, {threads} threads, {dt:.2f} s wall (~{dt * threads:.0f} thread-seconds)",
-           "host_cpus": os.cpu_count()}
+           "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
     # one thread on a smaller prefix of the same rays
     one = min(sample, 2_000_000)
     t0 = time.perf_counter()
@@ -109,6 +137,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-vs-plain annual sweep")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the Halton-position and record-mode launches beside the headline")
     ap.add_argument("--sampling", choices=["stratified", "philox", "halton"], default="stratified",
                     help="sun-plane position sampling (stratified: 64-ray wave-coherent cells)")
     ap.add_argument("--kernel", choices=["auto", "mega", "queued", "grouped"], default="auto",
@@ -206,7 +236,7 @@ def main():
                    "rays_per_step": rays, "flux_bins": f"{w.flux_nx}x{w.flux_ny}", "max_depth": w.max_depth,
                    "sun": "pillbox 4.65 mrad, el 60 az 180", "slope_error_mrad": 1.5, "reflectivity": 0.93,
                    "seed": args.seed, "sampling": args.sampling, "parallelism": "single GPU"},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s",
                      "frac": achieved / VALU_PEAK_TOPS, "traffic": read_traffic(),
                      "peak_plain_fp32_measured": plain_peak, "frac_of_plain_fp32": achieved / plain_peak,
                      "ops_per_ray": w_ops, "kernel_ms": kernel_ms,
@@ -218,11 +248,40 @@ def main():
         "segments_per_s": float(per_call[1]) * args.steps / wall,
         "receiver_hits_per_ray": float(per_call[3] / rays),
     }
+    line["launch"] = sc.last_launch()
+    if not args.no_variants:
+        # the reference's raygen (Halton positions, cuda/sun.cu:16-38) on the same rays
+        sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=_abi.POS_HALTON)
+        hdesc = sc.make_desc(rays, seed=args.seed, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny,
+                             zero=True, block_size=args.block, grid_blocks=args.grid, stream=stream.cuda_stream,
+                             kernel=kernel)
+        hres = sc.alloc_outputs(hdesc)
+        hms = time_launches(sc, hdesc, hres, stream, 5)
+        line["halton_positions"] = {"kernel_ms": hms, "rays_per_s": rays / (hms * 1e-3), "launch": sc.last_launch(),
+                                    "receiver_hits_per_ray": float(hres.stats.cpu()[3]) / rays}
+        # reference-compatible record mode: every launch writes the hit-record
+        # buffer (lib/soltrace_system.cpp:118-125) plus the element id of each slot
+        sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=pos)
+        rdesc = sc.make_desc(rays, seed=args.seed, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny,
+                             zero=True, records=True, block_size=args.block, grid_blocks=args.grid,
+                             stream=stream.cuda_stream, kernel=kernel)
+        rres = sc.alloc_outputs(rdesc)
+        rms = time_launches(sc, rdesc, rres, stream, 3)
+        rbytes = float(rays) * w.max_depth * RECORD_BYTES_PER_SLOT
+        line["record_mode"] = {"kernel_ms": rms, "rays_per_s": rays / (rms * 1e-3),
+                               "record_bytes_per_launch": rbytes,
+                               "record_gbs": rbytes / (rms * 1e-3) / 1e9,
+                               "frac_of_hbm_peak": rbytes / (rms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                               "hbm_floor_ms": rbytes / (HBM_PEAK_GBS * 1e9) * 1e3}
+        del rres
+        torch.cuda.empty_cache()
     if not args.no_serve:
         try:
             from optixsoltracedemos_amd import serving
             scs = serving.build_serving_scenes(ctx)
-            line["latency"] = serving.quick_latency(ctx, serving_scenes=scs)
+            line["latency"] = serving.run_stream(ctx, n_jobs=2000, rate=200.0, seed=args.seed, serving_scenes=scs)
+            line["serving_load"] = [serving.run_stream(ctx, n_jobs=2000, rate=r, seed=args.seed, serving_scenes=scs)
+                                    for r in (500.0, 1000.0, 2000.0, 5000.0)]
             line["served_vs_plain"] = serving.compare_plain(ctx, n_jobs=400, seed=args.seed, serving_scenes=scs)
         except Exception as ex:  # reported, never silently replaced
             line["latency"] = {"error": f"{type(ex).__name__}: {ex}"}

@@ -232,6 +232,11 @@ int st_scene_update_elements(st_scene scene, const st_element_f64* elements, int
 int st_scene_aim_heliostats(st_scene scene, const double sun_vector[3], const double target[3]);
 /* Set the sun and recompute the sun plane on the GPU (FP64). */
 int st_scene_set_sun(st_scene scene, const st_sun_desc* sun);
+/* st_scene_set_sun + st_scene_aim_heliostats(sun->vector, target) with one
+ * sun-plane and grid build for the new state instead of two: the per-frame
+ * update of lib/soltrace_system.cpp:172-184 when the sun moves and the field
+ * tracks it (demos/demo_dynamic_scene.cpp:82-97). */
+int st_scene_set_sun_and_aim(st_scene scene, const st_sun_desc* sun, const double target[3]);
 int st_scene_get_sun_plane(st_scene scene, st_sun_plane* out);
 /* Host copies of derived FP64 data for inspection and tests. */
 int st_scene_get_frames(st_scene scene, double* l2g_rows /* n*9, row-major L2G */);

@@ -709,19 +709,52 @@ extern "C" int st_scene_aim_heliostats(st_scene s, const double sun_vector[3], c
     return ST_OK;
 }
 
-extern "C" int st_scene_set_sun(st_scene s, const st_sun_desc* sun) {
-    if (!s || !sun) return fail(ST_ERR_INVALID, "null argument");
-    double l = std::sqrt(sun->vector[0] * sun->vector[0] + sun->vector[1] * sun->vector[1] + sun->vector[2] * sun->vector[2]);
+namespace st {
+static int check_sun(const st_sun_desc* sun, double* len) {
+    const double l = std::sqrt(sun->vector[0] * sun->vector[0] + sun->vector[1] * sun->vector[1] +
+                               sun->vector[2] * sun->vector[2]);
     if (!(l > 0) || !std::isfinite(l)) return fail(ST_ERR_INVALID, "sun vector must be finite and non-zero");
     if (sun->shape != ST_SUN_PILLBOX && sun->shape != ST_SUN_GAUSSIAN) return fail(ST_ERR_INVALID, "unknown sun shape %d", sun->shape);
     if (!(sun->param >= 0) || sun->param > 0.2) return fail(ST_ERR_INVALID, "sun angle %g rad outside [0, 0.2]", sun->param);
     if (sun->position_sampling != ST_POS_PHILOX && sun->position_sampling != ST_POS_HALTON &&
         sun->position_sampling != ST_POS_STRATIFIED)
         return fail(ST_ERR_INVALID, "unknown position sampling %d", sun->position_sampling);
+    *len = l;
+    return ST_OK;
+}
+static void store_sun(st_scene s, const st_sun_desc* sun, double l) {
     s->sun = *sun;
     if (!(s->sun.dni > 0)) s->sun.dni = 1000.0;
     for (int k = 0; k < 3; ++k) s->sun_unit[k] = sun->vector[k] / l;
     s->sun_set = true;
+}
+}  // namespace st
+
+extern "C" int st_scene_set_sun_and_aim(st_scene s, const st_sun_desc* sun, const double target[3]) {
+    if (!s || !sun || !target) return fail(ST_ERR_INVALID, "null argument");
+    double l = 0;
+    ST_TRY(check_sun(sun, &l));
+    store_sun(s, sun, l);
+    const double st_in[6] = {s->sun_unit[0], s->sun_unit[1], s->sun_unit[2], target[0], target[1], target[2]};
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    hipStream_t st = s->ctx->stream;
+    HIP_TRY(hipMemcpyAsync(s->d_aim.ptr, st_in, sizeof(st_in), hipMemcpyHostToDevice, st), "upload");
+    HIP_TRY(launch_aim(st, (st_element_f64*)s->d_elems.ptr, s->n, (const double*)s->d_aim.ptr,
+                       (const double*)s->d_aim.ptr + 3),
+            "aim kernel");
+    ST_TRY(scene_refresh(s, false));
+    ST_TRY(scene_sun_plane(s));  // one sun plane and grid, for the new sun and aims
+    HIP_TRY(hipMemcpyAsync(s->host_elems.data(), s->d_elems.ptr, sizeof(st_element_f64) * s->n, hipMemcpyDeviceToHost, st),
+            "download");
+    HIP_TRY(hipStreamSynchronize(st), "aim sync");
+    return ST_OK;
+}
+
+extern "C" int st_scene_set_sun(st_scene s, const st_sun_desc* sun) {
+    if (!s || !sun) return fail(ST_ERR_INVALID, "null argument");
+    double l = 0;
+    ST_TRY(check_sun(sun, &l));
+    store_sun(s, sun, l);
     HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
     return scene_sun_plane(s);
 }

@@ -152,12 +152,14 @@ def run_stream(ctx, n_jobs=2000, rate=200.0, seed=0, hbm_budget_bytes=64 << 30, 
     traced = int(sum(int(rays[i]) for i, _ in ok))
     span = max(r["finish_s"] for _, r in ok) - min(r["submit_s"] for _, r in ok) if ok else float("nan")
     return {
-        "jobs": n_jobs, "completed": len(ok), "rejected": int(st["rejected"]), "errors": len(errors),
+        "jobs": n_jobs, "admitted": int(st["admitted"]), "completed": len(ok), "rejected": int(st["rejected"]),
+        "errors": len(errors),
         "arrival_rate_per_s": rate, "offered_rays_per_s": float(rays.sum() / t[-1]),
         "p50_latency_ms": float(np.percentile(lat, 50) * 1e3) if len(lat) else None,
         "p99_latency_ms": float(np.percentile(lat, 99) * 1e3) if len(lat) else None,
         "mean_latency_ms": float(lat.mean() * 1e3) if len(lat) else None,
         "aggregate_rays_per_s": traced / span if ok else None,
+        "served_vs_offered": (traced / span) / float(rays.sum() / t[-1]) if ok else None,
         "launches": int(st["launches"]), "peak_resident_mb": st["peak_resident_bytes"] / 2**20,
         "gpu_busy_s": st["busy_s"], "span_s": span,
         "sampling": {_abi.POS_PHILOX: "philox", _abi.POS_HALTON: "halton", _abi.POS_STRATIFIED: "stratified"}[sampling],
@@ -174,9 +176,11 @@ def compare_plain(ctx, n_jobs=200, seed=0, flux=(64, 32), max_active_jobs=64, se
     """Aggregate throughput of the served form against the plain per-scene loop
     on the same jobs of the serving mix (all submitted at once: throughput, not
     latency). Plain loop: per job, on a device-resident scene of its kind, set
-    the sun, re-aim the field, trace synchronously and copy the flux map to the
-    host (what a caller of lib/soltrace_system.cpp's update()/run() does per
-    scene state). Both give every job's flux map bitwise the same."""
+    the sun and re-aim the field in one set-up (st_scene_set_sun_and_aim),
+    trace synchronously and copy the flux map to the host (what a caller of
+    lib/soltrace_system.cpp's update()/run() does per scene state); the served
+    timer includes fetching every map to the host too. Both give every job's
+    flux map bitwise the same."""
     import torch
     sc_list = serving_scenes or build_serving_scenes(ctx)
     _, kind, suns, rays = poisson_jobs(n_jobs, seed=seed, weights=weights, rays_hi=rays_hi)
@@ -211,8 +215,7 @@ def compare_plain(ctx, n_jobs=200, seed=0, flux=(64, 32), max_active_jobs=64, se
     plain = []
     for i in range(n_jobs):
         w, ps = plain_scenes[kind[i]]
-        ps.set_sun(suns[i], 4.65e-3, position_sampling=sampling)
-        ps.aim_heliostats(suns[i], w.elements[w.receiver]["origin"])
+        ps.set_sun_and_aim(suns[i], 4.65e-3, w.elements[w.receiver]["origin"], position_sampling=sampling)
         r = ps.trace(int(rays[i]), seed=seed, job_id=i, max_depth=5, flux_nx=flux[0], flux_ny=flux[1])
         plain.append(r.flux.cpu().numpy())
     t_plain = time.perf_counter() - t0

@@ -7,9 +7,12 @@ Two ways to run the same work:
     the chunks of every step, captured in a hipGraph and replayed;
   * plain per-scene loop, the reference's update()/run() cycle
     (lib/soltrace_system.cpp:172-184, demos/demo_dynamic_scene.cpp:82-97):
-    per step re-aim + refit, new sun plane, one synchronous trace, flux copied
-    back to the host.
-Both produce identical per-step results (same Philox keys: seed, job = step).
+    per step one set-up (new sun, re-aim, refit, sun plane and grid:
+    st_scene_set_sun_and_aim), one synchronous trace, flux copied back to the
+    host.
+Both time the same work: the batched timer includes the copy of every step's
+flux map to the host, as the plain loop copies each. Both produce identical
+per-step results (same Philox keys: seed, job = step).
 """
 from __future__ import annotations
 
@@ -30,13 +33,14 @@ def run_batched(ctx, w, suns, budgets, seed=0, flux=(256, 128), use_graph=True, 
     out = sc.trace_batch(suns, budgets, w.extra["target"], w.sun_param, position_sampling=position_sampling, **kw)
     torch.cuda.synchronize()                                   # first call: set-up + graph capture
     times = []
+    host = None
     for _ in range(reps):
         t0 = time.perf_counter()
         out = sc.trace_batch(suns, budgets, w.extra["target"], w.sun_param, position_sampling=position_sampling,
                              out=out, **kw)
-        torch.cuda.synchronize()
+        host = out["flux"].cpu().numpy()                       # every step's map on the host, as the plain loop
         times.append(time.perf_counter() - t0)
-    return min(times), out
+    return min(times), out, host
 
 
 def run_plain_loop(ctx, w, suns, budgets, seed=0, flux=(256, 128), position_sampling=_abi.POS_STRATIFIED):
@@ -44,8 +48,7 @@ def run_plain_loop(ctx, w, suns, budgets, seed=0, flux=(256, 128), position_samp
     flux_maps = []
     t0 = time.perf_counter()
     for k in range(len(budgets)):
-        sc.set_sun(suns[k], w.sun_param, position_sampling=position_sampling)
-        sc.aim_heliostats(suns[k], w.extra["target"])
+        sc.set_sun_and_aim(suns[k], w.sun_param, w.extra["target"], position_sampling=position_sampling)
         r = sc.trace(int(budgets[k]), seed=seed, job_id=k, max_depth=w.max_depth, flux_nx=flux[0], flux_ny=flux[1])
         flux_maps.append(r.flux.cpu().numpy())
     return time.perf_counter() - t0, flux_maps
@@ -55,10 +58,8 @@ def compare(ctx, steps=96, seed=0):
     w = scenes.heliostat_field(10_000)
     suns, budgets = scenes.annual_sweep(steps, seed=seed)
     total = int(budgets.sum())
-    tb, out = run_batched(ctx, w, suns, budgets, seed=seed)
-    fb = out["flux"].cpu().numpy()
-    tw, out_bvh = run_batched(ctx, w, suns, budgets, seed=seed, bvh_only=True)
-    fw = out_bvh["flux"].cpu().numpy()
+    tb, _, fb = run_batched(ctx, w, suns, budgets, seed=seed)
+    tw, _, fw = run_batched(ctx, w, suns, budgets, seed=seed, bvh_only=True)
     tp, plain = run_plain_loop(ctx, w, suns, budgets, seed=seed)
     identical = all(np.array_equal(fb[k], plain[k]) and np.array_equal(fw[k], plain[k]) for k in range(steps))
     return {"steps": steps, "total_rays": total, "batched_s": tb, "batched_bvh_only_s": tw, "plain_loop_s": tp,

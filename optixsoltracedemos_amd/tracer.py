@@ -116,6 +116,13 @@ class Scene:
         carr, n = _abi.elements_to_ctypes(elements)
         call("st_scene_update_elements", self.h, carr, n)
 
+    def set_sun_and_aim(self, vector, param, target, shape=_abi.SUN_PILLBOX, dni=1000.0,
+                        position_sampling=_abi.POS_PHILOX):
+        """set_sun + aim_heliostats(vector, target) with one sun-plane / grid build."""
+        s = _abi.sun_desc(vector, param, shape, dni, position_sampling)
+        t = (C.c_double * 3)(*[float(x) for x in target])
+        call("st_scene_set_sun_and_aim", self.h, C.byref(s), t)
+
     def aim_heliostats(self, sun_vector, target):
         s = (C.c_double * 3)(*[float(x) for x in sun_vector])
         t = (C.c_double * 3)(*[float(x) for x in target])

@@ -494,6 +494,14 @@ def test_update_aim_and_sun(gpu_ctx, oracle_mod):
     os_.set_sun(w1.sun_vector, w1.sun_param)
     o = os_.trace(n, max_depth=5, flux_nx=64, flux_ny=32)
     assert flux_l1(a["flux"], o["flux"]) <= FLUX_L1_MAX
+    # the combined set-up gives the same scene state as the two calls
+    both = tracer.Scene(gpu_ctx, w0.elements)
+    both.set_sun_and_aim(w1.sun_vector, w1.sun_param, w1.extra["target"])
+    assert np.array_equal(both.frames(), sc.frames())
+    assert both.sun_plane()["area"] == sc.sun_plane()["area"]
+    c = both.trace(n, max_depth=5, flux_nx=64, flux_ny=32).numpy()
+    for k in ("flux", "element_hits", "stats"):
+        assert np.array_equal(a[k], c[k]), k
 
 
 def test_frames_match_oracle(gpu_ctx, oracle_mod):
