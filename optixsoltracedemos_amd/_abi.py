@@ -13,7 +13,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(HERE, "_lib")
-LIB_PATH = os.path.join(LIB_DIR, "libsoltrace_hip.so")
+# ST_HIP_LIB selects another build of the library (A/B timing of two builds)
+LIB_PATH = os.environ.get("ST_HIP_LIB") or os.path.join(LIB_DIR, "libsoltrace_hip.so")
 
 ST_OK = 0
 ST_ERR_INVALID = -1

@@ -209,15 +209,17 @@ def compare_plain(ctx, n_jobs=200, seed=0, flux=(64, 32), max_active_jobs=64, se
     for _, w, _ in sc_list:
         ps = tracer.Scene(ctx, w.elements)
         ps.set_sun(w.sun_vector, w.sun_param)
-        plain_scenes.append((w, ps))
+        res = ps.alloc_outputs(ps.make_desc(1, max_depth=5, flux_nx=flux[0], flux_ny=flux[1]))  # reused per job
+        plain_scenes.append((w, ps, res))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     plain = []
     for i in range(n_jobs):
-        w, ps = plain_scenes[kind[i]]
+        w, ps, res = plain_scenes[kind[i]]
         ps.set_sun_and_aim(suns[i], 4.65e-3, w.elements[w.receiver]["origin"], position_sampling=sampling)
-        r = ps.trace(int(rays[i]), seed=seed, job_id=i, max_depth=5, flux_nx=flux[0], flux_ny=flux[1])
-        plain.append(r.flux.cpu().numpy())
+        ps.trace_into(ps.make_desc(int(rays[i]), seed=seed, job_id=i, max_depth=5, flux_nx=flux[0], flux_ny=flux[1],
+                                   zero=True), res)
+        plain.append(res.flux.cpu().numpy())
     t_plain = time.perf_counter() - t0
     same = all(np.array_equal(a, b) for a, b in zip(served, plain))
     total = int(rays.sum())

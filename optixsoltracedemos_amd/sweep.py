@@ -32,26 +32,33 @@ def run_batched(ctx, w, suns, budgets, seed=0, flux=(256, 128), use_graph=True, 
     sc.set_sun(suns[0], w.sun_param, position_sampling=position_sampling)
     out = sc.trace_batch(suns, budgets, w.extra["target"], w.sun_param, position_sampling=position_sampling, **kw)
     torch.cuda.synchronize()                                   # first call: set-up + graph capture
+    host = torch.empty(out["flux"].shape, dtype=out["flux"].dtype, pin_memory=True)
     times = []
-    host = None
     for _ in range(reps):
         t0 = time.perf_counter()
         out = sc.trace_batch(suns, budgets, w.extra["target"], w.sun_param, position_sampling=position_sampling,
                              out=out, **kw)
-        host = out["flux"].cpu().numpy()                       # every step's map on the host, as the plain loop
+        host.copy_(out["flux"])                                # every step's map on the host, as the plain loop
         times.append(time.perf_counter() - t0)
-    return min(times), out, host
+    return min(times), out, host.numpy().copy()
 
 
 def run_plain_loop(ctx, w, suns, budgets, seed=0, flux=(256, 128), position_sampling=_abi.POS_STRATIFIED):
+    import torch
     sc = tracer.Scene(ctx, w.elements)
-    flux_maps = []
+    K = len(budgets)
+    host = torch.empty((K, flux[1], flux[0]), dtype=torch.int64, pin_memory=True)
+    desc = sc.make_desc(1, max_depth=w.max_depth, flux_nx=flux[0], flux_ny=flux[1])
+    res = sc.alloc_outputs(desc)                               # device outputs reused by every step
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(len(budgets)):
+    for k in range(K):
         sc.set_sun_and_aim(suns[k], w.sun_param, w.extra["target"], position_sampling=position_sampling)
-        r = sc.trace(int(budgets[k]), seed=seed, job_id=k, max_depth=w.max_depth, flux_nx=flux[0], flux_ny=flux[1])
-        flux_maps.append(r.flux.cpu().numpy())
-    return time.perf_counter() - t0, flux_maps
+        d = sc.make_desc(int(budgets[k]), seed=seed, job_id=k, max_depth=w.max_depth, flux_nx=flux[0],
+                         flux_ny=flux[1], zero=True)
+        sc.trace_into(d, res)
+        host[k].copy_(res.flux)
+    return time.perf_counter() - t0, host.numpy().copy()
 
 
 def compare(ctx, steps=96, seed=0):
