@@ -608,6 +608,46 @@ __device__ __forceinline__ void log_pending(uint32_t* log, uint32_t& len, const 
     }
 }
 
+// A ray's whole record at once, where the ray starts: slot 0 = its sun-plane
+// origin, every later slot empty (all zero, element 0xFFFFFFFF); hits overwrite
+// their slots later, instead of each slot taking a partial cache line as the
+// ray advances. A full wave of 64 consecutive rays owns one contiguous block
+// of 64 * max_depth slots and writes it lane by lane in order (each store
+// instruction covers 1 KiB of records, 256 B of element ids), the origins
+// fetched from their lanes; other waves write ray by ray.
+__device__ __forceinline__ void record_start(const JobDev& J, uint64_t ray, V3 o) {
+    if (!J.records) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t r0 = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(ray >> 32)) << 32) |
+                        (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)ray);
+    if (__ballot(ray == r0 + lane) == ~0ull) {
+        const uint32_t D = (uint32_t)J.max_depth;
+        const uint64_t base = (r0 - J.record_base) * (uint64_t)D;
+        // slot idx = lane + 64 j of the block belongs to ray r = idx / D, slot s = idx % D
+        const uint32_t dr = 64u / D, ds = 64u % D;
+        uint32_t r = lane / D, s = lane - r * D;
+        for (uint32_t j = 0; j < D; ++j) {
+            const float x = __shfl(o.x, (int)r), y = __shfl(o.y, (int)r), z = __shfl(o.z, (int)r);
+            st4(J.records + base + lane + 64u * j, s == 0 ? make_float4(0.f, x, y, z) : make_float4(0.f, 0.f, 0.f, 0.f));
+            if (J.rec_elem) st1(J.rec_elem + base + lane + 64u * j, s == 0 ? 0xFFFFFFFEu : 0xFFFFFFFFu);
+            r += dr;
+            s += ds;
+            if (s >= D) {
+                s -= D;
+                ++r;
+            }
+        }
+        return;
+    }
+    const uint64_t i = (ray - J.record_base) * (uint64_t)J.max_depth;
+    st4(J.records + i, make_float4(0.f, o.x, o.y, o.z));
+    for (int k = 1; k < J.max_depth; ++k) st4(J.records + i + k, make_float4(0.f, 0.f, 0.f, 0.f));
+    if (J.rec_elem) {
+        st1(J.rec_elem + i, 0xFFFFFFFEu);
+        for (int k = 1; k < J.max_depth; ++k) st1(J.rec_elem + i + k, 0xFFFFFFFFu);
+    }
+}
+
 __device__ __forceinline__ void record(const JobDev& J, uint64_t ray, int slot, float tag, V3 p, uint32_t elem) {
     if (J.records) {
         const uint64_t i = (ray - J.record_base) * (uint64_t)J.max_depth + (uint64_t)slot;
@@ -661,7 +701,7 @@ __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame
         }
         d = normalize(fma3(ct, w, st * fma3(cp, su, sp * sv)));
     }
-    record(J, ray, 0, 0.f, o, 0xFFFFFFFEu);
+    record_start(J, ray, o);
 }
 
 // One segment of a ray: closest hit from o along d, then the hit element's
@@ -826,10 +866,8 @@ __device__ __forceinline__ uint64_t ray_of_slot(const JobDev& J, uint64_t s) {
 }
 
 // empty slots are all-zero (the reference memsets the buffer, lib/soltrace_system.cpp:125)
-__device__ __forceinline__ void finish_ray(const JobDev& J, uint64_t ray, int last) {
-    if (J.records)
-        for (int k = last + 1; k < J.max_depth; ++k) record(J, ray, k, 0.f, v3(0.f, 0.f, 0.f), 0xFFFFFFFFu);
-}
+// slots after the ray's last recorded one stay as record_start left them (empty)
+__device__ __forceinline__ void finish_ray(const JobDev&, uint64_t, int) {}
 
 // work_counter[0] hands out chunks, work_counter[1] counts waves that ran out
 // of work; the last wave to leave resets both to zero, so the next launch on
