@@ -227,6 +227,10 @@ int st_scene_num_elements(st_scene scene, int32_t* n);
 int st_scene_update_aim(st_scene scene, const double* aim_points, const double* zrot_deg);
 /* Replace the element set (same count and types) and refit. */
 int st_scene_update_elements(st_scene scene, const st_element_f64* elements, int32_t n);
+/* New element poses (same count and types; NULL keeps them) and a new sun
+ * (NULL keeps it) in one set-up: frames, refit, then one sun plane and grid.
+ * The per-frame SolTraceSystem::update() (lib/soltrace_system.cpp:172-184). */
+int st_scene_update(st_scene scene, const st_element_f64* elements, int32_t n, const st_sun_desc* sun);
 /* Re-aim every mirror at `target` along the bisector of the sun vector and the
  * direction to target (FP64, on the GPU), then refit. */
 int st_scene_aim_heliostats(st_scene scene, const double sun_vector[3], const double target[3]);

@@ -219,6 +219,7 @@ SIGNATURES = {
     "st_scene_num_elements": (C.c_int, [P, C.POINTER(C.c_int32)]),
     "st_scene_update_aim": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "st_scene_update_elements": (C.c_int, [P, C.POINTER(Element), C.c_int32]),
+    "st_scene_update": (C.c_int, [P, C.POINTER(Element), C.c_int32, C.POINTER(SunDesc)]),
     "st_scene_aim_heliostats": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "st_scene_set_sun": (C.c_int, [P, C.POINTER(SunDesc)]),
     "st_scene_set_sun_and_aim": (C.c_int, [P, C.POINTER(SunDesc), C.POINTER(C.c_double)]),

@@ -689,26 +689,6 @@ extern "C" int st_scene_update_elements(st_scene s, const st_element_f64* elemen
     return ST_OK;
 }
 
-extern "C" int st_scene_aim_heliostats(st_scene s, const double sun_vector[3], const double target[3]) {
-    if (!s || !sun_vector || !target) return fail(ST_ERR_INVALID, "null argument");
-    double l = std::sqrt(sun_vector[0] * sun_vector[0] + sun_vector[1] * sun_vector[1] + sun_vector[2] * sun_vector[2]);
-    if (!(l > 0)) return fail(ST_ERR_INVALID, "zero sun vector");
-    double st_in[6] = {sun_vector[0] / l, sun_vector[1] / l, sun_vector[2] / l, target[0], target[1], target[2]};
-    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
-    hipStream_t st = s->ctx->stream;
-    // sun unit vector and target share the scene's aim scratch (6 doubles)
-    HIP_TRY(hipMemcpyAsync(s->d_aim.ptr, st_in, sizeof(st_in), hipMemcpyHostToDevice, st), "upload");
-    HIP_TRY(launch_aim(st, (st_element_f64*)s->d_elems.ptr, s->n, (const double*)s->d_aim.ptr,
-                       (const double*)s->d_aim.ptr + 3),
-            "aim kernel");
-    ST_TRY(scene_refresh(s, false));
-    ST_TRY(scene_sun_plane(s));  // synchronizes the stream
-    HIP_TRY(hipMemcpyAsync(s->host_elems.data(), s->d_elems.ptr, sizeof(st_element_f64) * s->n, hipMemcpyDeviceToHost, st),
-            "download");
-    HIP_TRY(hipStreamSynchronize(st), "aim sync");
-    return ST_OK;
-}
-
 namespace st {
 static int check_sun(const st_sun_desc* sun, double* len) {
     const double l = std::sqrt(sun->vector[0] * sun->vector[0] + sun->vector[1] * sun->vector[1] +
@@ -729,6 +709,53 @@ static void store_sun(st_scene s, const st_sun_desc* sun, double l) {
     s->sun_set = true;
 }
 }  // namespace st
+
+extern "C" int st_scene_update(st_scene s, const st_element_f64* elements, int32_t n, const st_sun_desc* sun) {
+    if (!s) return fail(ST_ERR_INVALID, "null scene");
+    if (!elements && !sun) return fail(ST_ERR_INVALID, "nothing to update");
+    if (elements) {
+        if (n != s->n) return fail(ST_ERR_INVALID, "element count changed (%d -> %d); create a new scene", s->n, n);
+        ST_TRY(validate_elements(elements, n));
+        for (int32_t i = 0; i < n; ++i)
+            if (elements[i].surface != s->host_elems[i].surface || elements[i].role != s->host_elems[i].role)
+                return fail(ST_ERR_INVALID, "element %d changed type or role; create a new scene", i);
+    }
+    double l = 0;
+    if (sun) ST_TRY(check_sun(sun, &l));
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    hipStream_t st = s->ctx->stream;
+    if (elements) {
+        s->host_elems.assign(elements, elements + n);
+        HIP_TRY(hipMemcpyAsync(s->d_elems.ptr, elements, sizeof(st_element_f64) * n, hipMemcpyHostToDevice, st),
+                "upload");
+        ST_TRY(scene_refresh(s, false));
+    }
+    if (sun) store_sun(s, sun, l);
+    ST_TRY(scene_sun_plane(s));  // once, for the new poses and sun
+    HIP_TRY(hipStreamSynchronize(st), "update sync");
+    return ST_OK;
+}
+
+extern "C" int st_scene_aim_heliostats(st_scene s, const double sun_vector[3], const double target[3]) {
+    if (!s || !sun_vector || !target) return fail(ST_ERR_INVALID, "null argument");
+    double l = std::sqrt(sun_vector[0] * sun_vector[0] + sun_vector[1] * sun_vector[1] + sun_vector[2] * sun_vector[2]);
+    if (!(l > 0)) return fail(ST_ERR_INVALID, "zero sun vector");
+    double st_in[6] = {sun_vector[0] / l, sun_vector[1] / l, sun_vector[2] / l, target[0], target[1], target[2]};
+    HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
+    hipStream_t st = s->ctx->stream;
+    // sun unit vector and target share the scene's aim scratch (6 doubles)
+    HIP_TRY(hipMemcpyAsync(s->d_aim.ptr, st_in, sizeof(st_in), hipMemcpyHostToDevice, st), "upload");
+    HIP_TRY(launch_aim(st, (st_element_f64*)s->d_elems.ptr, s->n, (const double*)s->d_aim.ptr,
+                       (const double*)s->d_aim.ptr + 3),
+            "aim kernel");
+    ST_TRY(scene_refresh(s, false));
+    ST_TRY(scene_sun_plane(s));  // synchronizes the stream
+    HIP_TRY(hipMemcpyAsync(s->host_elems.data(), s->d_elems.ptr, sizeof(st_element_f64) * s->n, hipMemcpyDeviceToHost, st),
+            "download");
+    HIP_TRY(hipStreamSynchronize(st), "aim sync");
+    return ST_OK;
+}
+
 
 extern "C" int st_scene_set_sun_and_aim(st_scene s, const st_sun_desc* sun, const double target[3]) {
     if (!s || !sun || !target) return fail(ST_ERR_INVALID, "null argument");

@@ -140,11 +140,10 @@ void SolTraceSystem::initialize() {
     check(st_buffer_alloc(m_ctx, elems.size() * sizeof(uint64_t), (void**)&m_hits), "element hits");
     check(st_buffer_alloc(m_ctx, ST_STAT_COUNT * sizeof(uint64_t), (void**)&m_stats), "stats");
 
-    std::vector<GeometryDataST> geo;
-    for (const auto& e : m_element_list) geo.push_back(e->toDeviceGeometryData());
-    data_manager->allocateGeometryDataArray(geo);
-    data_manager->allocateLaunchParams();
-    data_manager->updateLaunchParams();
+    // launch_params_H is kept current on the host. The reference uploads it and
+    // a GeometryDataST array for its OptiX programs (lib/soltrace_system.cpp:
+    // 130-143); no kernel here reads them, so no device copies are made
+    // (dataManager still uploads them for a caller that uses it directly).
     if (m_verbose) print_launch_params();
     m_timer_setup.stop();
 }
@@ -155,6 +154,7 @@ void SolTraceSystem::run() {
     std::memset(&d, 0, sizeof(d));
     d.num_rays = (uint64_t)m_num_sunpoints;
     d.seed = m_seed;
+    d.job_id = m_job_id;
     d.max_depth = m_max_depth;
     d.flags = ST_TRACE_ZERO_OUTPUTS | (m_record_hits ? ST_TRACE_RECORDS : 0u) |
               (m_flux_direct ? ST_TRACE_FLUX_DIRECT_SUN : 0u);
@@ -178,7 +178,6 @@ void SolTraceSystem::run() {
 void SolTraceSystem::update() {
     if (!m_scene) throw std::runtime_error("SolTraceSystem::update before initialize");
     const std::vector<st_element_f64> elems = collect_elements();
-    check(st_scene_update_elements(m_scene, elems.data(), (int32_t)elems.size()), "st_scene_update_elements");
     st_sun_desc sun;
     std::memset(&sun, 0, sizeof(sun));
     const Vector3d s = m_sun_vector.normalized();
@@ -187,7 +186,8 @@ void SolTraceSystem::update() {
     sun.dni = m_dni;
     sun.shape = m_sun_shape;
     sun.position_sampling = m_position_sampling;
-    check(st_scene_set_sun(m_scene, &sun), "st_scene_set_sun");
+    // new poses and the sun in one set-up (one sun plane and grid)
+    check(st_scene_update(m_scene, elems.data(), (int32_t)elems.size(), &sun), "st_scene_update");
     soltrace::LaunchParams& p = data_manager->launch_params_H;
     p.sun_vector = mathUtil::toFloat3(s);
     p.max_sun_angle = (float)m_sun_angle;
@@ -196,10 +196,6 @@ void SolTraceSystem::update() {
     p.sun_v1 = make_float3((float)pl.v1[0], (float)pl.v1[1], (float)pl.v1[2]);
     p.sun_v2 = make_float3((float)pl.v2[0], (float)pl.v2[1], (float)pl.v2[2]);
     p.sun_v3 = make_float3((float)pl.v3[0], (float)pl.v3[1], (float)pl.v3[2]);
-    std::vector<GeometryDataST> geo;
-    for (const auto& e : m_element_list) geo.push_back(e->toDeviceGeometryData());
-    data_manager->updateGeometryDataArray(geo);
-    data_manager->updateLaunchParams();
 }
 
 void SolTraceSystem::clean_up() {

@@ -46,6 +46,7 @@ public:
     void set_sun_shape(int st_sun_shape) { m_sun_shape = st_sun_shape; }        // ST_SUN_PILLBOX / GAUSSIAN
     void set_position_sampling(int mode) { m_position_sampling = mode; }        // ST_POS_HALTON (default) / PHILOX
     void set_seed(uint64_t seed) { m_seed = seed; }
+    void set_job_id(uint32_t job) { m_job_id = job; }                           // Philox stream of run()
     void set_max_depth(int d) { m_max_depth = d; }
     void set_dni(double dni) { m_dni = dni; }
     void set_flux_grid(int nx, int ny) { m_flux_nx = nx; m_flux_ny = ny; }     // bins on the receiver
@@ -81,6 +82,7 @@ private:
     int m_sun_shape = ST_SUN_PILLBOX;
     int m_position_sampling = ST_POS_HALTON;
     uint64_t m_seed = 0;
+    uint32_t m_job_id = 0;
     int m_max_depth = 5;
     double m_dni = 1000.0;
     int m_flux_nx = 0, m_flux_ny = 0;

@@ -61,6 +61,29 @@ def run_plain_loop(ctx, w, suns, budgets, seed=0, flux=(256, 128), position_samp
     return time.perf_counter() - t0, host.numpy().copy()
 
 
+def run_dropin_loop(w, suns, budgets, flux=(256, 128)):
+    """The reference's frame loop through the C++ drop-in API (demos/demo_sweep.cpp:
+    per step host FP64 re-aim of every heliostat, SolTraceSystem::update(), run(),
+    flux map read back), in its own process. Returns (loop seconds, maps)."""
+    import os
+    import subprocess
+    import tempfile
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "demo_sweep")
+    with tempfile.TemporaryDirectory() as td:
+        paths = [os.path.join(td, f) for f in ("elements.bin", "suns.bin", "budgets.bin", "flux.bin")]
+        np.ascontiguousarray(w.elements).tofile(paths[0])
+        np.ascontiguousarray(suns, dtype=np.float64).tofile(paths[1])
+        np.ascontiguousarray(budgets, dtype=np.uint64).tofile(paths[2])
+        r = subprocess.run([exe, *paths, str(flux[0]), str(flux[1])], stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"demo_sweep failed ({r.returncode}): {r.stdout[-2000:]}")
+        t = [x.strip() for x in r.stdout.strip().splitlines()[-1].split(",")]
+        loop_s = float(t[t.index("loop_s") + 1])
+        maps = np.fromfile(paths[3], dtype=np.int64).reshape(len(budgets), flux[1], flux[0])
+    return loop_s, maps
+
+
 def compare(ctx, steps=96, seed=0):
     w = scenes.heliostat_field(10_000)
     suns, budgets = scenes.annual_sweep(steps, seed=seed)
@@ -69,6 +92,12 @@ def compare(ctx, steps=96, seed=0):
     tw, _, fw = run_batched(ctx, w, suns, budgets, seed=seed, bvh_only=True)
     tp, plain = run_plain_loop(ctx, w, suns, budgets, seed=seed)
     identical = all(np.array_equal(fb[k], plain[k]) and np.array_equal(fw[k], plain[k]) for k in range(steps))
-    return {"steps": steps, "total_rays": total, "batched_s": tb, "batched_bvh_only_s": tw, "plain_loop_s": tp,
-            "batched_rays_per_s": total / tb, "plain_rays_per_s": total / tp, "speedup": tp / tb,
-            "identical_flux": bool(identical)}
+    out = {"steps": steps, "total_rays": total, "batched_s": tb, "batched_bvh_only_s": tw, "plain_loop_s": tp,
+           "batched_rays_per_s": total / tb, "plain_rays_per_s": total / tp, "speedup": tp / tb,
+           "identical_flux": bool(identical)}
+    if seed == 0:
+        td, dm = run_dropin_loop(w, suns, budgets)
+        diff = float(np.abs(dm - fb).sum()) / max(float(fb.sum()), 1.0)
+        out["dropin_frame_loop"] = {"loop_s": td, "rays_per_s": total / td, "batched_speedup": td / tb,
+                                    "flux_l1_vs_batched": diff}
+    return out

@@ -123,6 +123,13 @@ class Scene:
         t = (C.c_double * 3)(*[float(x) for x in target])
         call("st_scene_set_sun_and_aim", self.h, C.byref(s), t)
 
+    def update(self, elements: np.ndarray | None, vector=None, param=4.65e-3, shape=_abi.SUN_PILLBOX, dni=1000.0,
+               position_sampling=_abi.POS_PHILOX):
+        """st_scene_update: new element poses and/or a new sun in one set-up."""
+        carr, n = _abi.elements_to_ctypes(elements) if elements is not None else (None, 0)
+        s = _abi.sun_desc(vector, param, shape, dni, position_sampling) if vector is not None else None
+        call("st_scene_update", self.h, carr, n, C.byref(s) if s is not None else None)
+
     def aim_heliostats(self, sun_vector, target):
         s = (C.c_double * 3)(*[float(x) for x in sun_vector])
         t = (C.c_double * 3)(*[float(x) for x in target])

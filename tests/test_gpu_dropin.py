@@ -159,3 +159,26 @@ def test_dynamic_scene_demo_vs_oracle(oracle_mod):
         o = s.trace(n, max_depth=5)
         assert abs(int(row["mirror_hits"]) - int(o["stats"][2])) <= 2, f
         assert abs(int(row["receiver_hits"]) - int(o["stats"][3])) <= 2, f
+
+
+def test_dropin_frame_loop_matches_the_batched_sweep(gpu_ctx):
+    """The reference's frame loop through the drop-in API (demo_sweep: host FP64
+    re-aim, SolTraceSystem::update(), run() per step, with the step's Philox
+    stream) gives the batched sweep's flux maps: same rays, same re-aim formula
+    (FP64 on the host vs the device may differ in the last bits of an aim point,
+    so at most a few rays may change bins)."""
+    import torch
+
+    from optixsoltracedemos_amd import sweep, tracer
+    w = scenes.heliostat_field(1000)
+    suns, budgets = scenes.annual_sweep(4, seed=0)
+    budgets = (budgets // 8).astype(np.uint64)
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sc.set_sun(suns[0], w.sun_param, position_sampling=_abi.POS_STRATIFIED)
+    out = sc.trace_batch(suns, budgets, w.extra["target"], w.sun_param, seed=0, max_depth=5, flux_nx=64, flux_ny=32,
+                         position_sampling=_abi.POS_STRATIFIED)
+    torch.cuda.synchronize()
+    fb = out["flux"].cpu().numpy()
+    _, fd = sweep.run_dropin_loop(w, suns, budgets, flux=(64, 32))
+    assert fd.shape == fb.shape and fb.sum() > 10_000
+    assert np.abs(fd - fb).sum() <= 1e-4 * fb.sum(), np.abs(fd - fb).sum()

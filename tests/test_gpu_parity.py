@@ -494,6 +494,16 @@ def test_update_aim_and_sun(gpu_ctx, oracle_mod):
     os_.set_sun(w1.sun_vector, w1.sun_param)
     o = os_.trace(n, max_depth=5, flux_nx=64, flux_ny=32)
     assert flux_l1(a["flux"], o["flux"]) <= FLUX_L1_MAX
+    # st_scene_update (new poses and sun in one set-up) gives the same state
+    upd = tracer.Scene(gpu_ctx, w0.elements)
+    upd.set_sun(w0.sun_vector, w0.sun_param)
+    upd.update(w1.elements, w1.sun_vector, w1.sun_param)
+    assert np.array_equal(upd.frames(), fresh.frames())
+    assert upd.sun_plane()["area"] == fresh.sun_plane()["area"]
+    e = upd.trace(n, max_depth=5, flux_nx=64, flux_ny=32).numpy()
+    f = fresh.trace(n, max_depth=5, flux_nx=64, flux_ny=32).numpy()
+    for k in ("flux", "element_hits", "stats"):
+        assert np.array_equal(e[k], f[k]), k
     # the combined set-up gives the same scene state as the two calls
     both = tracer.Scene(gpu_ctx, w0.elements)
     both.set_sun_and_aim(w1.sun_vector, w1.sun_param, w1.extra["target"])
