@@ -705,15 +705,18 @@ __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame
 }
 
 // One segment of a ray: closest hit from o along d, then the hit element's
-// program. Returns true when the ray continues (o, d, tmin, depth updated for
-// the next segment), false when it ended.
+// program: a receiver's here; a mirror's (shade_mirror) is left to the caller,
+// to run before the next segment. Returns true when the ray hit a mirror and
+// continues (o = hit point, tmin, depth and key updated for the next segment,
+// pend = the mirror's primitive for shade_mirror, d unchanged), false when it
+// ended.
 // DEFER: element hits and flux bins are left in ph / pf for log_pending
 // instead of being logged here.
 template <int STRIDE, bool DEFER>
 __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int32_t* stk, uint32_t* wc,
                                               uint32_t* wh, uint32_t* wl, V3& o, V3& d, float& tmin, int& depth,
-                                              int& last, SegList L, int& key, uint32_t& ph, uint32_t& pf) {
-    const uint32_t rlo = (uint32_t)ray, rhi = (uint32_t)(ray >> 32);
+                                              int& last, SegList L, int& key, int& pend, uint32_t& ph,
+                                              uint32_t& pf) {
     float t = kTmax;
     int slot = -1, cap = 0;
     uint32_t bk = 0xFFFFFFFFu;
@@ -731,11 +734,11 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
     const int nd = depth + 1;
     const uint32_t m = ld1(J.meta + slot);
     const uint32_t kind = meta_kind(m), elem = meta_elem(m);
-    const float4 q0 = ld4(J.prims + 4 * slot + 0), q1 = ld4(J.prims + 4 * slot + 1),
-                 q2 = ld4(J.prims + 4 * slot + 2), q3 = ld4(J.prims + 4 * slot + 3);
     const V3 p = fma3(t, d, o);
-    const V3 r = o - v3(q0);
     if (meta_receiver(m)) {
+        const float4 q0 = ld4(J.prims + 4 * slot + 0), q1 = ld4(J.prims + 4 * slot + 1),
+                     q2 = ld4(J.prims + 4 * slot + 2), q3 = ld4(J.prims + 4 * slot + 3);
+        const V3 r = o - v3(q0);
         // cuda/materials.cu:86-121 (flat: front face only) and :123-163 (cylinder)
         if (kind == kKindFlat && !(dot(d, v3(q3)) < 0.f)) {
             count(wc, ST_STAT_BACK_FACE);
@@ -804,6 +807,33 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
         else add_one_grouped<64>(elem, [&](uint32_t e, uint32_t n) { log_add(wl, J, false, e, n); });
     }
     count(wc, ST_STAT_MIRROR_HITS);
+    // the mirror's program runs as the first step of the ray's next segment
+    // (shade_mirror): d stays the incoming direction until then
+    o = p;
+    tmin = kTminBounce;
+    pend = slot | (cap ? (int)0x80000000u : 0);
+    // key of the next segment's list: a first reflection's list depends on its
+    // mirror only; a second reflection's on where it sits in that list
+    key = nd == 1 ? (int)elem : (nd == 2 && listed) ? (int)bk : -1;
+    depth = nd;
+    return true;
+}
+
+// The program of a mirror hit (cuda/materials.cu:26-84, 170-231) of a ray at
+// its hit point o with incoming direction d on primitive pend & 0x7FFFFFFF
+// (sign bit: an end-cap hit), bounce depth nd: reflectivity draw, surface
+// normal, slope error on the normal, reflection, specular error on the
+// reflected ray. Returns false when the ray is absorbed; otherwise d is the
+// reflected direction. The queued forms run it at the start of the ray's next
+// segment, where the wave holds only continuing rays, instead of in the step
+// that found the hit, where the lanes whose ray ended would sit idle through it.
+__device__ __forceinline__ bool shade_mirror(const JobDev& J, uint64_t ray, int pend, int nd, V3 o, V3& d,
+                                             uint32_t* wc) {
+    const uint32_t rlo = (uint32_t)ray, rhi = (uint32_t)(ray >> 32);
+    const int slot = pend & 0x7FFFFFFF;
+    const bool cap = pend < 0;
+    const uint32_t m = ld1(J.meta + slot);
+    const uint32_t kind = meta_kind(m), elem = meta_elem(m);
     // draws of bounce nd: block 2nd-1 = (reflectivity, slope xr, slope xp,
     // specular xr), block 2nd = (specular xp, -, -, -), each drawn only when
     // one of its optics terms is active
@@ -817,15 +847,18 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
     }
     V3 n;
     if (kind == kKindFlat) {
-        n = v3(q3);
+        n = v3(ld4(J.prims + 4 * slot + 3));
     } else {
+        const float4 q0 = ld4(J.prims + 4 * slot + 0), q1 = ld4(J.prims + 4 * slot + 1),
+                     q2 = ld4(J.prims + 4 * slot + 2), q3 = ld4(J.prims + 4 * slot + 3);
         const V3 ax = v3(q1), ay = v3(q2), az = v3(q3);
-        const float lx = fmaf(t, dot(d, ax), dot(r, ax)), ly = fmaf(t, dot(d, ay), dot(r, ay));
+        const V3 r = o - v3(q0);
+        const float lx = dot(r, ax), ly = dot(r, ay);
         V3 nl;
         if (kind == kKindParabolic) {
             nl = normalize(v3(-q2.w * lx, -q3.w * ly, 1.f));
         } else {
-            const float lz = fmaf(t, dot(d, az), dot(r, az));
+            const float lz = dot(r, az);
             nl = cap ? v3(0.f, ly < 0.f ? -1.f : 1.f, 0.f) : normalize(v3(lx, 0.f, lz));
         }
         n = normalize(fma3(nl.x, ax, fma3(nl.y, ay, nl.z * az)));
@@ -837,13 +870,7 @@ __device__ __forceinline__ bool trace_segment(const JobDev& J, uint64_t ray, int
         const U4 us = philox4x32_10(U4{rlo, rhi, 2u * (uint32_t)nd, J.job_id}, J.seed_lo, J.seed_hi);
         rd = perturb(rd, op.z, u01(ue.w), u01(us.x));
     }
-    o = p;
     d = rd;
-    tmin = kTminBounce;
-    // key of the next segment's list: a first reflection's list depends on its
-    // mirror only; a second reflection's on where it sits in that list
-    key = nd == 1 ? (int)elem : (nd == 2 && listed) ? (int)bk : -1;
-    depth = nd;
     return true;
 }
 
@@ -879,8 +906,10 @@ __device__ __forceinline__ void finish_ray(const JobDev&, uint64_t, int) {}
 // wave moves to the next 64 rays when its slowest ray ends, so after the first
 // segment the lanes whose ray ended sit idle. QUEUED = true: sun rays are traced
 // 64 at a time; a ray that continues after a mirror is parked in the wave's LDS
-// queue (hit point, reflected direction, ray offset, depth) and the queue is
-// traced in full waves of 64 continuing rays (the rest at the chunk's end).
+// queue (hit point, incoming direction, ray offset, depth, list key, mirror)
+// and the queue is traced in full waves of 64 continuing rays (the rest at the
+// chunk's end), each step running the mirror's program (shade_mirror) and then
+// the segment it starts.
 // Both give identical results: every per-ray quantity depends on the ray index
 // and its own path only, and all accumulations are integer sums.
 constexpr int kQueueCap = 64;  // < 64 parked between steps (lanes refill from the queue first)
@@ -903,7 +932,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                                                        int32_t chunk_rays, int32_t static_chunks, JobDev single) {
     __shared__ int32_t lds_stack[STACK * BLOCK];
     __shared__ uint32_t lds_counters[BLOCK / 64][kNumCounters];
-    __shared__ float lds_queue[QUEUED ? BLOCK / 64 : 1][QUEUED ? 7 * kQueueCap : 1];
+    __shared__ float lds_queue[QUEUED ? BLOCK / 64 : 1][QUEUED ? 8 * kQueueCap : 1];
     __shared__ uint32_t lds_log[BLOCK / 64][1 + kLogCap];
     __shared__ uint32_t lds_hits[LHITS > 0 ? BLOCK / 64 : 1][LHITS > 0 ? LHITS : 1];
     int32_t* stk = lds_stack + threadIdx.x;
@@ -934,6 +963,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
             V3 o = v3(0.f, 0.f, 0.f), d = o;
             uint32_t off = 0;    // its offset in the chunk
             int depth = 0, key = -1;
+            int pend = 0;        // the mirror it hit last, for shade_mirror
             // lanes without a ray take parked ones (up to `want` in total)
             auto refill = [&](uint32_t want) {
                 const unsigned long long dead = __ballot(!alive);
@@ -948,6 +978,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                         off = tag & 0x7FFu;
                         depth = (int)((tag >> 11) & 15u);
                         key = (int)(tag >> 15) - 1;
+                        pend = (int)__float_as_uint(q[7 * kQueueCap + i]);
                         alive = true;
                     }
                 }
@@ -976,8 +1007,9 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                         const uint64_t ray = ray_of_slot(J, first + off);
                         int last = depth;
                         float tmin = kTminBounce;
-                        alive = trace_segment<BLOCK, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last, L, key,
-                                                           ph, pf);
+                        alive = shade_mirror(J, ray, pend, depth, o, d, wc) &&
+                                trace_segment<BLOCK, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last, L, key,
+                                                           pend, ph, pf);
                         if (!alive) finish_ray(J, ray, last);
                     }
                 } else {
@@ -985,8 +1017,9 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                         const uint64_t ray = ray_of_slot(J, first + off);
                         int last = depth;
                         float tmin = kTminBounce;
-                        alive = trace_segment<BLOCK, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
-                                                           next_list(J, depth, key), key, ph, pf);
+                        alive = shade_mirror(J, ray, pend, depth, o, d, wc) &&
+                                trace_segment<BLOCK, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
+                                                           next_list(J, depth, key), key, pend, ph, pf);
                         if (!alive) finish_ray(J, ray, last);
                     }
                 }
@@ -1007,7 +1040,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                     key = -1;
                     off = (uint32_t)(i - first);
                     alive = trace_segment<BLOCK, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
-                                                       cell_list(J, cell), key, ph, pf);
+                                                       cell_list(J, cell), key, pend, ph, pf);
                     if (!alive) finish_ray(J, ray, last);
                 }
                 log_pending(wl, len, J, ph, pf);
@@ -1029,6 +1062,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                     q[4 * kQueueCap + i2] = d.y;
                     q[5 * kQueueCap + i2] = d.z;
                     q[6 * kQueueCap + i2] = __uint_as_float(queue_tag(off, depth, key));
+                    q[7 * kQueueCap + i2] = __uint_as_float((uint32_t)pend);
                 }
                 qn += (uint32_t)__popcll(m);
                 alive = false;
@@ -1051,12 +1085,13 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                 uint32_t cell;
                 sun_ray(J, S, ray, wc, o, d, cell);
                 float tmin = kTminPrimary;
-                int depth = 0, last = 0, key = -1;
+                int depth = 0, last = 0, key = -1, pend = 0;
                 uint32_t ph = kNoEntry, pf = kNoEntry;  // unused: this form logs as it goes
                 if (trace_segment<BLOCK, false>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last, cell_list(J, cell),
-                                                key, ph, pf))
-                    while (trace_segment<BLOCK, false>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
-                                                       next_list(J, depth, key), key, ph, pf)) {
+                                                key, pend, ph, pf))
+                    while (shade_mirror(J, ray, pend, depth, o, d, wc) &&
+                           trace_segment<BLOCK, false>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
+                                                       next_list(J, depth, key), key, pend, ph, pf)) {
                     }
                 finish_ray(J, ray, last);
             }
@@ -1769,7 +1804,7 @@ static int resident_blocks_per_cu() {
     return nb;
 }
 
-// Compiled shapes. The queued form keeps 3.5 KB of parked rays per wave in
+// Compiled shapes. The queued form keeps 2 KB of parked rays per wave in
 // LDS on top of the stack, which caps it at 5 waves per SIMD (stack 16) or 3
 // (stack 32); its register budgets are chosen to match.
 bool grouped_variant_exists(int block, int stack, int waves) {
