@@ -205,6 +205,10 @@ typedef struct st_server_s* st_server;
 int st_abi_version(void);
 const char* st_last_error(void);            /* thread-local message of the last failure */
 int st_device_count(int* count);
+/* Free and total device memory of a GPU (either pointer may be NULL); the
+ * drop-in's memory report around run() takes free memory before and after,
+ * as the reference does with cudaMemGetInfo (lib/soltrace_system.cpp:77, 151). */
+int st_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
 int st_context_create(int device, st_context* out);
 int st_context_destroy(st_context ctx);
 

@@ -207,6 +207,7 @@ SIGNATURES = {
     "st_abi_version": (C.c_int, []),
     "st_last_error": (C.c_char_p, []),
     "st_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "st_device_memory": (C.c_int, [C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "st_context_create": (C.c_int, [C.c_int, C.POINTER(P)]),
     "st_context_destroy": (C.c_int, [P]),
     "st_buffer_alloc": (C.c_int, [P, C.c_size_t, C.POINTER(P)]),

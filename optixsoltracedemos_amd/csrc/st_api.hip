@@ -152,6 +152,18 @@ extern "C" int st_device_count(int* count) {
     return ST_OK;
 }
 
+extern "C" int st_device_memory(int device, size_t* free_bytes, size_t* total_bytes) {
+    int n = 0;
+    ST_TRY(st_device_count(&n));
+    if (device < 0 || device >= n) return fail(ST_ERR_INVALID, "device %d outside [0, %d)", device, n);
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return ST_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Device scratch of st_trace_noalloc: a file-scope array, one copy per GPU for
 // the life of the process. [0, kScratchCounterBytes) holds the trace's chunk

@@ -47,7 +47,7 @@ int main(int argc, char** argv) {
     const auto st = system.get_stats();
     const auto hits = system.get_element_hits();
     std::cout << "num_rays, " << num_rays << ", timing_setup, " << system.get_time_setup() << ", timing_trace, "
-              << system.get_time_trace() << std::endl;
+              << system.get_time_trace() << ", launch_memory_mb, " << system.get_launch_memory_mb() << std::endl;
     std::cout << "receiver_hits, " << st[ST_STAT_RECEIVER_HITS] << ", mirror_hits, " << st[ST_STAT_MIRROR_HITS]
               << ", flux_binned, " << st[ST_STAT_FLUX_BINNED] << ", element_hits";
     for (auto h : hits) std::cout << ", " << h;

@@ -90,6 +90,7 @@ void SolTraceSystem::initialize() {
     if (m_element_list.size() < 2) throw std::runtime_error("SolTraceSystem needs at least one heliostat and a receiver");
     if (m_num_sunpoints <= 0) throw std::runtime_error("number of sun points must be positive");
     if (!m_ctx) check(st_context_create(0, &m_ctx), "st_context_create");
+    check(st_device_memory(0, &m_mem_free_before, nullptr), "st_device_memory");
     data_manager->set_context(m_ctx);
     const std::vector<st_element_f64> elems = collect_elements();
     if (m_scene) check(st_scene_destroy(m_scene), "st_scene_destroy");
@@ -168,6 +169,10 @@ void SolTraceSystem::run() {
     o.stats = m_stats;
     o.records = (float*)data_manager->launch_params_H.hit_point_buffer;
     o.record_elements = m_record_elements;
+    size_t free_now = 0;
+    check(st_device_memory(0, &free_now, nullptr), "st_device_memory");
+    m_launch_memory_mb = ((double)m_mem_free_before - (double)free_now) / (1024.0 * 1024.0);
+    if (m_verbose) std::cout << "run: " << m_launch_memory_mb << " MiB of device memory taken since initialize()" << std::endl;
     m_timer_trace.start();
     check(st_trace(m_scene, &d, &o), "st_trace");
     m_timer_trace.stop();

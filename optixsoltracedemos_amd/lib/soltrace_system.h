@@ -41,6 +41,10 @@ public:
     double get_time_trace() { return m_timer_trace.get_time_sec(); }
     double get_time_setup() { return m_timer_setup.get_time_sec(); }
     void print_launch_params();
+    // device memory taken between the start of initialize() and the start of
+    // the last run() (free memory before minus after, in MiB), the figure the
+    // reference prints in run() (lib/soltrace_system.cpp:77, 149-151)
+    double get_launch_memory_mb() const { return m_launch_memory_mb; }
 
     // ---- extensions (no reference counterpart) -------------------------------
     void set_sun_shape(int st_sun_shape) { m_sun_shape = st_sun_shape; }        // ST_SUN_PILLBOX / GAUSSIAN
@@ -104,6 +108,8 @@ private:
     uint64_t* m_stats = nullptr;            // device
     size_t m_traced_rays = 0;
     int m_traced_depth = 0;
+    size_t m_mem_free_before = 0;
+    double m_launch_memory_mb = 0.0;
 
     Timer m_timer_setup;
     Timer m_timer_trace;

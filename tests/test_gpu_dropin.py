@@ -67,6 +67,10 @@ def test_three_heliostats_demo_vs_oracle(oracle_mod, tmp_path):
         assert np.array_equal(mine[:, 1], rec_o[r][m][:, 0])
         k += len(mine)
     assert skipped <= 1 and k > 1000
+    # memory report of run() (free memory at initialize() minus at run()): at
+    # least the hit buffer initialize() allocates, n rays x 5 slots x 16 B
+    first = kv([ln for ln in lines if ln.startswith("num_rays")][0])
+    assert n * 5 * 16 / 2**20 <= float(first["launch_memory_mb"]) < 4096
 
 
 # OPTICAL rows of the fixture (front side): reflectivity, RMS slope, RMS specularity (mrad)
