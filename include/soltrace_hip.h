@@ -173,7 +173,11 @@ typedef struct st_trace_desc {
     int32_t flux_nx, flux_ny; /* flux grid; 0 -> no flux map                      */
     int32_t block_size;     /* 0 -> tuned default                                 */
     int32_t grid_blocks;    /* 0 -> tuned default (persistent kernel)              */
-    int32_t reserved;
+    int32_t waves_per_simd; /* 0 -> tuned default; else the register budget of the
+                               kernel variant to run (queued forms: 4 or 5 with the
+                               traversal stack in LDS, 6 or 8 with it in private
+                               memory; whole-ray form: 4, 5, 6 or 8); a block size
+                               and budget without a compiled variant is an error */
     void* stream;           /* hipStream_t; NULL = the scene's own stream          */
 } st_trace_desc;
 
@@ -271,7 +275,9 @@ int st_scene_list_info(st_scene scene, uint64_t* grid_items, uint64_t* reflectio
  * the scene's default (used when st_trace_desc.block_size and grid_blocks are
  * 0) and reports it. */
 typedef struct st_tune_result {
-    int32_t block, grid, stack, waves;
+    int32_t block, grid, stack, waves;  /* stack: LDS stack entries per lane; 0 = the
+                                           stack in private memory (frees LDS for 8
+                                           waves per SIMD) */
     int32_t trials;
     int32_t queued;               /* 1: ST_KERNEL_QUEUED form chosen; 2: the same
                                      with list and BVH segments traced in separate
@@ -311,7 +317,7 @@ int st_trace_noalloc(st_scene scene, const st_trace_desc* desc, const st_outputs
  * profiling). */
 typedef struct st_launch_info {
     int32_t kernel;          /* ST_KERNEL_MEGA / _QUEUED / _QUEUED_GROUPED launched */
-    int32_t block_size, grid_blocks, stack, waves_per_simd, chunk_rays;
+    int32_t block_size, grid_blocks, stack, waves_per_simd, chunk_rays;  /* stack 0: private */
     int32_t lds_hits;        /* element hits counted per wave in LDS              */
     int32_t flux_copies;     /* copies the flux map was accumulated in (1 = none) */
     uint64_t chunks;

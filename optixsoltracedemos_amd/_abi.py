@@ -108,7 +108,7 @@ class TraceDesc(C.Structure):
         ("flux_ny", C.c_int32),
         ("block_size", C.c_int32),
         ("grid_blocks", C.c_int32),
-        ("reserved", C.c_int32),
+        ("waves_per_simd", C.c_int32),
         ("stream", C.c_void_p),
     ]
 

@@ -77,7 +77,7 @@ int DevBuf::ensure(size_t n, const char* what) {
 }
 
 LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel, int32_t bvh_depth,
-                       const LaunchShape* tuned, int32_t min_elements) {
+                       const LaunchShape* tuned, int32_t min_elements, int32_t waves, bool listed) {
     LaunchShape sh;
     const int cus = device_cu_count();
     // ST_KERNEL_MEGA: whole ray per lane; ST_KERNEL_QUEUED: continuing rays
@@ -85,7 +85,7 @@ LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kerne
     // an explicit grouped form is a queued form that must keep a grouped variant
     const bool want_grouped = kernel == ST_KERNEL_QUEUED_GROUPED;
     const bool want_queued = kernel == ST_KERNEL_QUEUED || want_grouped;
-    const bool use_tuned = tuned && block <= 0 && grid <= 0 &&
+    const bool use_tuned = tuned && block <= 0 && grid <= 0 && waves <= 0 &&
                            (kernel == ST_KERNEL_AUTO || want_queued == tuned->queued);
     if (use_tuned) {
         sh = *tuned;
@@ -95,14 +95,29 @@ LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kerne
         sh.grid = grid > 0 ? grid : cus * 8;
         sh.stack = bvh_depth <= 16 && sh.block >= 128 ? 16 : 32;
         sh.waves = sh.queued ? (sh.stack == 16 ? 5 : 3) : (sh.stack == 16 ? 6 : 4);
+        if (listed && sh.queued && waves <= 0 && bvh_depth <= kStackDepth &&
+            trace_variant_exists(sh.block, 0, 8, true)) {
+            sh.stack = 0;
+            sh.waves = 8;
+        }
     }
-    if (sh.stack < bvh_depth) sh.stack = 32;
-    if (!trace_variant_exists(sh.block, sh.stack, sh.waves, sh.queued)) {
-        sh.stack = 32;
-        sh.waves = sh.queued ? 3 : 4;
-        if (!trace_variant_exists(sh.block, sh.stack, sh.waves, sh.queued)) sh.block = 128;
+    if (waves > 0 && !use_tuned) {
+        sh.waves = waves;
+        for (int stk : {16, 32, 0})
+            if (stack_entries(stk) >= bvh_depth && trace_variant_exists(sh.block, stk, waves, sh.queued)) {
+                sh.stack = stk;
+                break;
+            }
+    } else {
+        if (stack_entries(sh.stack) < bvh_depth) sh.stack = 32;
+        if (!trace_variant_exists(sh.block, sh.stack, sh.waves, sh.queued)) {
+            sh.stack = 32;
+            sh.waves = sh.queued ? 3 : 4;
+            if (!trace_variant_exists(sh.block, sh.stack, sh.waves, sh.queued)) sh.block = 128;
+        }
     }
-    if (sh.queued && !want_grouped && min_elements <= kLdsHitElements && bvh_depth <= 16 && block <= 0) {
+    if (sh.queued && !want_grouped && min_elements <= kLdsHitElements && bvh_depth <= 16 && block <= 0 &&
+        waves <= 0) {
         sh.block = 256;
         sh.stack = bvh_depth <= 8 ? 8 : 16;
         sh.waves = 5;
@@ -113,8 +128,9 @@ LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kerne
     // that fits one chunk per resident wave gets exactly that (launch_trace
     // then skips the counter); larger ones get about four chunks per wave.
     const uint64_t resident = (uint64_t)trace_resident_waves(sh);
-    const uint64_t waves = (uint64_t)sh.grid * (uint64_t)(sh.block / 64);
-    uint64_t chunk = resident > 0 && rays <= resident * 2048 ? (rays + resident - 1) / resident : rays / (4 * waves);
+    const uint64_t grid_waves = (uint64_t)sh.grid * (uint64_t)(sh.block / 64);
+    uint64_t chunk = resident > 0 && rays <= resident * 2048 ? (rays + resident - 1) / resident
+                                                              : rays / (4 * grid_waves);
     chunk = (chunk + 63) / 64 * 64;
     sh.chunk = (int)std::min<uint64_t>(std::max<uint64_t>(chunk, 64), 2048);
     sh.kernel = sh.queued ? ST_KERNEL_QUEUED : ST_KERNEL_MEGA;
@@ -924,6 +940,7 @@ int validate_trace(st_scene s, const st_trace_desc* d, const st_outputs* out) {
     if (d->block_size != 0 && d->block_size != 64 && d->block_size != 128 && d->block_size != 256)
         return fail(ST_ERR_INVALID, "block_size must be 0, 64, 128 or 256");
     if (d->grid_blocks < 0) return fail(ST_ERR_INVALID, "negative grid_blocks");
+    if (d->waves_per_simd < 0) return fail(ST_ERR_INVALID, "negative waves_per_simd");
     if (d->kernel < 0 || d->kernel > ST_KERNEL_QUEUED_GROUPED)
         return fail(ST_ERR_INVALID, "unknown kernel %d", d->kernel);
     if ((d->flags & ST_TRACE_RECORDS) && (!out || !out->records))
@@ -942,7 +959,10 @@ namespace st {
 // second-reflection lists).
 int trace_shape(st_scene s, const st_trace_desc* d, const JobDev& J, LaunchShape* out, uint64_t* chunks) {
     LaunchShape sh = pick_shape(J.ray_count, d->block_size, d->grid_blocks, d->kernel, s->topo.depth,
-                                s->tuned_valid ? &s->tuned : nullptr, s->n);
+                                s->tuned_valid ? &s->tuned : nullptr, s->n, d->waves_per_simd, J.bounce_range != nullptr);
+    if (d->waves_per_simd > 0 && !trace_variant_exists(sh.block, sh.stack, sh.waves, sh.queued))
+        return fail(ST_ERR_INVALID, "no %s trace variant for block %d at %d waves/SIMD (BVH depth %d)",
+                    sh.queued ? "queued" : "whole-ray", sh.block, d->waves_per_simd, s->topo.depth);
     bool group = false;
     if (d->kernel == ST_KERNEL_QUEUED_GROUPED) {
         group = true;
@@ -1113,7 +1133,7 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
         return ST_OK;
     };
     const int cus = device_cu_count();
-    const LaunchShape def = pick_shape(sample_rays, 0, 0, ST_KERNEL_AUTO, s->topo.depth, nullptr, s->n);
+    const LaunchShape def = pick_shape(sample_rays, 0, 0, ST_KERNEL_AUTO, s->topo.depth, nullptr, s->n, 0, s->bounce_valid);
     double def_ms = 0;
     int rc = time_shape(def, &def_ms);
     LaunchShape best = def;
@@ -1123,14 +1143,14 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
     if (log_trials)
         fprintf(stderr, "st_scene_autotune: default block %d stack %d waves %d queued %d grid %d chunk %d: %.3f ms\n",
                 def.block, def.stack, def.waves, (int)def.queued, def.grid, def.chunk, def_ms);
-    const int blocks[] = {64, 128, 256}, stacks[] = {16, 32}, wavesv[] = {3, 4, 5, 6, 8}, per_cu[] = {4, 8, 16};
+    const int blocks[] = {64, 128, 256}, stacks[] = {0, 16, 32}, wavesv[] = {3, 4, 5, 6, 8}, per_cu[] = {4, 8, 16};
     for (int rf = 0; rf < 2; ++rf)
     for (int b : blocks)
         for (int stk : stacks)
             for (int w : wavesv)
                 for (int pc : per_cu) {
                     if (rc != ST_OK) break;
-                    if (stk < s->topo.depth || !trace_variant_exists(b, stk, w, rf != 0)) continue;
+                    if (stack_entries(stk) < s->topo.depth || !trace_variant_exists(b, stk, w, rf != 0)) continue;
                     LaunchShape sh = def;
                     sh.block = b;
                     sh.stack = stk;
@@ -1421,7 +1441,7 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
         uint64_t total_rays = 0;
         for (int k = 0; k < K; ++k) total_rays += d->budgets[k];
         b.shape = pick_shape(total_rays, d->block_size, d->grid_blocks, ST_KERNEL_AUTO, s->topo.depth,
-                             s->tuned_valid ? &s->tuned : nullptr, s->n);
+                             s->tuned_valid ? &s->tuned : nullptr, s->n, 0, b.lists);
         use_grouped_bvh(b.shape, b.lists);  // per-step first-reflection lists, no pair lists
         std::vector<JobDev> jobs(K);
         std::vector<uint64_t> prefix(K + 1, 0);

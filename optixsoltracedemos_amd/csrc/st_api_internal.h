@@ -30,14 +30,19 @@ struct DevBuf {
     int ensure(size_t n, const char* what);  // (re)allocate to at least n bytes
 };
 
-// launch shape for `rays` rays; block/grid > 0 override, else the scene's tuned
-// shape when there is one, else defaults (16-entry stack when the BVH allows);
-// bvh_depth: the deepest scene in the launch; min_elements: the smallest scene
-// in the launch (when it has at most kLdsHitElements elements, the launch
-// counts element hits below that index per wave in LDS: a small scene's few
-// counters would otherwise take every wave's device atomics)
+// launch shape for `rays` rays; block/grid/waves > 0 override, else the scene's
+// tuned shape when there is one, else defaults (16-entry stack when the BVH
+// allows); bvh_depth: the deepest scene in the launch; min_elements: the
+// smallest scene in the launch (when it has at most kLdsHitElements elements,
+// the launch counts element hits below that index per wave in LDS: a small
+// scene's few counters would otherwise take every wave's device atomics).
+// An explicit waves-per-SIMD budget takes the smallest compiled stack that holds
+// the BVH; the caller checks that the variant exists (trace_variant_exists).
+// listed: the launch's reflected rays take candidate lists, so its BVH walks
+// are rare and the default queued shape keeps the stack in private memory (8
+// waves per SIMD instead of 5).
 LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel, int32_t bvh_depth,
-                       const LaunchShape* tuned, int32_t min_elements);
+                       const LaunchShape* tuned, int32_t min_elements, int32_t waves = 0, bool listed = false);
 
 struct BatchCache {
     bool valid = false;

@@ -82,18 +82,20 @@ struct LaunchShape {
     int grid = 0;        // persistent blocks (0 -> 8 per CU)
     int chunk = 2048;    // rays per chunk (multiple of 64)
     int kernel = ST_KERNEL_MEGA;
-    int stack = 32;      // traversal stack entries per lane (16 or 32, >= BVH depth)
+    int stack = 32;      // traversal stack entries per lane in LDS (16 or 32, >= BVH depth; 0: kStackDepth in private memory)
     int waves = 4;       // waves per SIMD the register allocation is built for
     bool queued = false; // continuing rays parked in LDS, traced in full waves
     bool lds_hits = false;  // hit counts of elements below kLdsHitElements kept per wave in
                             // LDS (launches with a scene that small; queued, block 256, stack 8 or 16)
     bool group_bvh = false; // queued form tracing list and BVH segments in separate steps
-                            // (use_grouped_bvh; compiled for stack 16, 5 waves/SIMD)
+                            // (use_grouped_bvh; compiled for stack 16 at 5 waves/SIMD and stack 0 at 6 or 8)
 };
 // Scenes this small count element hits in LDS: their few counters would
 // otherwise take a device-scope atomic from nearly every wave of the launch.
 constexpr int kLdsHitElements = 128;
 bool trace_variant_exists(int block, int stack, int waves, bool queued);
+// traversal stack entries of a shape's stack setting
+inline int stack_entries(int stack) { return stack > 0 ? stack : kStackDepth; }
 bool grouped_variant_exists(int block, int stack, int waves);
 // A launch of jobs whose reflected rays partly have candidate lists and partly
 // walk the BVH (first-reflection lists without second-reflection lists, or a

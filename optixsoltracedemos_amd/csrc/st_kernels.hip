@@ -925,17 +925,23 @@ __device__ __forceinline__ uint32_t queue_tag(uint32_t offset, int depth, int ke
 
 // LHITS > 0: element hit counts of scenes with at most LHITS elements are
 // kept per wave in LDS and added to the output once per chunk.
+// STACK = 0: the traversal stack (kStackDepth entries) is the lane's private
+// memory instead of LDS. A BVH walk then goes through the scratch path, but the
+// LDS left to each wave (parked rays, log, counters) is small enough for 8
+// waves per SIMD: the form for scenes whose rays rarely walk the BVH.
 template <int BLOCK, int STACK, int MINW, int QUEUED, int LHITS>
 __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict__ jobs,
                                                        const uint64_t* __restrict__ chunk_prefix, int32_t n_jobs,
                                                        uint64_t total_chunks, uint32_t* __restrict__ work_counter,
                                                        int32_t chunk_rays, int32_t static_chunks, JobDev single) {
-    __shared__ int32_t lds_stack[STACK * BLOCK];
+    constexpr int SS = STACK > 0 ? BLOCK : 1;  // stride of the stack's entries
+    __shared__ int32_t lds_stack[STACK > 0 ? STACK * BLOCK : 1];
     __shared__ uint32_t lds_counters[BLOCK / 64][kNumCounters];
     __shared__ float lds_queue[QUEUED ? BLOCK / 64 : 1][QUEUED ? 8 * kQueueCap : 1];
     __shared__ uint32_t lds_log[BLOCK / 64][1 + kLogCap];
     __shared__ uint32_t lds_hits[LHITS > 0 ? BLOCK / 64 : 1][LHITS > 0 ? LHITS : 1];
-    int32_t* stk = lds_stack + threadIdx.x;
+    int32_t private_stack[STACK > 0 ? 1 : kStackDepth];
+    int32_t* stk = STACK > 0 ? lds_stack + threadIdx.x : private_stack;
     const int lane = threadIdx.x & 63;
     uint32_t* wc = lds_counters[threadIdx.x >> 6];
     if (lane < kNumCounters) wc[lane] = 0;
@@ -1008,7 +1014,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                         int last = depth;
                         float tmin = kTminBounce;
                         alive = shade_mirror(J, ray, pend, depth, o, d, wc) &&
-                                trace_segment<BLOCK, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last, L, key,
+                                trace_segment<SS, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last, L, key,
                                                            pend, ph, pf);
                         if (!alive) finish_ray(J, ray, last);
                     }
@@ -1018,7 +1024,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                         int last = depth;
                         float tmin = kTminBounce;
                         alive = shade_mirror(J, ray, pend, depth, o, d, wc) &&
-                                trace_segment<BLOCK, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
+                                trace_segment<SS, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
                                                            next_list(J, depth, key), key, pend, ph, pf);
                         if (!alive) finish_ray(J, ray, last);
                     }
@@ -1039,7 +1045,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                     depth = 0;
                     key = -1;
                     off = (uint32_t)(i - first);
-                    alive = trace_segment<BLOCK, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
+                    alive = trace_segment<SS, true>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
                                                        cell_list(J, cell), key, pend, ph, pf);
                     if (!alive) finish_ray(J, ray, last);
                 }
@@ -1087,10 +1093,10 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                 float tmin = kTminPrimary;
                 int depth = 0, last = 0, key = -1, pend = 0;
                 uint32_t ph = kNoEntry, pf = kNoEntry;  // unused: this form logs as it goes
-                if (trace_segment<BLOCK, false>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last, cell_list(J, cell),
+                if (trace_segment<SS, false>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last, cell_list(J, cell),
                                                 key, pend, ph, pf))
                     while (shade_mirror(J, ray, pend, depth, o, d, wc) &&
-                           trace_segment<BLOCK, false>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
+                           trace_segment<SS, false>(J, ray, stk, wc, wh, wl, o, d, tmin, depth, last,
                                                        next_list(J, depth, key), key, pend, ph, pf)) {
                     }
                 finish_ray(J, ray, last);
@@ -1806,12 +1812,15 @@ static int resident_blocks_per_cu() {
 
 // Compiled shapes. The queued form keeps 2 KB of parked rays per wave in
 // LDS on top of the stack, which caps it at 5 waves per SIMD (stack 16) or 3
-// (stack 32); its register budgets are chosen to match.
+// (stack 32), or 8 with the stack in private memory (stack 0); its register
+// budgets are chosen to match.
 bool grouped_variant_exists(int block, int stack, int waves) {
-    return stack == 16 && waves == 5 && (block == 128 || block == 256);
+    if (block != 128 && block != 256) return false;
+    return (stack == 16 && waves == 5) || (stack == 0 && (waves == 6 || waves == 8));
 }
 bool trace_variant_exists(int block, int stack, int waves, bool queued) {
     if (queued) {
+        if (stack == 0) return (block == 128 || block == 256) && (waves == 6 || waves == 8);
         if (stack == 16) return (block == 128 || block == 256) && (waves == 4 || waves == 5);
         if (stack == 32) return (block == 64 || block == 128) && waves == 3;
         return false;
@@ -1840,8 +1849,16 @@ bool trace_variant_exists(int block, int stack, int waves, bool queued) {
     ST_VARIANT(256, 16, 5, 1, 0)       \
     ST_VARIANT(64, 32, 3, 1, 0)        \
     ST_VARIANT(128, 32, 3, 1, 0)       \
+    ST_VARIANT(128, 0, 6, 1, 0)        \
+    ST_VARIANT(128, 0, 8, 1, 0)        \
+    ST_VARIANT(256, 0, 6, 1, 0)        \
+    ST_VARIANT(256, 0, 8, 1, 0)        \
     ST_VARIANT(128, 16, 5, 2, 0)       \
     ST_VARIANT(256, 16, 5, 2, 0)       \
+    ST_VARIANT(128, 0, 6, 2, 0)        \
+    ST_VARIANT(128, 0, 8, 2, 0)        \
+    ST_VARIANT(256, 0, 6, 2, 0)        \
+    ST_VARIANT(256, 0, 8, 2, 0)        \
     ST_VARIANT(256, 8, 5, 1, kLdsHitElements) \
     ST_VARIANT(256, 16, 5, 1, kLdsHitElements)
 

@@ -520,10 +520,10 @@ void worker_main(st_server srv) {
             depth = std::max(depth, j->scene->topo.depth);
             min_n = std::min(min_n, j->scene->n);
         }
-        LaunchShape sh = pick_shape(round_rays, srv->cfg.block_size, srv->cfg.grid_blocks, ST_KERNEL_AUTO, depth,
-                                    nullptr, min_n);
         bool lists = false;  // long jobs have first-reflection lists and no pair lists
         for (ServerJob* j : round) lists = lists || j->lists;
+        LaunchShape sh = pick_shape(round_rays, srv->cfg.block_size, srv->cfg.grid_blocks, ST_KERNEL_AUTO, depth,
+                                    nullptr, min_n, 0, lists);
         use_grouped_bvh(sh, lists);
         // slot `slot` was last used two rounds ago, and that round has retired
         const int slot = next_slot;

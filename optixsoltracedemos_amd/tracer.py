@@ -155,7 +155,7 @@ class Scene:
     def make_desc(self, num_rays, seed=0, job_id=0, max_depth=5, flux_nx=0, flux_ny=0, flux_element=-1,
                   ray_begin=0, ray_count=0, records=False, include_direct_sun=False, zero=True,
                   kernel=_abi.KERNEL_AUTO, block_size=0, grid_blocks=0, stream=None, bvh_only=False,
-                  count_work=False) -> TraceDesc:
+                  count_work=False, waves_per_simd=0) -> TraceDesc:
         d = TraceDesc()
         d.num_rays, d.ray_begin, d.ray_count, d.seed = int(num_rays), int(ray_begin), int(ray_count), int(seed)
         d.job_id, d.max_depth = int(job_id), int(max_depth)
@@ -163,7 +163,7 @@ class Scene:
                    | (_abi.TRACE_ZERO_OUTPUTS if zero else 0) | (_abi.TRACE_BVH_ONLY if bvh_only else 0)
                    | (_abi.TRACE_COUNT_WORK if count_work else 0))
         d.kernel, d.flux_element, d.flux_nx, d.flux_ny = int(kernel), int(flux_element), int(flux_nx), int(flux_ny)
-        d.block_size, d.grid_blocks = int(block_size), int(grid_blocks)
+        d.block_size, d.grid_blocks, d.waves_per_simd = int(block_size), int(grid_blocks), int(waves_per_simd)
         d.stream = stream
         return d
 

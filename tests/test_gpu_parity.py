@@ -272,10 +272,13 @@ def test_queued_and_whole_ray_loops_bitwise(gpu_ctx, scene):
     sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=pos)
     kw = dict(max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny, records=True, seed=11)
     ref = sc.trace(n, kernel=_abi.KERNEL_MEGA, **kw).numpy()
-    for bs, grid in ((256, 0), (128, 5), (64, 0)):
-        r = sc.trace(n, kernel=_abi.KERNEL_QUEUED, block_size=bs, grid_blocks=grid, **kw).numpy()
+    for bs, grid, waves in ((256, 0, 0), (128, 5, 0), (64, 0, 0), (128, 0, 8), (256, 9, 6)):
+        r = sc.trace(n, kernel=_abi.KERNEL_QUEUED, block_size=bs, grid_blocks=grid, waves_per_simd=waves,
+                     **kw).numpy()
+        if waves:
+            assert sc.last_launch()["stack"] == 0, sc.last_launch()
         for k in ("flux", "element_hits", "stats", "records", "record_elements"):
-            assert np.array_equal(r[k], ref[k]), (scene, bs, grid, k)
+            assert np.array_equal(r[k], ref[k]), (scene, bs, grid, waves, k)
     assert ref["stats"][0] == n and ray_outcomes_balance(ref["stats"].astype(np.int64))
 
 
@@ -359,11 +362,17 @@ def test_candidate_lists_equal_bvh(gpu_ctx, case):
     info = sc.list_info()
     if case.startswith("field") and case != "field_low_sun":
         assert info["grid_items"] > 0 and info["reflection_items"] > 0 and info["pair_items"] > 0, info
-    for kernel in (_abi.KERNEL_QUEUED, _abi.KERNEL_MEGA, _abi.KERNEL_QUEUED_GROUPED):
-        r = sc.trace(n, kernel=kernel, **kw).numpy()
-        assert sc.last_launch()["kernel"] == kernel, (case, kernel, sc.last_launch())
+    # (form, block, waves per SIMD): the default variants, then those whose
+    # traversal stack lives in private memory (8 and 6 waves per SIMD)
+    for kernel, bs, waves in ((_abi.KERNEL_QUEUED, 0, 0), (_abi.KERNEL_MEGA, 0, 0), (_abi.KERNEL_QUEUED_GROUPED, 0, 0),
+                              (_abi.KERNEL_QUEUED, 256, 8), (_abi.KERNEL_QUEUED_GROUPED, 128, 6)):
+        r = sc.trace(n, kernel=kernel, block_size=bs, waves_per_simd=waves, **kw).numpy()
+        li = sc.last_launch()
+        assert li["kernel"] == kernel, (case, kernel, li)
+        if waves:
+            assert li["stack"] == 0 and li["waves_per_simd"] == waves and li["block_size"] == bs, (case, li)
         for k in ("flux", "element_hits", "stats", "records", "record_elements"):
-            assert np.array_equal(r[k], ref[k]), (case, kernel, k)
+            assert np.array_equal(r[k], ref[k]), (case, kernel, waves, k)
 
 
 def test_lists_build_lazily_and_follow_the_sun(gpu_ctx):
@@ -608,6 +617,11 @@ def test_invalid_trace_requests_are_rejected(gpu_ctx):
     # an explicit grouped form that no variant fits is an error, not another kernel
     with pytest.raises(_abi.STError, match="grouped"):
         sc.trace(1000, kernel=_abi.KERNEL_QUEUED_GROUPED, block_size=64)
+    # so is a register budget without a compiled variant
+    with pytest.raises(_abi.STError, match="waves/SIMD"):
+        sc.trace(1000, kernel=_abi.KERNEL_QUEUED, block_size=256, waves_per_simd=7)
+    with pytest.raises(_abi.STError, match="waves/SIMD"):
+        sc.trace(1000, kernel=_abi.KERNEL_QUEUED, block_size=64, waves_per_simd=8)
 
 
 def test_injected_faults_fail_cleanly(gpu_ctx):
@@ -635,7 +649,8 @@ def test_autotune_keeps_results_bitwise(gpu_ctx):
     before = sc.trace(3_000_000, **kw).numpy()
     t = sc.autotune(sample_rays=4_000_000)
     assert t["trials"] > 1 and 0 < t["best_ms"] <= t["default_ms"]
-    assert t["block"] in (64, 128, 256) and t["stack"] >= sc.bvh_info()["depth"]
+    # stack 0: the traversal stack in private memory, 32 entries
+    assert t["block"] in (64, 128, 256) and (t["stack"] or 32) >= sc.bvh_info()["depth"]
     after = sc.trace(3_000_000, **kw).numpy()
     for k in ("flux", "element_hits", "stats"):
         assert np.array_equal(before[k], after[k]), k
