@@ -146,6 +146,8 @@ def main():
                          "segments in separate steps)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--waves", type=int, default=0,
+                    help="waves per SIMD of the trace variant (0: tuned; 6 or 8: stack in private memory)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -177,13 +179,14 @@ def main():
     pos = {"stratified": _abi.POS_STRATIFIED, "philox": _abi.POS_PHILOX, "halton": _abi.POS_HALTON}[args.sampling]
     sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=pos)
     tuning = None
-    if not args.no_autotune and args.block == 0 and args.grid == 0:
+    if not args.no_autotune and args.block == 0 and args.grid == 0 and args.waves == 0:
         tuning = sc.autotune(sample_rays=rays, max_depth=w.max_depth)   # tune at the measured size
     stream = torch.cuda.Stream(device=0)
     kernel = {"auto": _abi.KERNEL_AUTO, "mega": _abi.KERNEL_MEGA, "queued": _abi.KERNEL_QUEUED,
               "grouped": _abi.KERNEL_QUEUED_GROUPED}[args.kernel]
     desc = sc.make_desc(rays, seed=args.seed, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny,
-                        zero=False, block_size=args.block, grid_blocks=args.grid, stream=stream.cuda_stream,
+                        zero=False, block_size=args.block, grid_blocks=args.grid, waves_per_simd=args.waves,
+                        stream=stream.cuda_stream,
                         kernel=kernel)
     res = sc.alloc_outputs(desc)
     torch.cuda.synchronize()
@@ -209,7 +212,8 @@ def main():
     value = args.steps * rays / wall
     # one more launch of the same work, untimed, counting node visits and tests
     wdesc = sc.make_desc(rays, seed=args.seed, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny,
-                         zero=True, block_size=args.block, grid_blocks=args.grid, stream=stream.cuda_stream,
+                         zero=True, block_size=args.block, grid_blocks=args.grid, waves_per_simd=args.waves,
+                         stream=stream.cuda_stream,
                          kernel=kernel, count_work=True)
     wres = sc.alloc_outputs(wdesc)
     sc.trace_into(wdesc, wres)
@@ -254,7 +258,8 @@ def main():
         sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=_abi.POS_HALTON)
         sc.build_lists()   # as the headline's tuner does: no lazy list build inside the timed launches
         hdesc = sc.make_desc(rays, seed=args.seed, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny,
-                             zero=True, block_size=args.block, grid_blocks=args.grid, stream=stream.cuda_stream,
+                             zero=True, block_size=args.block, grid_blocks=args.grid, waves_per_simd=args.waves,
+                             stream=stream.cuda_stream,
                              kernel=kernel)
         hres = sc.alloc_outputs(hdesc)
         hms = time_launches(sc, hdesc, hres, stream, 5)
@@ -265,7 +270,7 @@ def main():
         sc.set_sun(w.sun_vector, w.sun_param, w.sun_shape, position_sampling=pos)
         sc.build_lists()
         rdesc = sc.make_desc(rays, seed=args.seed, max_depth=w.max_depth, flux_nx=w.flux_nx, flux_ny=w.flux_ny,
-                             zero=True, records=True, block_size=args.block, grid_blocks=args.grid,
+                             zero=True, records=True, block_size=args.block, grid_blocks=args.grid, waves_per_simd=args.waves,
                              stream=stream.cuda_stream, kernel=kernel)
         rres = sc.alloc_outputs(rdesc)
         rms = time_launches(sc, rdesc, rres, stream, 3)
