@@ -529,18 +529,13 @@ int scene_bounce_lists(st_scene s) {
     return ST_OK;
 }
 
-// Whether a trace's reflected rays mix list segments with BVH ones often enough
-// for the grouped form: first-reflection lists without pair lists, a share of
-// pair lists missing, or pair lists more than kPairsGrouped times as long as
-// the first-reflection lists (many blocked reflections: rays after a second
-// mirror walk the BVH). 10k field: the last ratio is 1.45 / 1.72 / 4.2 / 15.7
-// at 81 / 60 / 20 / 5.5 deg elevation, where the grouped form takes +2 / +2.5 /
-// -21 / -21 % of the plain queued form's time.
-static bool mixed_bvh_segments(st_scene s, const JobDev& J) {
-    if (!J.bounce_range) return false;
-    if (!J.pair_range || s->pair_missing > kPairMissingGrouped) return true;
-    return (double)s->pair_items > kPairsGrouped * (double)s->bounce_items;
-}
+// Whether a trace's reflected rays mix list segments with BVH ones, for the
+// grouped form: whenever first-reflection lists exist (rays after a second
+// mirror always walk the BVH, as do those whose pair list is missing). 10k
+// field, 1e8 rays, block 256 with the stack in private memory, one box: the
+// grouped form takes -8 / -19 / -22 / -23 / -11 / -1 / -1 / -1.5 % of the plain
+// queued form's time at 5 / 15 / 25 / 30 / 40 / 50 / 60 / 80 deg elevation.
+static bool mixed_bvh_segments(st_scene, const JobDev& J) { return J.bounce_range != nullptr; }
 
 int route_flux_copies(DevBuf& buf, JobDev* jobs, int32_t n, uint64_t bins, uint64_t rays, hipStream_t st,
                       uint32_t* copies) {

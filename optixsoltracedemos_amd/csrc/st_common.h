@@ -440,6 +440,49 @@ ST_INLINE bool rect_sweep_misses(const RectSweep& S, const float* b) {
     return rs_plane_of(S, d, 1.0, m) && rs_max_r(S, m) < rs_min_b(m, b);
 }
 
+// A flat candidate of a reflection list: its rectangle (centre c, unit axes X,
+// Y and normal N of its frame, half sizes hw, hh) thickened by eps along each
+// of them, eps the FP32 allowance of the traversal boxes (1 mm and 2e-6 of its
+// coordinates): a ray's FP32 hit on the candidate lies within it.
+struct RectCand {
+    double c[3], X[3], Y[3], N[3];
+    double hw, hh, eps;
+};
+ST_INLINE void rect_cand_init(RectCand& R, const double c[3], double hw, double hh, const double* f) {
+    for (int k = 0; k < 3; ++k) {
+        R.c[k] = c[k];
+        R.X[k] = f[3 * k];
+        R.Y[k] = f[3 * k + 1];
+        R.N[k] = f[3 * k + 2];
+    }
+    R.hw = hw;
+    R.hh = hh;
+    R.eps = 1e-3 + 2e-6 * (fabs(c[0]) + fabs(c[1]) + fabs(c[2]) + hw + hh);
+}
+// smallest m . x over the thickened rectangle
+ST_INLINE double rs_min_rect(const double m[3], const RectCand& R) {
+    return rs_dot(m, R.c) - fabs(rs_dot(m, R.X)) * (R.hw + R.eps) - fabs(rs_dot(m, R.Y)) * (R.hh + R.eps) -
+           fabs(rs_dot(m, R.N)) * R.eps;
+}
+// rect_sweep_misses for a flat candidate, tighter than its box: the same planes
+// against the rectangle itself, and the candidate's own plane, oriented against
+// the cone, when every cone direction leaves on one side of it (rays that start
+// on that side, as all of R does, never reach it)
+ST_INLINE bool rect_sweep_misses_rect(const RectSweep& S, const RectCand& R) {
+    for (int j = 0; j < S.nm; ++j)
+        if (rs_max_r(S, S.M[j]) < rs_min_rect(S.M[j], R)) return true;
+    const double d[3] = {R.c[0] - S.pc[0], R.c[1] - S.pc[1], R.c[2] - S.pc[2]};
+    double m[3];
+    if (rs_plane_of(S, d, 1.0, m) && rs_max_r(S, m) < rs_min_rect(m, R)) return true;
+    const double na = rs_dot(R.N, S.a);
+    if (fabs(na) >= S.sph) {
+        const double s = na > 0.0 ? -1.0 : 1.0;
+        const double mn[3] = {s * R.N[0], s * R.N[1], s * R.N[2]};
+        if (rs_max_r(S, mn) < rs_min_rect(mn, R)) return true;
+    }
+    return false;
+}
+
 // Cone from the apex (origin) with unit axis a and half angle phi; pieces of
 // the cone are bounded by boxes over [0, first], then [t, t * ratio] up to the
 // far corner of the tested box (ratio fixed per launch from the scene size).

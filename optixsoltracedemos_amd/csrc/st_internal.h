@@ -98,11 +98,9 @@ bool trace_variant_exists(int block, int stack, int waves, bool queued);
 inline int stack_entries(int stack) { return stack > 0 ? stack : kStackDepth; }
 bool grouped_variant_exists(int block, int stack, int waves);
 // A launch of jobs whose reflected rays partly have candidate lists and partly
-// walk the BVH (first-reflection lists without second-reflection lists, or a
-// noticeable share of second-reflection lists missing) traces the two kinds
-// of segment in separate steps when its shape allows it.
-constexpr double kPairMissingGrouped = 0.01;
-constexpr double kPairsGrouped = 2.0;
+// walk the BVH (any launch with first-reflection lists: segments after a
+// second mirror, and those without a second-reflection list, walk it) traces
+// the two kinds of segment in separate steps when its shape allows it.
 inline void use_grouped_bvh(LaunchShape& sh, bool mixed) {
     sh.group_bvh = mixed && sh.queued && !sh.lds_hits && grouped_variant_exists(sh.block, sh.stack, sh.waves);
 }

@@ -1563,8 +1563,12 @@ __device__ __forceinline__ void sun_reflection_axis(const double* __restrict__ f
 }
 
 // Reflection-list candidates of a flat mirror culled against its rectangle
-// (rect_sweep_misses); returns the number kept, compacted in place.
-__device__ __forceinline__ uint32_t cull_by_rectangle(const st_element_f64& E, const double* __restrict__ f,
+// (rect_sweep_misses on the candidate's box; for a flat candidate also
+// rect_sweep_misses_rect on its own rectangle, element frames of this scene
+// state); returns the number kept, compacted in place.
+__device__ __forceinline__ uint32_t cull_by_rectangle(const st_element_f64* __restrict__ elems,
+                                                      const st_element_f64& E, const double* __restrict__ frames,
+                                                      const double* __restrict__ f, const uint32_t* __restrict__ meta,
                                                       const Cone& C, double phi, const float* __restrict__ bounds_slot,
                                                       int32_t* items, uint32_t len) {
     RectSweep S;
@@ -1572,7 +1576,15 @@ __device__ __forceinline__ uint32_t cull_by_rectangle(const st_element_f64& E, c
     uint32_t kept = 0;
     for (uint32_t i = 0; i < len; ++i) {
         const int32_t v = items[i];
-        if (!rect_sweep_misses(S, bounds_slot + 6 * (size_t)v)) items[kept++] = v;
+        if (rect_sweep_misses(S, bounds_slot + 6 * (size_t)v)) continue;
+        const uint32_t m = meta[v];
+        if (meta_kind(m) == kKindFlat) {
+            const uint32_t e = meta_elem(m);
+            RectCand R;
+            rect_cand_init(R, elems[e].origin, 0.5 * elems[e].dim_x, 0.5 * elems[e].dim_y, frames + 9 * (size_t)e);
+            if (rect_sweep_misses_rect(S, R)) continue;
+        }
+        items[kept++] = v;
     }
     return kept;
 }
@@ -1659,8 +1671,9 @@ __global__ __launch_bounds__(kConeBlock) void k_cone_lists(const st_element_f64*
         none();
         return;
     }
-    if (elems[from].surface == ST_SURFACE_FLAT) len = cull_by_rectangle(elems[from], frames + 9 * (size_t)from, C, phi,
-                                                                        bounds_slot, list + 1, len);
+    if (elems[from].surface == ST_SURFACE_FLAT)
+        len = cull_by_rectangle(elems, elems[from], frames, frames + 9 * (size_t)from, meta, C, phi, bounds_slot,
+                                list + 1, len);
     uint32_t nf = 0;  // flat candidates first
     for (uint32_t i = 0; i < len; ++i) {
         const int32_t v = list[1 + i];

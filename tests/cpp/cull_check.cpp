@@ -9,6 +9,8 @@
 //                                          scaled by phi_scale
 //   cull_check sweep <seed> <phi_scale>   rect_sweep_misses with the cone angle
 //                                          scaled by phi_scale when building it
+//   cull_check sweeprect <seed> <phi_scale> rect_sweep_misses_rect (flat candidates'
+//                                          rectangles), the same scaling
 //   cull_check grid <seed> <tan_scale>    grid_range / cell_meets with the cone's
 //                                          tangent scaled by tan_scale
 //   cull_check gridbox <seed> <tan_scale> the same for a curved primitive (box
@@ -154,6 +156,90 @@ int check_sweep(double phi_scale) {
     }
     std::printf("sweep reachable_boxes %ld reachable_dropped %ld boxes %ld dropped %ld dropped_but_hit %ld\n",
                 reach_boxes, reach_dropped, boxes, dropped, dropped_hit);
+    return 0;
+}
+
+// exact ray-rectangle test (rectangle centre c, frame f columns x, y, normal;
+// half sizes hw, hh), forward hits only
+bool ray_hits_rect(const double o[3], const double d[3], const double c[3], const double f[9], double hw, double hh) {
+    const double n[3] = {f[2], f[5], f[8]}, X[3] = {f[0], f[3], f[6]}, Y[3] = {f[1], f[4], f[7]};
+    const double dn = dot(d, n);
+    if (std::fabs(dn) < 1e-300) return false;
+    const double oc[3] = {c[0] - o[0], c[1] - o[1], c[2] - o[2]};
+    const double t = dot(oc, n) / dn;
+    if (!(t > 0.0)) return false;
+    const double w[3] = {o[0] + t * d[0] - c[0], o[1] + t * d[1] - c[1], o[2] + t * d[2] - c[2]};
+    return std::fabs(dot(w, X)) <= hw && std::fabs(dot(w, Y)) <= hh;
+}
+
+// The same sweep against flat candidates' rectangles (rect_sweep_misses_rect):
+// (i) rectangles through points those rays pass are never dropped; (ii) every
+// dropped rectangle is missed by many such rays.
+int check_sweep_rect(double phi_scale) {
+    long reach = 0, reach_dropped = 0, dropped = 0, dropped_hit = 0, rects = 0, box_kept = 0;
+    for (int cfg = 0; cfg < 150; ++cfg) {
+        double f[9];
+        random_frame(f);
+        const double pc[3] = {uni(-800, 800), uni(-800, 800), uni(-20, 40)};
+        const double hw = uni(0.3, 6.0), hh = uni(0.3, 6.0);
+        const double phi = std::exp(uni(std::log(2e-3), std::log(0.3)));
+        double a[3];
+        random_unit(a);
+        RectSweep S;
+        rect_sweep_init(S, pc, hw, hh, f, a, phi * phi_scale);
+        auto ray = [&](double o[3], double u[3]) {
+            const double x = uni(-hw, hw), y = uni(-hh, hh);
+            for (int k = 0; k < 3; ++k) o[k] = pc[k] + x * f[3 * k] + y * f[3 * k + 1];
+            cone_direction(a, phi, u);
+        };
+        auto cand = [&](const double c[3], const double g[9], double chw, double chh, RectCand& R) {
+            rect_cand_init(R, c, chw, chh, g);
+        };
+        for (int i = 0; i < 1000; ++i) {  // (i) reachable rectangles
+            double o[3], u[3];
+            ray(o, u);
+            const double t = std::exp(uni(std::log(1e-2), std::log(400.0)));
+            const double q[3] = {o[0] + t * u[0], o[1] + t * u[1], o[2] + t * u[2]};
+            double g[9];
+            random_frame(g);
+            const double chw = uni(0.2, 6.0), chh = uni(0.2, 6.0), x = uni(-chw, chw), y = uni(-chh, chh);
+            double c[3];
+            for (int k = 0; k < 3; ++k) c[k] = q[k] - x * g[3 * k] - y * g[3 * k + 1];  // q on it, anywhere
+            RectCand R;
+            cand(c, g, chw, chh, R);
+            ++reach;
+            if (rect_sweep_misses_rect(S, R)) ++reach_dropped;
+        }
+        for (int i = 0; i < 100; ++i) {  // (ii) random rectangles near R
+            const double c[3] = {pc[0] + uni(-40, 40), pc[1] + uni(-40, 40), pc[2] + uni(-40, 40)};
+            double g[9];
+            random_frame(g);
+            const double chw = uni(0.2, 6.0), chh = uni(0.2, 6.0);
+            RectCand R;
+            cand(c, g, chw, chh, R);
+            ++rects;
+            if (!rect_sweep_misses_rect(S, R)) continue;
+            ++dropped;
+            // how many of the dropped ones the box test kept: the gain of the rectangle test
+            double h[3];
+            for (int k = 0; k < 3; ++k)
+                h[k] = std::fabs(g[3 * k]) * chw + std::fabs(g[3 * k + 1]) * chh;
+            float b[6];
+            box_around(c, h, b);
+            if (!rect_sweep_misses(S, b)) ++box_kept;
+            for (int r = 0; r < 1000; ++r) {
+                double o[3], u[3];
+                ray(o, u);
+                if (ray_hits_rect(o, u, c, g, chw, chh)) {
+                    ++dropped_hit;
+                    break;
+                }
+            }
+        }
+    }
+    std::printf("sweeprect reachable %ld reachable_dropped %ld rects %ld dropped %ld dropped_box_kept %ld "
+                "dropped_but_hit %ld\n",
+                reach, reach_dropped, rects, dropped, box_kept, dropped_hit);
     return 0;
 }
 
@@ -318,13 +404,14 @@ int check_grid(double tan_scale, bool flat) {
 
 int main(int argc, char** argv) {
     if (argc != 4) {
-        std::fprintf(stderr, "usage: cull_check cone|sweep|grid|gridbox <seed> <scale>\n");
+        std::fprintf(stderr, "usage: cull_check cone|sweep|sweeprect|grid|gridbox <seed> <scale>\n");
         return 2;
     }
     rng.seed(std::strtoull(argv[2], nullptr, 10));
     const double scale = std::atof(argv[3]);
     if (!std::strcmp(argv[1], "cone")) return check_cone(scale);
     if (!std::strcmp(argv[1], "sweep")) return check_sweep(scale);
+    if (!std::strcmp(argv[1], "sweeprect")) return check_sweep_rect(scale);
     if (!std::strcmp(argv[1], "grid")) return check_grid(scale, true);
     if (!std::strcmp(argv[1], "gridbox")) return check_grid(scale, false);
     return 2;

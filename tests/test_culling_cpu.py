@@ -8,7 +8,8 @@ the same functions the list-building kernels run):
 * rect_sweep_misses (first- and second-reflection lists of flat mirrors): no
   box around a point that a ray from the mirror's rectangle, within the
   reflection cone, passes through is ever dropped, and every dropped box is
-  missed by thousands of such rays;
+  missed by thousands of such rays; rect_sweep_misses_rect, the same for flat
+  candidates' rectangles;
 * grid_range / cell_meets (sun-plane grid cells): every ray that hits a flat
   primitive's rectangle, or enters a curved primitive's traversal box, traced
   back within the sun cone to the sun plane, starts in a listed cell.
@@ -62,6 +63,22 @@ def test_reflection_sweep_culling_is_conservative(exe, seed):
 
 def test_reflection_sweep_check_catches_a_narrow_cone(exe):
     r = run(exe, "sweep", 1, 0.5)
+    assert r["reachable_dropped"] > 0 and r["dropped_but_hit"] > 0, r
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_reflection_sweep_rectangle_culling_is_conservative(exe, seed):
+    """rect_sweep_misses_rect (flat candidates culled by their rectangle, not
+    their box): rectangles through points that rays pass are never dropped, and
+    dropped rectangles are missed by every sampled ray; some rectangles whose
+    box the box test keeps are dropped."""
+    r = run(exe, "sweeprect", seed, 1.0)
+    assert r["reachable"] > 100_000 and r["reachable_dropped"] == 0, r
+    assert r["dropped"] > 10_000 and r["dropped_but_hit"] == 0 and r["dropped_box_kept"] > 0, r
+
+
+def test_reflection_sweep_rectangle_check_catches_a_narrow_cone(exe):
+    r = run(exe, "sweeprect", 1, 0.5)
     assert r["reachable_dropped"] > 0 and r["dropped_but_hit"] > 0, r
 
 
