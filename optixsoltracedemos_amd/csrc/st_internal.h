@@ -154,7 +154,8 @@ hipError_t launch_grid_fill(hipStream_t s, const float* bounds, const float4* pr
 // element pair_owner[t] and hit primitive pair_items[t]. The list goes to
 // scratch[t * kMaxBounceList ...]; counts[t] gets its length (kNoBounceList:
 // none) and, when range is given, range[t] = (t * kMaxBounceList, + length) or
-// (1, 0) for none, so the scratch serves as the item array directly.
+// (1, 0) for none, so the scratch serves as the item array directly. First-
+// reflection tasks run in BVH leaf order (their outputs stay at t).
 // launch_list_compact packs counted lists at items[range[t].x ...].
 constexpr uint32_t kNoBounceList = 0xFFFFFFFFu;
 constexpr uint32_t kMaxBounceList = 64;

@@ -1607,9 +1607,14 @@ __global__ __launch_bounds__(kConeBlock) void k_cone_lists(const st_element_f64*
                              const int32_t* __restrict__ pair_items, const int32_t* __restrict__ pair_owner,
                              uint32_t* __restrict__ counts, uint2* __restrict__ range, int32_t* __restrict__ scratch) {
     __shared__ int32_t lds_stack[(kStackDepth + 1) * kConeBlock];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= tasks) return;
-    const int k = t / tasks_per_step;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= tasks) return;
+    const int k = g / tasks_per_step;
+    // first reflections: thread g computes the list of the element whose
+    // primitive sits at leaf slot g % n, so a wave's 64 mirrors are BVH
+    // neighbours, whose cones walk nearly the same nodes; the list still goes
+    // to task t = k * n + element
+    const int t = pair_items ? g : k * tasks_per_step + (int)meta_elem(meta[g - k * tasks_per_step]);
     const double* frames = frames_all + (size_t)k * n * 9;
     const float* bounds_slot = bounds_all + (size_t)k * n * 6;
     const float4* nodes = nodes_all + (size_t)k * n_nodes * 4;
