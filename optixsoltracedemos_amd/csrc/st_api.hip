@@ -96,9 +96,9 @@ LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kerne
         sh.stack = bvh_depth <= 16 && sh.block >= 128 ? 16 : 32;
         sh.waves = sh.queued ? (sh.stack == 16 ? 5 : 3) : (sh.stack == 16 ? 6 : 4);
         if (listed && sh.queued && waves <= 0 && bvh_depth <= kStackDepth &&
-            trace_variant_exists(sh.block, 0, 8, true)) {
+            trace_variant_exists(sh.block, 0, 6, true)) {
             sh.stack = 0;
-            sh.waves = 8;
+            sh.waves = 6;
         }
     }
     if (waves > 0 && !use_tuned) {
@@ -1290,6 +1290,9 @@ static int batch_plan_lists(st_scene s, hipStream_t st) {
         ST_TRY(b.d_frames.ensure(sizeof(double) * 9 * list_tasks, "batch frames"));
         ST_TRY(b.d_bounce_range.ensure(sizeof(uint2) * list_tasks, "batch reflection ranges"));
         ST_TRY(b.d_bounce_scratch.ensure(sizeof(int32_t) * kMaxBounceList * list_tasks, "batch reflection lists"));
+        ST_TRY(b.d_sup_spheres.ensure(sizeof(double4) * (size_t)n, "batch element spheres"));
+        ST_TRY(b.d_sup_range.ensure(sizeof(uint2) * (size_t)n, "batch re-aim list ranges"));
+        ST_TRY(b.d_sup_scratch.ensure(sizeof(int32_t) * kMaxBounceList * (size_t)n, "batch re-aim lists"));
     }
     const uint64_t grid_bytes = (uint64_t)K * (3 * kGridCells + 1) * sizeof(uint32_t);
     if (grid_bytes > kBatchListBytes / 4) return ST_OK;
@@ -1340,14 +1343,23 @@ static int batch_enqueue(st_scene s, hipStream_t st) {
                                  (uint32_t*)b.d_grid_cursor.ptr, (int32_t*)b.d_grid_items.ptr, b.grid_cap),
                 "batched grid fill");
     }
-    if (b.lists)
+    if (b.lists) {
+        // every mirror is re-aimed at the target: candidates over all re-aims
+        // once, then each step keeps those its cones reach
+        HIP_TRY(launch_reaim_lists(st, (const st_element_f64*)s->d_elems.ptr, n, (const int32_t*)s->d_elem_slot.ptr,
+                                   (const uint32_t*)s->d_meta.ptr, (const double*)b.d_target.ptr,
+                                   sun_cone_angle(b.sun), (double4*)b.d_sup_spheres.ptr, (uint2*)b.d_sup_range.ptr,
+                                   (int32_t*)b.d_sup_scratch.ptr),
+                "re-aim candidate lists");
         HIP_TRY(launch_cone_lists(st, (const st_element_f64*)s->d_elems.ptr, K * n, n, n, s->n_nodes,
                                   (const int32_t*)s->d_elem_slot.ptr, (const uint32_t*)s->d_meta.ptr,
                                   (const double*)b.d_frames.ptr, (const float*)b.d_bounds.ptr,
                                   (const float4*)b.d_nodes.ptr, (const SunFrame*)b.d_sun.ptr, sun_cone_angle(b.sun),
                                   nullptr, nullptr, nullptr, (uint2*)b.d_bounce_range.ptr,
-                                  (int32_t*)b.d_bounce_scratch.ptr),
+                                  (int32_t*)b.d_bounce_scratch.ptr, (const uint2*)b.d_sup_range.ptr,
+                                  (const int32_t*)b.d_sup_scratch.ptr),
                 "batched reflection lists");
+    }
     JobDev dummy;
     memset(&dummy, 0, sizeof(dummy));
     HIP_TRY(launch_trace(st, (const JobDev*)b.d_jobs.ptr, (const uint64_t*)b.d_prefix.ptr, K, b.total_chunks,

@@ -39,8 +39,9 @@ struct DevBuf {
 // An explicit waves-per-SIMD budget takes the smallest compiled stack that holds
 // the BVH; the caller checks that the variant exists (trace_variant_exists).
 // listed: the launch's reflected rays take candidate lists, so its BVH walks
-// are rare and the default queued shape keeps the stack in private memory (8
-// waves per SIMD instead of 5).
+// are rare and the default queued shape keeps the stack in private memory (6
+// waves per SIMD with 80 registers instead of 5; the 8-wave variants' 64
+// registers spill more than the extra waves repay).
 LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel, int32_t bvh_depth,
                        const LaunchShape* tuned, int32_t min_elements, int32_t waves = 0, bool listed = false);
 
@@ -66,6 +67,11 @@ struct BatchCache {
     uint32_t grid_cap = 0;
     DevBuf d_frames, d_grid_counts, d_grid_start, d_grid_cursor, d_grid_stats, d_grid_items, d_bounce_range,
         d_bounce_scratch;
+    // first-reflection candidates over every re-aim at the batch's target (the
+    // reflection axis of a re-aimed mirror points at the target whatever the
+    // sun), rebuilt inside every run; each step's lists test these instead of
+    // walking the BVH
+    DevBuf d_sup_spheres, d_sup_range, d_sup_scratch;
     // per-step copies of small flux maps (route_flux_copies)
     DevBuf d_flux_copies;
     uint32_t flux_copies = 1;

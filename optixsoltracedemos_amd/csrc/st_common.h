@@ -210,6 +210,60 @@ ST_INLINE U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
     return c;
 }
 
+// cell of the stratified sun-plane sampling: 64 consecutive rays per cell of a
+// 256 x 256 grid, cells in a keyed 16-bit Feistel order within each sweep
+ST_INLINE uint32_t stratified_cell_of_group(uint64_t group, uint32_t seed_lo, uint32_t seed_hi, uint32_t job) {
+    const uint64_t sweep = group >> 16;
+    const U4 k = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), 0x53545254u, job}, seed_lo, seed_hi);
+    const uint32_t keys[4] = {k.x, k.y, k.z, k.w};
+    uint32_t L = ((uint32_t)group >> 8) & 0xFFu, R = (uint32_t)group & 0xFFu;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t F = ((R ^ keys[r]) * 0x9E3779B1u) >> 24;
+        const uint32_t nl = R;
+        R = L ^ F;
+        L = nl;
+    }
+    return (L << 8) | R;
+}
+// The inverse: the group of sweep `sweep` (bits 16 and up of the result) whose
+// cell is `cell` (the Feistel rounds undone in reverse order).
+ST_INLINE uint64_t stratified_group_of_cell(uint64_t sweep, uint32_t cell, uint32_t seed_lo, uint32_t seed_hi,
+                                            uint32_t job) {
+    const U4 k = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), 0x53545254u, job}, seed_lo, seed_hi);
+    const uint32_t keys[4] = {k.x, k.y, k.z, k.w};
+    uint32_t L = (cell >> 8) & 0xFFu, R = cell & 0xFFu;
+#pragma unroll
+    for (int r = 3; r >= 0; --r) {
+        const uint32_t pr = L;
+        L = R ^ (((L ^ keys[r]) * 0x9E3779B1u) >> 24);
+        R = pr;
+    }
+    return (sweep << 16) | (L << 8) | R;
+}
+// Cell (y << 8 | x) at position h of a Hilbert curve over the 256 x 256 grid
+// (the quadrant-by-quadrant construction: each step reads two bits of h and
+// rotates or reflects the sub-square it enters).
+ST_INLINE uint32_t hilbert_cell(uint32_t h) {
+    uint32_t x = 0, y = 0;
+    for (uint32_t s = 1; s < 256u; s <<= 1) {
+        const uint32_t rx = 1u & (h >> 1), ry = 1u & (h ^ rx);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = s - 1 - x;
+                y = s - 1 - y;
+            }
+            const uint32_t tmp = x;
+            x = y;
+            y = tmp;
+        }
+        x += s * rx;
+        y += s * ry;
+        h >>= 2;
+    }
+    return (y << 8) | x;
+}
+
 // 23-bit uniform ((x >> 9) + 0.5) / 2^23 in (0,1); u and 1 - u are exact in FP32
 ST_INLINE float u01(uint32_t x) { return ((float)(x >> 9) + 0.5f) * (1.0f / 8388608.0f); }
 

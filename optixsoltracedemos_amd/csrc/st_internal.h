@@ -28,8 +28,9 @@ struct JobDev {
     int32_t flux_slot;       // primitive slot of the flux element (-1 none)
     int32_t agg_slot;        // receiver whose hit count is kept in registers (-1 none)
     int32_t flux_nx, flux_ny;
-    uint32_t order_head;     // Halton trace order (set_trace_order): slots before the first whole block
-    uint32_t order_blocks;   // whole blocks of kHaltonBlock indices traced cell by cell (0: index order)
+    uint32_t order_head;     // trace order (set_trace_order): slots before the first whole block
+    uint32_t order_blocks;   // whole blocks traced cell by cell (0: index order)
+    uint32_t order_kind;     // kOrderHalton or kOrderStratified (with order_blocks > 0)
     uint32_t flux_bins;      // flux_nx * flux_ny
     uint32_t flux_copy_mask; // flux holds flux_copy_mask + 1 copies of the map; wave w adds to copy w & mask
     int64_t* flux;
@@ -56,15 +57,28 @@ constexpr uint32_t kHaltonCells = 256u * 243u, kHaltonBlock = 64u * kHaltonCells
 // Halton positions take the 32-bit ray index (the reference's unsigned launch
 // index, cuda/sun.cu:94-134): jobs whose ray range passes 2^32 are rejected
 constexpr uint64_t kHaltonMaxRays = 1ull << 32;
+// Stratified positions: a sweep of kStratSweep rays gives each of the 65,536
+// sun-plane cells one group of 64 rays, cells in a keyed Feistel order (so that
+// every prefix of a sweep samples the plane evenly). A job's whole sweeps are
+// traced in Hilbert-curve order of the cells instead, so that the groups a wave
+// traces one after the other lie side by side and their reflected rays walk
+// similar lists and nodes (jobs whose range starts on a group boundary).
+constexpr uint64_t kStratSweep = 64ull << 16;
+constexpr uint32_t kOrderHalton = 1, kOrderStratified = 2;
 inline void set_trace_order(JobDev* J, int32_t position_sampling) {
     J->order_head = 0;
     J->order_blocks = 0;
+    J->order_kind = 0;
     const uint64_t b = J->ray_begin, e = J->ray_begin + J->ray_count;
-    if (position_sampling != ST_POS_HALTON || e > (1ull << 32)) return;
-    const uint64_t b1 = (b + kHaltonBlock - 1) / kHaltonBlock * kHaltonBlock, e1 = e / kHaltonBlock * kHaltonBlock;
-    if (e1 <= b1) return;
+    uint64_t blk = 0;
+    if (position_sampling == ST_POS_HALTON && e <= (1ull << 32)) blk = kHaltonBlock;
+    if (position_sampling == ST_POS_STRATIFIED && b % 64 == 0) blk = kStratSweep;
+    if (!blk) return;
+    const uint64_t b1 = (b + blk - 1) / blk * blk, e1 = e / blk * blk;
+    if (e1 <= b1 || (e1 - b1) / blk >= (1ull << 32)) return;
     J->order_head = (uint32_t)(b1 - b);
-    J->order_blocks = (uint32_t)((e1 - b1) / kHaltonBlock);
+    J->order_blocks = (uint32_t)((e1 - b1) / blk);
+    J->order_kind = blk == kHaltonBlock ? kOrderHalton : kOrderStratified;
 }
 
 // a grid is used unless its lists grow long: a wave whose rays share a cell
@@ -156,6 +170,9 @@ hipError_t launch_grid_fill(hipStream_t s, const float* bounds, const float4* pr
 // none) and, when range is given, range[t] = (t * kMaxBounceList, + length) or
 // (1, 0) for none, so the scratch serves as the item array directly. First-
 // reflection tasks run in BVH leaf order (their outputs stay at t).
+// sup_range / sup_items: candidate lists (per element) holding every primitive
+// an element's first reflections reach, whose boxes first-reflection tasks
+// test instead of walking the BVH.
 // launch_list_compact packs counted lists at items[range[t].x ...].
 constexpr uint32_t kNoBounceList = 0xFFFFFFFFu;
 constexpr uint32_t kMaxBounceList = 64;
@@ -163,7 +180,14 @@ hipError_t launch_cone_lists(hipStream_t s, const st_element_f64* elems, int32_t
                              int32_t n, int32_t n_nodes, const int32_t* elem_slot, const uint32_t* meta,
                              const double* frames, const float* bounds_slot, const float4* nodes, const SunFrame* sun,
                              double theta_sun, const int32_t* pair_items, const int32_t* pair_owner, uint32_t* counts,
-                             uint2* range, int32_t* scratch);
+                             uint2* range, int32_t* scratch, const uint2* sup_range = nullptr,
+                             const int32_t* sup_items = nullptr);
+// first-reflection candidates of every mirror over all bisector re-aims at
+// target (k_reaim_lists): range[e] / scratch[e * kMaxBounceList ...] as
+// k_cone_lists writes them, by element; spheres: n double4 of scratch
+hipError_t launch_reaim_lists(hipStream_t s, const st_element_f64* elems, int32_t n, const int32_t* elem_slot,
+                              const uint32_t* meta, const double* target, double theta_sun, double4* spheres,
+                              uint2* range, int32_t* scratch);
 hipError_t launch_list_compact(hipStream_t s, int32_t tasks, const uint2* range, const int32_t* scratch,
                                int32_t* items);
 hipError_t launch_fill_u64(hipStream_t s, void* p, uint64_t n_u64, uint64_t value);

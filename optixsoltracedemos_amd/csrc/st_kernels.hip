@@ -149,24 +149,6 @@ __device__ __forceinline__ float halton3(uint32_t i) {
     return (float)((double)num / 847288609443.0);  // 3^25
 }
 
-// cell of the stratified sun-plane sampling: 64 consecutive rays per cell of a
-// 256 x 256 grid, cells in a keyed 16-bit Feistel order within each sweep
-__device__ __forceinline__ uint32_t stratified_cell_of_group(uint64_t group, uint32_t seed_lo, uint32_t seed_hi,
-                                                             uint32_t job) {
-    const uint64_t sweep = group >> 16;
-    const U4 k = philox4x32_10(U4{(uint32_t)sweep, (uint32_t)(sweep >> 32), 0x53545254u, job}, seed_lo, seed_hi);
-    const uint32_t keys[4] = {k.x, k.y, k.z, k.w};
-    uint32_t L = ((uint32_t)group >> 8) & 0xFFu, R = (uint32_t)group & 0xFFu;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const uint32_t F = ((R ^ keys[r]) * 0x9E3779B1u) >> 24;
-        const uint32_t nl = R;
-        R = L ^ F;
-        L = nl;
-    }
-    return (L << 8) | R;
-}
-
 // ---------------------------------------------------------------------------
 // Intersections (FP32). Same root selection and bounds rules as the
 // reference programs; the roots are evaluated in cancellation-free form.
@@ -659,7 +641,7 @@ __device__ __forceinline__ void record(const JobDev& J, uint64_t ray, int slot, 
 // Sun sample of one ray: origin on the sun plane, direction in the sun cone
 // (cuda/sun.cu:94-134). Counts the ray and writes its record slot 0.
 __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame& S, uint64_t ray, uint32_t* wc,
-                                        V3& o, V3& d, uint32_t& cell) {
+                                        V3& o, V3& d, uint32_t& cell, uint32_t cell_hint = 0xFFFFFFFFu) {
     const uint32_t rlo = (uint32_t)ray, rhi = (uint32_t)(ray >> 32);
     // ---- sun sample: cuda/sun.cu:94-134
     const U4 u = philox4x32_10(U4{rlo, rhi, 0u, J.job_id}, J.seed_lo, J.seed_hi);
@@ -671,7 +653,8 @@ __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame
         const uint64_t g = ray >> 6;
         const uint64_t g0 = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(g >> 32)) << 32) |
                             (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)g);
-        cell = stratified_cell_of_group(g0, J.seed_lo, J.seed_hi, J.job_id);
+        // cell_hint: the caller's cell of group g0
+        cell = cell_hint != 0xFFFFFFFFu ? cell_hint : stratified_cell_of_group(g0, J.seed_lo, J.seed_hi, J.job_id);
         if (g != g0) cell = stratified_cell_of_group(g, J.seed_lo, J.seed_hi, J.job_id);
         a = ((float)(cell & 255u) + u01(u.x)) * (1.f / 256.f);
         b = ((float)(cell >> 8) + u01(u.y)) * (1.f / 256.f);
@@ -875,12 +858,24 @@ __device__ __forceinline__ bool shade_mirror(const JobDev& J, uint64_t ray, int 
 }
 
 // Ray index of slot s of a job's range; slots are traced in order. Inside the
-// range's whole blocks of kHaltonBlock aligned indices (Halton positions only,
-// order_blocks > 0) a block is traced cell by cell: its 64 slots 64k..64k+63
-// hold the indices base + k + j * kHaltonCells, j < 64, which share a Halton
-// cell. Elsewhere slot s is index ray_begin + s.
+// range's whole blocks of kHaltonBlock aligned indices (Halton positions,
+// kOrderHalton) a block is traced cell by cell: its 64 slots 64k..64k+63 hold
+// the indices base + k + j * kHaltonCells, j < 64, which share a Halton cell.
+// Inside whole stratified sweeps (kOrderStratified) slots 64k..64k+63 hold the
+// group whose cell is the k-th of the sweep's Hilbert curve. Elsewhere slot s
+// is index ray_begin + s. Either way the 64 slots from any multiple of 64 hold
+// consecutive indices when the order is stratified.
 __device__ __forceinline__ uint64_t ray_of_slot(const JobDev& J, uint64_t s) {
-    if (J.order_blocks && s >= J.order_head) {
+    if (J.order_kind == kOrderStratified && s >= J.order_head) {
+        const uint64_t t = s - J.order_head;
+        if ((t >> 22) < (uint64_t)J.order_blocks) {
+            const uint64_t sweep = ((J.ray_begin + J.order_head) >> 22) + (t >> 22);
+            const uint64_t g = stratified_group_of_cell(sweep, hilbert_cell((uint32_t)(t >> 6) & 0xFFFFu), J.seed_lo,
+                                                        J.seed_hi, J.job_id);
+            return (g << 6) | (t & 63u);
+        }
+    }
+    if (J.order_kind == kOrderHalton && s >= J.order_head) {
         const uint64_t t = s - J.order_head;
         if (t < (uint64_t)J.order_blocks * kHaltonBlock) {
             const uint32_t u = (uint32_t)t;  // the range lies below 2^32
@@ -890,6 +885,25 @@ __device__ __forceinline__ uint64_t ray_of_slot(const JobDev& J, uint64_t s) {
         }
     }
     return J.ray_begin + s;
+}
+
+// First ray index of the 64 slots from s (a multiple of 64 slots in a job whose
+// order is not Halton's: they hold consecutive indices) and, for stratified
+// positions, the sun-plane cell of their group (else 0xFFFFFFFF).
+__device__ __forceinline__ uint64_t group_of_slot(const JobDev& J, const ST_CONST SunFrame& S, uint64_t s,
+                                                  uint32_t* cell) {
+    *cell = 0xFFFFFFFFu;
+    if (J.order_kind == kOrderStratified && s >= J.order_head) {
+        const uint64_t t = s - J.order_head;
+        if ((t >> 22) < (uint64_t)J.order_blocks) {
+            const uint64_t sweep = ((J.ray_begin + J.order_head) >> 22) + (t >> 22);
+            *cell = hilbert_cell((uint32_t)(t >> 6) & 0xFFFFu);
+            return stratified_group_of_cell(sweep, *cell, J.seed_lo, J.seed_hi, J.job_id) << 6;
+        }
+    }
+    const uint64_t r = J.ray_begin + s;
+    if (S.position_sampling == ST_POS_STRATIFIED) *cell = stratified_cell_of_group(r >> 6, J.seed_lo, J.seed_hi, J.job_id);
+    return r;
 }
 
 // empty slots are all-zero (the reference memsets the buffer, lib/soltrace_system.cpp:125)
@@ -940,6 +954,13 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
     __shared__ float lds_queue[QUEUED ? BLOCK / 64 : 1][QUEUED ? 8 * kQueueCap : 1];
     __shared__ uint32_t lds_log[BLOCK / 64][1 + kLogCap];
     __shared__ uint32_t lds_hits[LHITS > 0 ? BLOCK / 64 : 1][LHITS > 0 ? LHITS : 1];
+    // queued form (not the small-scene LDS-hit variants): first ray index and
+    // stratified cell of each group of 64 slots of the chunk, computed for the
+    // whole chunk at its start, 32 groups in 32 lanes (every order but
+    // Halton's, whose groups' indices are not consecutive)
+    constexpr bool TABLE = QUEUED && LHITS == 0;
+    __shared__ uint64_t lds_bases[TABLE ? BLOCK / 64 : 1][TABLE ? kMaxChunkRays / 64 : 1];
+    __shared__ uint32_t lds_cells[TABLE ? BLOCK / 64 : 1][TABLE ? kMaxChunkRays / 64 : 1];
     int32_t private_stack[STACK > 0 ? 1 : kStackDepth];
     int32_t* stk = STACK > 0 ? lds_stack + threadIdx.x : private_stack;
     const int lane = threadIdx.x & 63;
@@ -964,6 +985,21 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
         uint32_t len = 0;  // queued form: entries in the wave's log (wave-uniform)
         if (QUEUED) {
             float* q = lds_queue[threadIdx.x >> 6];
+            uint64_t* bases = lds_bases[TABLE ? threadIdx.x >> 6 : 0];
+            uint32_t* cells = lds_cells[TABLE ? threadIdx.x >> 6 : 0];
+            const bool table = TABLE && J.order_kind != kOrderHalton;
+            if (table && (uint32_t)lane < (kMaxChunkRays / 64)) {
+                const uint64_t s0 = first + 64u * (uint32_t)lane;
+                if (s0 < end) {
+                    uint32_t c0;
+                    bases[lane] = group_of_slot(J, S, s0, &c0);
+                    cells[lane] = c0;
+                }
+            }
+            // ray index of the slot at offset u of this chunk
+            auto chunk_ray = [&](uint32_t u) -> uint64_t {
+                return table ? bases[u >> 6] + (u & 63u) : ray_of_slot(J, first + u);
+            };
             uint32_t qn = 0;  // wave-uniform queue length, < kQueueCap between steps
             bool alive = false;  // this lane holds a ray with a next segment
             V3 o = v3(0.f, 0.f, 0.f), d = o;
@@ -1010,7 +1046,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                     const bool bvh = alive && L.lb > L.le;
                     const unsigned long long bm = __ballot(bvh);
                     if (alive && bvh == (__popcll(bm) >= 32 || bm == __ballot(alive))) {
-                        const uint64_t ray = ray_of_slot(J, first + off);
+                        const uint64_t ray = chunk_ray(off);
                         int last = depth;
                         float tmin = kTminBounce;
                         alive = shade_mirror(J, ray, pend, depth, o, d, wc) &&
@@ -1020,7 +1056,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                     }
                 } else {
                     if (alive) {
-                        const uint64_t ray = ray_of_slot(J, first + off);
+                        const uint64_t ray = chunk_ray(off);
                         int last = depth;
                         float tmin = kTminBounce;
                         alive = shade_mirror(J, ray, pend, depth, o, d, wc) &&
@@ -1036,10 +1072,15 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                 const uint64_t i = b + (uint64_t)lane;
                 alive = false;
                 uint32_t ph = kNoEntry, pf = kNoEntry;
+                // the group's first index and cell from the chunk's table (b is a
+                // multiple of 64 slots)
+                const uint32_t gi = (uint32_t)(b - first) >> 6;
+                const uint64_t base = table ? bases[gi] : 0;
+                const uint32_t hint = table ? cells[gi] : 0xFFFFFFFFu;
                 if (i < end) {
-                    const uint64_t ray = ray_of_slot(J, i);
+                    const uint64_t ray = table ? base + (uint64_t)lane : ray_of_slot(J, i);
                     uint32_t cell;
-                    sun_ray(J, S, ray, wc, o, d, cell);
+                    sun_ray(J, S, ray, wc, o, d, cell, hint);
                     float tmin = kTminPrimary;
                     int last = 0;
                     depth = 0;
@@ -1085,8 +1126,9 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
                 // find it full go out as direct atomics
                 if (wl[0] > kLogCap / 2) log_flush(wl, J);
                 const uint64_t i = i0 + (uint64_t)lane;
+                const uint64_t base = J.order_kind == kOrderHalton ? 0 : ray_of_slot(J, i0);
                 if (i >= end) continue;
-                const uint64_t ray = ray_of_slot(J, i);
+                const uint64_t ray = J.order_kind == kOrderHalton ? ray_of_slot(J, i) : base + (uint64_t)lane;
                 V3 o, d;
                 uint32_t cell;
                 sun_ray(J, S, ray, wc, o, d, cell);
@@ -1507,6 +1549,14 @@ __device__ __forceinline__ bool bounce_cone(const st_element_f64& E, double thet
     return p < 0.35;
 }
 
+// whether a ray leaving box hb in a direction within the cone C can reach the
+// box [lo, hi]
+__device__ __forceinline__ bool cone_reaches(const Cone& C, const float* hb, const float lo[3], const float hi[3]) {
+    const double dlo[3] = {(double)lo[0] - hb[3], (double)lo[1] - hb[4], (double)lo[2] - hb[5]};
+    const double dhi[3] = {(double)hi[0] - hb[0], (double)hi[1] - hb[1], (double)hi[2] - hb[2]};
+    return cone_meets_box(C, dlo, dhi);
+}
+
 // Primitives whose boxes a ray leaving box hb in a direction within the cone C
 // can reach, found by a BVH walk; the first `cap` are written to items[0 ...].
 // Returns the count, or cap + 1 as soon as it exceeds cap. stack: this lane's
@@ -1516,9 +1566,7 @@ __device__ __forceinline__ uint32_t cone_walk(const float4* __restrict__ nodes, 
                                               const float* hb, const Cone& C, int32_t* __restrict__ items,
                                               uint32_t cap, int32_t* __restrict__ stack) {
     auto meets = [&](const float lo[3], const float hi[3]) __attribute__((always_inline)) {
-        const double dlo[3] = {(double)lo[0] - hb[3], (double)lo[1] - hb[4], (double)lo[2] - hb[5]};
-        const double dhi[3] = {(double)hi[0] - hb[0], (double)hi[1] - hb[1], (double)hi[2] - hb[2]};
-        return cone_meets_box(C, dlo, dhi);
+        return cone_reaches(C, hb, lo, hi);
     };
     uint32_t count = 0;  // the stack holds <= one pending sibling per level (depth <= 31)
     int sp = 0;
@@ -1589,6 +1637,126 @@ __device__ __forceinline__ uint32_t cull_by_rectangle(const st_element_f64* __re
     return kept;
 }
 
+// Radius of the sphere around element E's origin that holds its surface in
+// every orientation: the distance to the farthest corner of its local box
+// (paraboloid sag and cylinder depth included).
+__device__ __forceinline__ double element_radius(const st_element_f64& E) {
+    const double w = E.dim_x, h = E.dim_y;
+    double zlo = 0, zhi = 0;
+    if (E.surface == ST_SURFACE_PARABOLIC) {
+        const double sx = 0.5 * E.curv_x * (w / 2) * (w / 2), sy = 0.5 * E.curv_y * (h / 2) * (h / 2);
+        zlo = fmin(0.0, sx) + fmin(0.0, sy);
+        zhi = fmax(0.0, sx) + fmax(0.0, sy);
+    } else if (E.surface == ST_SURFACE_CYLINDER) {
+        zlo = -w / 2;
+        zhi = w / 2;
+    }
+    return sqrt(0.25 * w * w + 0.25 * h * h + fmax(zlo * zlo, zhi * zhi));
+}
+
+// Bounding spheres of the elements (origin, element_radius + 1 mm), for
+// k_reaim_lists
+__global__ void k_element_spheres(const st_element_f64* __restrict__ elems, int32_t n, double4* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const st_element_f64& E = elems[e];
+    out[e] = make_double4(E.origin[0], E.origin[1], E.origin[2], element_radius(E) + 1e-3);
+}
+
+// First-reflection candidates of mirrors that every state re-aims at `target`
+// by the bisector rule. Such a mirror's reflection axis is the direction t
+// from its origin to the target whatever the sun, so its first reflections
+// leave its sphere (k_element_spheres) in directions within phi (bounce_cone)
+// of t, and reach element B only if B's sphere subtends that cone: the angle
+// between t and d = c_B - c_A is at most phi + asin(R / |d|), R = r_A + r_B (or
+// |d| <= R). The spheres' 1 mm and phi's 1e-6 rad margins exceed the FP32
+// rounding of the traced rays. One wave per mirror tests every element, the
+// block's spheres staged through LDS; the list (flat candidates first, header
+// = their count, as k_cone_lists writes them) goes to scratch[A *
+// kMaxBounceList ...], range[A] = its (begin, end), (1, 0) for none or longer
+// than kMaxBounceList.
+constexpr int kReaimBlock = 1024;
+__global__ __launch_bounds__(kReaimBlock) void k_reaim_lists(const st_element_f64* __restrict__ elems, int32_t n,
+                                                             const double4* __restrict__ spheres,
+                                                             const int32_t* __restrict__ elem_slot,
+                                                             const uint32_t* __restrict__ meta,
+                                                             const double* __restrict__ target, double theta_sun,
+                                                             uint2* __restrict__ range, int32_t* __restrict__ scratch) {
+    __shared__ double4 lds_spheres[kReaimBlock];
+    __shared__ int32_t lds_items[kReaimBlock / 64][kMaxBounceList];
+    const int lane = threadIdx.x & 63;
+    const int A = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    int32_t* items = lds_items[threadIdx.x >> 6];
+    double t[3] = {0, 0, 0}, cp = 1.0, sp = 0.0;
+    double4 a = make_double4(0, 0, 0, 0);
+    bool active = false;  // wave-uniform: A is a mirror with a cone
+    if (A < n) {
+        const st_element_f64& E = elems[A];
+        double phi = 0.0;
+        a = spheres[A];
+        const double v[3] = {target[0] - a.x, target[1] - a.y, target[2] - a.z};
+        const double lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        active = bounce_cone(E, theta_sun, &phi) && lv > 0.0;
+        if (active) {
+            for (int j = 0; j < 3; ++j) t[j] = v[j] / lv;
+            cp = cos(phi + 1e-6);
+            sp = sin(phi + 1e-6);
+        } else if (lane == 0) {
+            range[A] = make_uint2(1u, 0u);
+        }
+    }
+    uint32_t count = 0;
+    bool over = false;
+    for (int b0 = 0; b0 < n; b0 += kReaimBlock) {  // every wave of the block runs every tile
+        __syncthreads();
+        if (b0 + (int)threadIdx.x < n) lds_spheres[threadIdx.x] = spheres[b0 + threadIdx.x];
+        __syncthreads();
+        if (!active || over) continue;
+        const int tile = min(kReaimBlock, n - b0);
+        for (int j0 = 0; j0 < tile; j0 += 64) {
+            const int j = j0 + lane;
+            bool pass = false;
+            if (j < tile) {
+                const double4 c = lds_spheres[j];
+                const double d[3] = {c.x - a.x, c.y - a.y, c.z - a.z};
+                const double L = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                const double R = a.w + c.w + 1e-9 * (L + fabs(a.x) + fabs(a.y) + fabs(a.z));
+                if (L <= R) {
+                    pass = true;
+                } else {
+                    const double sn = R / L, cs = sqrt(fmax(0.0, 1.0 - sn * sn));
+                    pass = d[0] * t[0] + d[1] * t[1] + d[2] * t[2] >= L * (cp * cs - sp * sn);
+                }
+            }
+            const unsigned long long m = __ballot(pass);
+            const uint32_t cnt = (uint32_t)__popcll(m);
+            if (count + cnt > kMaxBounceList - 1) {
+                over = true;
+                break;
+            }
+            if (pass) items[count + lane_rank(m)] = elem_slot[b0 + j];
+            count += cnt;
+        }
+    }
+    if (!active) return;
+    if (over) {
+        if (lane == 0) range[A] = make_uint2(1u, 0u);
+        return;
+    }
+    // flat candidates first
+    int32_t* list = scratch + (size_t)A * kMaxBounceList;
+    const bool have = (uint32_t)lane < count;
+    const int32_t slot = have ? items[lane] : 0;
+    const bool flat = have && meta_kind(meta[slot]) == kKindFlat;
+    const unsigned long long mf = __ballot(flat), mc = __ballot(have && !flat);
+    const uint32_t nf = (uint32_t)__popcll(mf);
+    if (have) list[1 + (flat ? lane_rank(mf) : nf + lane_rank(mc))] = slot;
+    if (lane == 0) {
+        list[0] = (int32_t)nf;
+        range[A] = make_uint2((uint32_t)A * kMaxBounceList, (uint32_t)A * kMaxBounceList + count + 1);
+    }
+}
+
 // Task t of scene state k = t / tasks_per_step, i = t % tasks_per_step: first
 // reflections off element i (pair_items == null), or second reflections of
 // rays that left element pair_owner[t] (< 0: none) and hit the primitive
@@ -1597,6 +1765,10 @@ __device__ __forceinline__ uint32_t cull_by_rectangle(const st_element_f64* __re
 // counts[t] (if given) gets its length with that header, kNoBounceList for
 // none or longer than kMaxBounceList, and range[t] (if given) its (begin, end)
 // in the scratch, (1, 0) for none.
+// sup_range / sup_items (first reflections): candidate lists that hold every
+// primitive the element's first reflections can reach (k_reaim_lists); an
+// element that has one tests its candidates' boxes instead of walking the BVH
+// (the same leaf test, so the list holds every primitive the walk would).
 constexpr int kConeBlock = 64;
 __global__ __launch_bounds__(kConeBlock) void k_cone_lists(const st_element_f64* __restrict__ elems, int32_t tasks,
                                                            int32_t tasks_per_step,
@@ -1605,7 +1777,8 @@ __global__ __launch_bounds__(kConeBlock) void k_cone_lists(const st_element_f64*
                              const float* __restrict__ bounds_all, const float4* __restrict__ nodes_all,
                              const SunFrame* __restrict__ sun_all, double theta_sun,
                              const int32_t* __restrict__ pair_items, const int32_t* __restrict__ pair_owner,
-                             uint32_t* __restrict__ counts, uint2* __restrict__ range, int32_t* __restrict__ scratch) {
+                             uint32_t* __restrict__ counts, uint2* __restrict__ range, int32_t* __restrict__ scratch,
+                             const uint2* __restrict__ sup_range, const int32_t* __restrict__ sup_items) {
     __shared__ int32_t lds_stack[(kStackDepth + 1) * kConeBlock];
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= tasks) return;
@@ -1670,8 +1843,21 @@ __global__ __launch_bounds__(kConeBlock) void k_cone_lists(const st_element_f64*
     const double reach = 2.0 * sqrt(ex * ex + ey * ey + ez * ez) + 2.0 * kConeFirst;
     C.ratio = pow(reach / kConeFirst, 1.0 / (kConePieces - 2));
     int32_t* list = scratch + (size_t)t * kMaxBounceList;
-    uint32_t len = cone_walk<kConeBlock>(nodes, bounds_slot, bounds_slot + 6 * (size_t)elem_slot[from], C,
-                                         list + 1, kMaxBounceList - 1, lds_stack + threadIdx.x);
+    const float* hb = bounds_slot + 6 * (size_t)elem_slot[from];
+    uint32_t len = 0;
+    const uint2 sr = sup_range && !pair_items ? sup_range[from] : make_uint2(1u, 0u);
+    if (sr.x <= sr.y) {  // the element's candidates over every re-aimed state (entry sr.x is its header)
+        for (uint32_t i = sr.x + 1; i < sr.y && len < kMaxBounceList; ++i) {
+            const int32_t v = sup_items[i];
+            const float* b = bounds_slot + 6 * (size_t)v;
+            if (cone_reaches(C, hb, b, b + 3)) {
+                if (len < kMaxBounceList - 1) list[1 + len] = v;
+                ++len;
+            }
+        }
+    } else {
+        len = cone_walk<kConeBlock>(nodes, bounds_slot, hb, C, list + 1, kMaxBounceList - 1, lds_stack + threadIdx.x);
+    }
     if (len > kMaxBounceList - 1) {
         none();
         return;
@@ -1944,13 +2130,25 @@ hipError_t launch_cone_lists(hipStream_t s, const st_element_f64* elems, int32_t
                              int32_t n, int32_t n_nodes, const int32_t* elem_slot, const uint32_t* meta,
                              const double* frames, const float* bounds_slot, const float4* nodes, const SunFrame* sun,
                              double theta_sun, const int32_t* pair_items, const int32_t* pair_owner, uint32_t* counts,
-                             uint2* range, int32_t* scratch) {
+                             uint2* range, int32_t* scratch, const uint2* sup_range, const int32_t* sup_items) {
     if (tasks <= 0) return hipSuccess;
     if (tasks_per_step <= 0 || tasks % tasks_per_step != 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_cone_lists, dim3((tasks + kConeBlock - 1) / kConeBlock), dim3(kConeBlock), 0, s, elems, tasks,
                        tasks_per_step, n,
                        n_nodes, elem_slot, meta, frames, bounds_slot, nodes, sun, theta_sun, pair_items, pair_owner,
-                       counts, range, scratch);
+                       counts, range, scratch, sup_range, sup_items);
+    return hipGetLastError();
+}
+
+hipError_t launch_reaim_lists(hipStream_t s, const st_element_f64* elems, int32_t n, const int32_t* elem_slot,
+                              const uint32_t* meta, const double* target, double theta_sun, double4* spheres,
+                              uint2* range, int32_t* scratch) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_element_spheres, dim3((n + 255) / 256), dim3(256), 0, s, elems, n, spheres);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_reaim_lists, dim3((unsigned)(((int64_t)n * 64 + kReaimBlock - 1) / kReaimBlock)),
+                       dim3(kReaimBlock), 0, s, elems, n, spheres, elem_slot, meta, target, theta_sun, range, scratch);
     return hipGetLastError();
 }
 
