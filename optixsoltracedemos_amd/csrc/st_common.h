@@ -540,6 +540,23 @@ ST_INLINE bool rect_sweep_misses_rect(const RectSweep& S, const RectCand& R) {
 // Cone from the apex (origin) with unit axis a and half angle phi; pieces of
 // the cone are bounded by boxes over [0, first], then [t, t * ratio] up to the
 // far corner of the tested box (ratio fixed per launch from the scene size).
+// Whether a ray that leaves the sphere (ca, ra) in a direction within angle
+// phi of the unit axis t (cp = cos phi, sp = sin phi) can enter the sphere
+// (cb, rb): the directions from the first sphere's points to the second's
+// fill the cone of half-angle asin(R / |d|) around d = cb - ca (R = ra + rb,
+// and every direction when |d| <= R), which meets the ray cone when the angle
+// between t and d is at most phi + asin(R / |d|). R grows by 1e-9 of the
+// coordinates' size for the FP64 rounding of the test.
+ST_INLINE bool sphere_cone_reaches(const double ca[3], double ra, const double t[3], double cp, double sp,
+                                   const double cb[3], double rb) {
+    const double d[3] = {cb[0] - ca[0], cb[1] - ca[1], cb[2] - ca[2]};
+    const double L = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double R = ra + rb + 1e-9 * (L + fabs(ca[0]) + fabs(ca[1]) + fabs(ca[2]));
+    if (L <= R) return true;
+    const double sn = R / L, cs = sqrt(fmax(0.0, 1.0 - sn * sn));
+    return d[0] * t[0] + d[1] * t[1] + d[2] * t[2] >= L * (cp * cs - sp * sn);
+}
+
 struct Cone {
     double a[3];
     double tp, cp, sp;  // tan, cos, sin of phi

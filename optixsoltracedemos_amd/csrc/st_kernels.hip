@@ -1718,15 +1718,8 @@ __global__ __launch_bounds__(kReaimBlock) void k_reaim_lists(const st_element_f6
             bool pass = false;
             if (j < tile) {
                 const double4 c = lds_spheres[j];
-                const double d[3] = {c.x - a.x, c.y - a.y, c.z - a.z};
-                const double L = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-                const double R = a.w + c.w + 1e-9 * (L + fabs(a.x) + fabs(a.y) + fabs(a.z));
-                if (L <= R) {
-                    pass = true;
-                } else {
-                    const double sn = R / L, cs = sqrt(fmax(0.0, 1.0 - sn * sn));
-                    pass = d[0] * t[0] + d[1] * t[1] + d[2] * t[2] >= L * (cp * cs - sp * sn);
-                }
+                const double ca[3] = {a.x, a.y, a.z}, cb[3] = {c.x, c.y, c.z};
+                pass = sphere_cone_reaches(ca, a.w, t, cp, sp, cb, c.w);
             }
             const unsigned long long m = __ballot(pass);
             const uint32_t cnt = (uint32_t)__popcll(m);

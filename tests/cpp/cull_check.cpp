@@ -15,6 +15,9 @@
 //                                          tangent scaled by tan_scale
 //   cull_check gridbox <seed> <tan_scale> the same for a curved primitive (box
 //                                          footprint), rays ending in its box
+//   cull_check reaim <seed> <phi_scale>   sphere_cone_reaches as k_reaim_lists
+//                                          uses it (bisector-aimed mirror, axis
+//                                          to the target), phi scaled
 // A scale of 1 is the shipped test (violations must be 0); a scale below 1
 // plants a too-narrow cone, which the sampling must catch.
 #include <cmath>
@@ -402,6 +405,84 @@ int check_grid(double tan_scale, bool flat) {
 
 }  // namespace
 
+// Re-aim candidates (k_reaim_lists): a mirror A (rectangle w x h) aimed by the
+// bisector rule at a target for a random sun reflects sun rays (directions
+// within phi of reflect(-sun, n)) from its rectangle. (i) Spheres around points
+// those rays pass through must pass sphere_cone_reaches with the kernel's
+// arguments (A's circumscribed sphere + 1 mm, axis from A's centre to the
+// target, phi + 1e-6); (ii) spheres that fail are shot at with many such rays,
+// none of which may enter them.
+int check_reaim(double phi_scale) {
+    long reach = 0, reach_dropped = 0, spheres = 0, dropped = 0, dropped_hit = 0;
+    for (int cfg = 0; cfg < 200; ++cfg) {
+        const double pc[3] = {uni(-800, 800), uni(-800, 800), uni(-3, 3)};
+        const double target[3] = {uni(-50, 50), uni(-50, 50), uni(20, 250)};
+        const double hw = uni(0.3, 6.0), hh = uni(0.3, 6.0);
+        const double phi = std::exp(uni(std::log(2e-3), std::log(0.2)));
+        double sun[3];
+        do random_unit(sun);
+        while (sun[2] < 0.05);
+        double aim[3];
+        bisector_aim_point(pc, sun, target, aim);
+        double n[3] = {aim[0] - pc[0], aim[1] - pc[1], aim[2] - pc[2]};
+        normalize(n);
+        double x[3], y[3], tmp[3];
+        random_unit(tmp);
+        cross(tmp, n, x);
+        normalize(x);
+        cross(n, x, y);
+        // the traced reflection axis, reflect(-sun, n)
+        double a[3];
+        const double dn = -dot(sun, n);
+        for (int k = 0; k < 3; ++k) a[k] = -sun[k] - 2.0 * dn * n[k];
+        normalize(a);
+        // the kernel's arguments
+        double t[3] = {target[0] - pc[0], target[1] - pc[1], target[2] - pc[2]};
+        normalize(t);
+        const double ra = std::sqrt(hw * hw + hh * hh) + 1e-3, pk = phi * phi_scale + 1e-6;
+        const double cp = std::cos(pk), sp = std::sin(pk);
+        auto ray = [&](double o[3], double u[3]) {
+            const double px = uni(-hw, hw), py = uni(-hh, hh);
+            for (int k = 0; k < 3; ++k) o[k] = pc[k] + px * x[k] + py * y[k];
+            cone_direction(a, phi, u);
+        };
+        for (int i = 0; i < 1000; ++i) {  // (i) reachable spheres
+            double o[3], u[3];
+            ray(o, u);
+            const double s = std::exp(uni(std::log(1e-2), std::log(1500.0)));
+            const double rb = uni(0.05, 8.0);
+            double v[3];
+            random_unit(v);
+            const double off = uni(0.0, rb);
+            const double c[3] = {o[0] + s * u[0] + off * v[0], o[1] + s * u[1] + off * v[1], o[2] + s * u[2] + off * v[2]};
+            ++reach;
+            if (!sphere_cone_reaches(pc, ra, t, cp, sp, c, rb + 1e-3)) ++reach_dropped;
+        }
+        for (int i = 0; i < 200; ++i) {  // (ii) random spheres near A
+            const double c[3] = {pc[0] + uni(-60, 60), pc[1] + uni(-60, 60), pc[2] + uni(-20, 60)};
+            const double rb = uni(0.1, 8.0);
+            ++spheres;
+            if (sphere_cone_reaches(pc, ra, t, cp, sp, c, rb + 1e-3)) continue;
+            ++dropped;
+            for (int r = 0; r < 1000; ++r) {
+                double o[3], u[3];
+                ray(o, u);
+                // closest approach of the ray to the sphere's centre
+                const double w[3] = {c[0] - o[0], c[1] - o[1], c[2] - o[2]};
+                const double s = std::fmax(0.0, dot(w, u));
+                const double q[3] = {o[0] + s * u[0] - c[0], o[1] + s * u[1] - c[1], o[2] + s * u[2] - c[2]};
+                if (dot(q, q) <= rb * rb) {
+                    ++dropped_hit;
+                    break;
+                }
+            }
+        }
+    }
+    std::printf("reaim reachable %ld reachable_dropped %ld spheres %ld dropped %ld dropped_but_hit %ld\n", reach,
+                reach_dropped, spheres, dropped, dropped_hit);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc != 4) {
         std::fprintf(stderr, "usage: cull_check cone|sweep|sweeprect|grid|gridbox <seed> <scale>\n");
@@ -414,5 +495,6 @@ int main(int argc, char** argv) {
     if (!std::strcmp(argv[1], "sweeprect")) return check_sweep_rect(scale);
     if (!std::strcmp(argv[1], "grid")) return check_grid(scale, true);
     if (!std::strcmp(argv[1], "gridbox")) return check_grid(scale, false);
+    if (!std::strcmp(argv[1], "reaim")) return check_reaim(scale);
     return 2;
 }

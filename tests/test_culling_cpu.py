@@ -12,7 +12,11 @@ the same functions the list-building kernels run):
   candidates' rectangles;
 * grid_range / cell_meets (sun-plane grid cells): every ray that hits a flat
   primitive's rectangle, or enters a curved primitive's traversal box, traced
-  back within the sun cone to the sun plane, starts in a listed cell.
+  back within the sun cone to the sun plane, starts in a listed cell;
+* sphere_cone_reaches (re-aim candidates of batched sweeps, k_reaim_lists):
+  spheres around points that a bisector-aimed mirror's reflections pass
+  through are never dropped, with the cone's axis taken towards the target as
+  the kernel takes it, and dropped spheres are missed by sampled reflections.
 
 Each check also runs with a planted mistake (a cone narrower than the real one
 when the test is built) and must then report violations, so the sampling has
@@ -103,3 +107,15 @@ def test_grid_cells_of_curved_primitives_are_conservative(exe, seed):
 def test_grid_box_check_catches_a_narrow_cone(exe):
     r = run(exe, "gridbox", 1, 0.3)
     assert r["unlisted"] > 0, r
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_reaim_sphere_test_is_conservative(exe, seed):
+    r = run(exe, "reaim", seed, 1.0)
+    assert r["reachable"] > 100_000 and r["reachable_dropped"] == 0, r
+    assert r["dropped"] > 10_000 and r["dropped_but_hit"] == 0, r
+
+
+def test_reaim_check_catches_a_narrow_cone(exe):
+    r = run(exe, "reaim", 1, 0.5)
+    assert r["reachable_dropped"] > 0 and r["dropped_but_hit"] > 0, r
