@@ -116,11 +116,13 @@ LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kerne
             if (!trace_variant_exists(sh.block, sh.stack, sh.waves, sh.queued)) sh.block = 128;
         }
     }
-    if (sh.queued && !want_grouped && min_elements <= kLdsHitElements && bvh_depth <= 16 && block <= 0 &&
-        waves <= 0) {
+    if (sh.queued && !want_grouped && min_elements <= kLdsHitElements && block <= 0 && waves <= 0 &&
+        (bvh_depth <= 16 || (listed && bvh_depth <= kStackDepth))) {
+        // listed launches (their BVH walks are rare) keep the stack in private
+        // memory, which leaves LDS for 6 waves per SIMD
         sh.block = 256;
-        sh.stack = bvh_depth <= 8 ? 8 : 16;
-        sh.waves = 5;
+        sh.stack = listed && bvh_depth <= kStackDepth ? 0 : bvh_depth <= 8 ? 8 : 16;
+        sh.waves = sh.stack == 0 ? 6 : 5;
         sh.lds_hits = true;
     }
     // Chunks of 64 to 2048 rays. Every fetch from the shared chunk counter (a
@@ -477,12 +479,19 @@ int scene_bounce_lists(st_scene s) {
     ++s->list_builds;
     s->bounce_valid = s->pair_valid = false;
     const double theta = sun_cone_angle(s->sun);
+    // mirrors aimed by the bisector rule at a target for the sun traced: the
+    // first-reflection lists test the re-aim candidates
+    const bool reaim = s->aim_bisector && s->aim_sun[0] == s->sun_unit[0] && s->aim_sun[1] == s->sun_unit[1] &&
+                       s->aim_sun[2] == s->sun_unit[2];
+    if (reaim) ST_TRY(ensure_reaim(s->reaim, s, s->aim_target, theta, st));
     auto common = [&](int32_t tasks, const int32_t* pitems, const int32_t* powner, uint32_t* c, int32_t* scratch) {
+        const bool sup = reaim && !pitems;
         return launch_cone_lists(st, (const st_element_f64*)s->d_elems.ptr, tasks, tasks, s->n, s->n_nodes,
                                  (const int32_t*)s->d_elem_slot.ptr, (const uint32_t*)s->d_meta.ptr,
                                  (const double*)s->d_frames.ptr, (const float*)s->d_bounds.ptr,
                                  (const float4*)s->d_nodes.ptr, (const SunFrame*)s->d_sun.ptr, theta, pitems, powner,
-                                 c, nullptr, scratch);
+                                 c, nullptr, scratch, sup ? (const uint2*)s->reaim.range.ptr : nullptr,
+                                 sup ? (const int32_t*)s->reaim.items.ptr : nullptr);
     };
     // first reflections: one list per element
     ST_TRY(build_cone_lists(s, (size_t)s->n, s->d_bounce_counts, s->d_bounce_range, s->d_bounce_items,
@@ -536,6 +545,28 @@ int scene_bounce_lists(st_scene s) {
 // grouped form takes -8 / -19 / -22 / -23 / -11 / -1 / -1 / -1.5 % of the plain
 // queued form's time at 5 / 15 / 25 / 30 / 40 / 50 / 60 / 80 deg elevation.
 static bool mixed_bvh_segments(st_scene, const JobDev& J) { return J.bounce_range != nullptr; }
+
+int ensure_reaim(ReaimCache& c, st_scene s, const double target[3], double theta, hipStream_t st) {
+    if (c.valid && c.epoch == s->geom_epoch && c.theta == theta && c.target[0] == target[0] &&
+        c.target[1] == target[1] && c.target[2] == target[2])
+        return ST_OK;
+    c.valid = false;
+    const size_t n = (size_t)s->n;
+    ST_TRY(c.spheres.ensure(sizeof(double4) * n, "re-aim spheres"));
+    ST_TRY(c.range.ensure(sizeof(uint2) * n, "re-aim list ranges"));
+    ST_TRY(c.items.ensure(sizeof(int32_t) * kMaxBounceList * n, "re-aim lists"));
+    ST_TRY(c.target_dev.ensure(sizeof(double) * 3, "re-aim target"));
+    for (int k = 0; k < 3; ++k) c.target[k] = target[k];
+    HIP_TRY(hipMemcpyAsync(c.target_dev.ptr, c.target, sizeof(double) * 3, hipMemcpyHostToDevice, st), "upload");
+    HIP_TRY(launch_reaim_lists(st, (const st_element_f64*)s->d_elems.ptr, s->n, (const int32_t*)s->d_elem_slot.ptr,
+                               (const uint32_t*)s->d_meta.ptr, (const double*)c.target_dev.ptr, theta,
+                               (double4*)c.spheres.ptr, (uint2*)c.range.ptr, (int32_t*)c.items.ptr),
+            "re-aim candidate lists");
+    c.theta = theta;
+    c.epoch = s->geom_epoch;
+    c.valid = true;
+    return ST_OK;
+}
 
 int route_flux_copies(DevBuf& buf, JobDev* jobs, int32_t n, uint64_t bins, uint64_t rays, hipStream_t st,
                       uint32_t* copies) {
@@ -686,6 +717,7 @@ extern "C" int st_scene_update_aim(st_scene s, const double* aim_points, const d
         for (int k = 0; k < 3; ++k) s->host_elems[i].aim_point[k] = aim_points[3 * i + k];
         if (zrot_deg) s->host_elems[i].zrot_deg = zrot_deg[i];
     }
+    s->aim_bisector = false;
     hipStream_t st = s->ctx->stream;
     HIP_TRY(hipMemcpyAsync(s->d_elems.ptr, s->host_elems.data(), sizeof(st_element_f64) * s->n, hipMemcpyHostToDevice, st),
             "upload");
@@ -704,6 +736,8 @@ extern "C" int st_scene_update_elements(st_scene s, const st_element_f64* elemen
             return fail(ST_ERR_INVALID, "element %d changed type or role; create a new scene", i);
     HIP_TRY(hipSetDevice(s->ctx->device), "hipSetDevice");
     s->host_elems.assign(elements, elements + n);
+    s->aim_bisector = false;
+    ++s->geom_epoch;
     hipStream_t st = s->ctx->stream;
     HIP_TRY(hipMemcpyAsync(s->d_elems.ptr, elements, sizeof(st_element_f64) * n, hipMemcpyHostToDevice, st), "upload");
     ST_TRY(scene_refresh(s, false));
@@ -749,6 +783,8 @@ extern "C" int st_scene_update(st_scene s, const st_element_f64* elements, int32
     hipStream_t st = s->ctx->stream;
     if (elements) {
         s->host_elems.assign(elements, elements + n);
+        s->aim_bisector = false;
+        ++s->geom_epoch;
         HIP_TRY(hipMemcpyAsync(s->d_elems.ptr, elements, sizeof(st_element_f64) * n, hipMemcpyHostToDevice, st),
                 "upload");
         ST_TRY(scene_refresh(s, false));
@@ -771,6 +807,11 @@ extern "C" int st_scene_aim_heliostats(st_scene s, const double sun_vector[3], c
     HIP_TRY(launch_aim(st, (st_element_f64*)s->d_elems.ptr, s->n, (const double*)s->d_aim.ptr,
                        (const double*)s->d_aim.ptr + 3),
             "aim kernel");
+    s->aim_bisector = true;
+    for (int k = 0; k < 3; ++k) {
+        s->aim_target[k] = target[k];
+        s->aim_sun[k] = st_in[k];
+    }
     ST_TRY(scene_refresh(s, false));
     ST_TRY(scene_sun_plane(s));  // synchronizes the stream
     HIP_TRY(hipMemcpyAsync(s->host_elems.data(), s->d_elems.ptr, sizeof(st_element_f64) * s->n, hipMemcpyDeviceToHost, st),
@@ -792,6 +833,11 @@ extern "C" int st_scene_set_sun_and_aim(st_scene s, const st_sun_desc* sun, cons
     HIP_TRY(launch_aim(st, (st_element_f64*)s->d_elems.ptr, s->n, (const double*)s->d_aim.ptr,
                        (const double*)s->d_aim.ptr + 3),
             "aim kernel");
+    s->aim_bisector = true;
+    for (int k = 0; k < 3; ++k) {
+        s->aim_target[k] = target[k];
+        s->aim_sun[k] = s->sun_unit[k];
+    }
     ST_TRY(scene_refresh(s, false));
     ST_TRY(scene_sun_plane(s));  // one sun plane and grid, for the new sun and aims
     HIP_TRY(hipMemcpyAsync(s->host_elems.data(), s->d_elems.ptr, sizeof(st_element_f64) * s->n, hipMemcpyDeviceToHost, st),

@@ -45,6 +45,23 @@ struct DevBuf {
 LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel, int32_t bvh_depth,
                        const LaunchShape* tuned, int32_t min_elements, int32_t waves = 0, bool listed = false);
 
+// First-reflection candidates of a scene's mirrors over every bisector re-aim
+// at a target (k_reaim_lists): while every mirror is aimed by the bisector rule
+// at that target for the sun being traced, each mirror's first reflections
+// point at the target, and the scene state's first-reflection lists test these
+// candidates instead of walking the BVH. Kept for one (target, sun cone angle,
+// element geometry) key.
+struct ReaimCache {
+    DevBuf spheres, range, items, target_dev;
+    bool valid = false;
+    double target[3] = {0, 0, 0};
+    double theta = -1.0;
+    uint64_t epoch = 0;
+};
+// the candidates of scene s for `target` and sun cone angle theta (rebuilt
+// when the key changed), asynchronous on stream st
+int ensure_reaim(ReaimCache& c, st_scene s, const double target[3], double theta, hipStream_t st);
+
 struct BatchCache {
     bool valid = false;
     int32_t steps = 0, max_depth = 0, flux_element = 0, flux_nx = 0, flux_ny = 0, block = 0, grid = 0;
@@ -147,6 +164,13 @@ struct st_scene_s {
     // lists would be long, e.g. at grazing sun elevations)
     st::DevBuf d_grid_counts, d_grid_start, d_grid_cursor, d_grid_items, d_grid_stats;
     st::DevBuf d_flux_copies;  // copies of a small flux map (route_flux_copies)
+    // every mirror aimed by the bisector rule at aim_target for the sun aim_sun
+    // (st_scene_set_sun_and_aim, st_scene_aim_heliostats); geometry changes
+    // (element updates) bump geom_epoch
+    bool aim_bisector = false;
+    double aim_target[3] = {0, 0, 0}, aim_sun[3] = {0, 0, 0};
+    uint64_t geom_epoch = 0;
+    st::ReaimCache reaim;
     bool grid_valid = false;
     uint32_t grid_items = 0, grid_longest = 0;
     // candidate lists of first reflections (per element) and second

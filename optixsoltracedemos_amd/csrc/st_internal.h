@@ -100,7 +100,8 @@ struct LaunchShape {
     int waves = 4;       // waves per SIMD the register allocation is built for
     bool queued = false; // continuing rays parked in LDS, traced in full waves
     bool lds_hits = false;  // hit counts of elements below kLdsHitElements kept per wave in
-                            // LDS (launches with a scene that small; queued, block 256, stack 8 or 16)
+                            // LDS (launches with a scene that small; queued, block 256, stack 8 or
+                            // 16 at 5 waves/SIMD, or stack 0 at 6 for listed launches)
     bool group_bvh = false; // queued form tracing list and BVH segments in separate steps
                             // (use_grouped_bvh; compiled for stack 16 at 5 waves/SIMD and stack 0 at 6 or 8)
 };
@@ -116,7 +117,10 @@ bool grouped_variant_exists(int block, int stack, int waves);
 // second mirror, and those without a second-reflection list, walk it) traces
 // the two kinds of segment in separate steps when its shape allows it.
 inline void use_grouped_bvh(LaunchShape& sh, bool mixed) {
-    sh.group_bvh = mixed && sh.queued && !sh.lds_hits && grouped_variant_exists(sh.block, sh.stack, sh.waves);
+    // with LDS hit counts: compiled for block 256, stack in private memory, 6 waves/SIMD
+    sh.group_bvh = mixed && sh.queued &&
+                   (sh.lds_hits ? sh.block == 256 && sh.stack == 0 && sh.waves == 6
+                                : grouped_variant_exists(sh.block, sh.stack, sh.waves));
 }
 // waves of this trace variant the device holds at once (0 if unknown)
 int trace_resident_waves(const LaunchShape& shape);

@@ -954,11 +954,12 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
     __shared__ float lds_queue[QUEUED ? BLOCK / 64 : 1][QUEUED ? 8 * kQueueCap : 1];
     __shared__ uint32_t lds_log[BLOCK / 64][1 + kLogCap];
     __shared__ uint32_t lds_hits[LHITS > 0 ? BLOCK / 64 : 1][LHITS > 0 ? LHITS : 1];
-    // queued form (not the small-scene LDS-hit variants): first ray index and
+    // queued form (not the LDS-stack variants with LDS hit counts, whose LDS
+    // is full): first ray index and
     // stratified cell of each group of 64 slots of the chunk, computed for the
     // whole chunk at its start, 32 groups in 32 lanes (every order but
     // Halton's, whose groups' indices are not consecutive)
-    constexpr bool TABLE = QUEUED && LHITS == 0;
+    constexpr bool TABLE = QUEUED && (LHITS == 0 || STACK == 0);
     __shared__ uint64_t lds_bases[TABLE ? BLOCK / 64 : 1][TABLE ? kMaxChunkRays / 64 : 1];
     __shared__ uint32_t lds_cells[TABLE ? BLOCK / 64 : 1][TABLE ? kMaxChunkRays / 64 : 1];
     int32_t private_stack[STACK > 0 ? 1 : kStackDepth];
@@ -2057,7 +2058,9 @@ bool trace_variant_exists(int block, int stack, int waves, bool queued) {
     ST_VARIANT(256, 0, 6, 2, 0)        \
     ST_VARIANT(256, 0, 8, 2, 0)        \
     ST_VARIANT(256, 8, 5, 1, kLdsHitElements) \
-    ST_VARIANT(256, 16, 5, 1, kLdsHitElements)
+    ST_VARIANT(256, 16, 5, 1, kLdsHitElements) \
+    ST_VARIANT(256, 0, 6, 1, kLdsHitElements) \
+    ST_VARIANT(256, 0, 6, 2, kLdsHitElements)
 
 int trace_resident_waves(const LaunchShape& shape) {
     const int st = shape.stack, w = shape.waves, b = shape.block, h = shape.lds_hits ? kLdsHitElements : 0;

@@ -10,7 +10,8 @@
 //      mirrors for the job's sun (FP64 bisector at the scene's last receiver),
 //      pack frames, refit the BVH, build the sun plane and its candidate grid
 //      and, for jobs of kServerListRays rays or more, the first-reflection
-//      candidate lists - all on the device, in buffers pooled per scene;
+//      candidate lists (each mirror's re-aim candidates, kept per scene, tested
+//      against the job's cones) - all on the device, in buffers pooled per scene;
 //   3. launch ONE trace kernel over a slice of the active jobs, shortest
 //      remaining job first, at most chunk_rays per job and round_rays in all
 //      (jobs are chunk lists of a single launch, so work of different jobs
@@ -64,7 +65,7 @@ struct OutBufs {
     DevBuf flux, hits, stats;
 };
 
-constexpr uint64_t kServerListRays = 1ull << 24;  // jobs this long get first-reflection lists
+constexpr uint64_t kServerListRays = 1ull << 21;  // jobs this long get first-reflection lists
 
 struct ServerJob {
     int64_t handle = 0;
@@ -111,6 +112,9 @@ struct st_server_s {
     uint32_t* h_grid_stats = nullptr;  // pinned, 2 per active slot
     // buffer pools: scene copies per scene slot (worker thread only), outputs (under mu)
     std::unordered_map<int32_t, std::vector<std::unique_ptr<WorkBufs>>> work_pool;
+    // per scene slot: first-reflection candidates over the re-aims of its jobs
+    // (every job re-aims the mirrors at the scene's receiver; worker thread only)
+    std::unordered_map<int32_t, std::unique_ptr<st::ReaimCache>> reaim;
     std::vector<std::unique_ptr<OutBufs>> out_pool;
     st_server_stats stats{};
     std::string worker_error;
@@ -258,12 +262,23 @@ int finish_setup(st_server srv, ServerJob* j, uint32_t grid_total, uint32_t grid
             hipSuccess)
             return hip_fail(e, "job grid fill");
     }
+    // a job that re-aims the mirrors at the receiver: its lists test the
+    // scene's re-aim candidates instead of walking the BVH
+    st::ReaimCache* ra = nullptr;
+    if (j->lists && !(d.flags & ST_JOB_KEEP_AIM)) {
+        auto& slot_cache = srv->reaim[d.scene_slot];
+        if (!slot_cache) slot_cache = std::make_unique<st::ReaimCache>();
+        ra = slot_cache.get();
+        if ((rc = ensure_reaim(*ra, sc, j->aim_host + 3, sun_cone_angle(sun), st)) != ST_OK) return rc;
+    }
     if (j->lists &&
         (e = launch_cone_lists(st, (const st_element_f64*)w.elems.ptr, sc->n, sc->n, sc->n, sc->n_nodes,
                                (const int32_t*)sc->d_elem_slot.ptr, (const uint32_t*)sc->d_meta.ptr,
                                (const double*)w.frames.ptr, (const float*)w.bounds.ptr, (const float4*)w.nodes.ptr,
                                (const SunFrame*)w.sun.ptr, sun_cone_angle(sun), nullptr, nullptr, nullptr,
-                               (uint2*)w.list_range.ptr, (int32_t*)w.list_items.ptr)) != hipSuccess)
+                               (uint2*)w.list_range.ptr, (int32_t*)w.list_items.ptr,
+                               ra ? (const uint2*)ra->range.ptr : nullptr,
+                               ra ? (const int32_t*)ra->items.ptr : nullptr)) != hipSuccess)
         return hip_fail(e, "job reflection lists");
     st_trace_desc td;
     memset(&td, 0, sizeof(td));
