@@ -591,6 +591,7 @@ int route_flux_copies(DevBuf& buf, JobDev* jobs, int32_t n, uint64_t bins, uint6
 int scene_grid(st_scene s) {
     hipStream_t st = s->ctx->stream;
     s->grid_valid = false;
+    if (s->n < kGridMinElements) return ST_OK;
     ST_TRY(s->d_grid_counts.ensure(sizeof(uint32_t) * kGridCells, "grid counts"));
     ST_TRY(s->d_grid_cursor.ensure(sizeof(uint32_t) * kGridCells, "grid cursor"));
     ST_TRY(s->d_grid_start.ensure(sizeof(uint32_t) * (kGridCells + 1), "grid start"));
@@ -1341,7 +1342,7 @@ static int batch_plan_lists(st_scene s, hipStream_t st) {
         ST_TRY(b.d_sup_scratch.ensure(sizeof(int32_t) * kMaxBounceList * (size_t)n, "batch re-aim lists"));
     }
     const uint64_t grid_bytes = (uint64_t)K * (3 * kGridCells + 1) * sizeof(uint32_t);
-    if (grid_bytes > kBatchListBytes / 4) return ST_OK;
+    if (grid_bytes > kBatchListBytes / 4 || n < kGridMinElements) return ST_OK;
     ST_TRY(b.d_grid_counts.ensure(sizeof(uint32_t) * kGridCells * (size_t)K, "batch grid counts"));
     ST_TRY(b.d_grid_cursor.ensure(sizeof(uint32_t) * kGridCells * (size_t)K, "batch grid cursor"));
     ST_TRY(b.d_grid_start.ensure(sizeof(uint32_t) * (kGridCells + 1) * (size_t)K, "batch grid start"));

@@ -87,6 +87,9 @@ inline void set_trace_order(JobDev* J, int32_t position_sampling) {
 // even lists of a few hundred entries beat the walk; the average bounds the
 // grid's memory (kGridMaxAverage * kGridCells items)
 constexpr uint32_t kGridMaxAverage = 16, kGridMaxLongest = 512;
+// scenes of fewer elements trace sun rays through the BVH (a few nodes): the
+// grid's passes over its 65,536 cells cost more than they save
+constexpr int32_t kGridMinElements = 16;
 __host__ __device__ inline bool grid_lists_pay(uint32_t total, uint32_t longest) {
     return total > 0 && total <= kGridMaxAverage * (uint32_t)kGridCells && longest <= kGridMaxLongest;
 }

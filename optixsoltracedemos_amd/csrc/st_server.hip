@@ -227,6 +227,7 @@ int setup_job(st_server srv, ServerJob* j, int slot) {
     if (nbins && (e = launch_fill_u64(st, o.flux.ptr, nbins, 0)) != hipSuccess) return hip_fail(e, "job zero");
     if ((e = launch_fill_u64(st, o.hits.ptr, n, 0)) != hipSuccess) return hip_fail(e, "job zero");
     if ((e = launch_fill_u64(st, o.stats.ptr, ST_STAT_COUNT, 0)) != hipSuccess) return hip_fail(e, "job zero");
+    if (sc->n < kGridMinElements) return ST_OK;  // no grid (finish_setup ignores the slot's totals)
     if ((e = launch_fill_u64(st, srv->d_grid_counts.ptr, kGridCells / 2, 0)) != hipSuccess)
         return hip_fail(e, "job grid clear");
     if ((e = launch_grid_count(st, (const float*)w.bounds.ptr, (const float4*)w.prims.ptr,
@@ -250,7 +251,7 @@ int finish_setup(st_server srv, ServerJob* j, uint32_t grid_total, uint32_t grid
     if (!(sun.dni > 0)) sun.dni = 1000.0;
     hipError_t e;
     int rc;
-    const bool grid = grid_lists_pay(grid_total, grid_longest);
+    const bool grid = sc->n >= kGridMinElements && grid_lists_pay(grid_total, grid_longest);
     if (grid) {
         if ((rc = w.grid_items.ensure(sizeof(int32_t) * grid_total, "job grid items")) != ST_OK) return rc;
         if ((e = launch_fill_u64(st, srv->d_grid_cursor.ptr, kGridCells / 2, 0)) != hipSuccess)

@@ -375,6 +375,33 @@ def test_candidate_lists_equal_bvh(gpu_ctx, case):
             assert np.array_equal(r[k], ref[k]), (case, kernel, waves, k)
 
 
+@pytest.mark.parametrize("elevation", [12.0, 60.0])
+def test_reaim_candidate_lists_equal_the_bvh(gpu_ctx, elevation):
+    """After st_scene_set_sun_and_aim every mirror is aimed by the bisector rule
+    at the target for the sun traced, and the first-reflection lists test each
+    mirror's re-aim candidates (k_reaim_lists) instead of walking the BVH; the
+    results equal the BVH's bitwise. A sun change that keeps the aims (the
+    candidates no longer apply: the lists walk the BVH) and an aim from a
+    different sun give the BVH's results too."""
+    w = scenes.heliostat_field(2_000)
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    sun = scenes.sun_vector(elevation, 150.0)
+    sc.set_sun_and_aim(sun, w.sun_param, w.extra["target"], position_sampling=_abi.POS_STRATIFIED)
+    kw = dict(max_depth=5, flux_nx=w.flux_nx, flux_ny=w.flux_ny, seed=3, records=True)
+    for step in range(3):
+        ref = sc.trace(500_000, bvh_only=True, **kw).numpy()
+        sc.build_lists()
+        assert sc.list_info()["reflection_items"] > 0
+        got = sc.trace(500_000, **kw).numpy()
+        for k in ("flux", "element_hits", "stats", "records", "record_elements"):
+            assert np.array_equal(got[k], ref[k]), (elevation, step, k)
+        if step == 0:  # a new sun, aims unchanged
+            sc.set_sun(scenes.sun_vector(elevation + 9.0, 120.0), w.sun_param, position_sampling=_abi.POS_STRATIFIED)
+        elif step == 1:  # aims from one sun, then another sun traced
+            sc.aim_heliostats(scenes.sun_vector(elevation, 200.0), w.extra["target"])
+            sc.set_sun(scenes.sun_vector(elevation + 4.0, 210.0), w.sun_param, position_sampling=_abi.POS_STRATIFIED)
+
+
 def test_lists_build_lazily_and_follow_the_sun(gpu_ctx):
     """Reflection lists appear only after 2^26 traced rays on a scene state (or
     an explicit build) and are dropped by a sun change; results never depend
