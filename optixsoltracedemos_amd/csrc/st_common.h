@@ -194,16 +194,37 @@ struct SunFrame {
 // key (seed_lo, seed_hi).
 struct U4 { uint32_t x, y, z, w; };
 
+// a ^ b ^ c. On the device one v_bitop3_b32 (truth table 0x96), where the
+// compiler issues two XORs; a vector instruction, so only for values that
+// differ across lanes (uniform ones stay on the scalar unit as plain XORs).
+ST_INLINE uint32_t xor3_lanes(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 // Each round multiplies through one 32x32 -> 64-bit product (a single
 // v_mad_u64_u32 on gfx950, where the separate high and low multiplies would
-// each cost as much).
+// each cost as much). LANES: a per-lane counter under a wave-uniform key (the
+// three-way XORs of a round as two bitop3 instructions instead of four XORs;
+// the key is made opaque at each call, so its ten-round schedule is recomputed
+// on the scalar unit instead of being hoisted out of the caller's loops into
+// registers that do not fit and are reloaded by vector instructions); the
+// results are the same either way.
+template <bool LANES = false>
 ST_INLINE U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (LANES) asm volatile("" : "+s"(k0), "+s"(k1));
+#endif
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c.x, p1 = (uint64_t)0xCD9E8D57u * c.z;
         const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
         const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        if (LANES) c = U4{xor3_lanes(hi1, c.y, k0), lo1, xor3_lanes(hi0, c.w, k1), lo0};
+        else c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }

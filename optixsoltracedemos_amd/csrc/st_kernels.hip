@@ -644,7 +644,7 @@ __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame
                                         V3& o, V3& d, uint32_t& cell, uint32_t cell_hint = 0xFFFFFFFFu) {
     const uint32_t rlo = (uint32_t)ray, rhi = (uint32_t)(ray >> 32);
     // ---- sun sample: cuda/sun.cu:94-134
-    const U4 u = philox4x32_10(U4{rlo, rhi, 0u, J.job_id}, J.seed_lo, J.seed_hi);
+    const U4 u = philox4x32_10<true>(U4{rlo, rhi, 0u, J.job_id}, J.seed_lo, J.seed_hi);
     float a, b;
     if (S.position_sampling == ST_POS_STRATIFIED) {
         // the 64 rays of a group share a cell and a wave's rays normally form
@@ -823,7 +823,7 @@ __device__ __forceinline__ bool shade_mirror(const JobDev& J, uint64_t ray, int 
     const float4 op = ld4(J.optics + elem);
     U4 ue = U4{0u, 0u, 0u, 0u};
     if (op.x < 1.f || op.y > 0.f || op.z > 0.f)
-        ue = philox4x32_10(U4{rlo, rhi, 2u * (uint32_t)nd - 1u, J.job_id}, J.seed_lo, J.seed_hi);
+        ue = philox4x32_10<true>(U4{rlo, rhi, 2u * (uint32_t)nd - 1u, J.job_id}, J.seed_lo, J.seed_hi);
     if (op.x < 1.f && u01(ue.x) >= op.x) {
         count(wc, ST_STAT_ABSORBED);
         return false;
@@ -850,7 +850,7 @@ __device__ __forceinline__ bool shade_mirror(const JobDev& J, uint64_t ray, int 
     if (op.y > 0.f) n = perturb(n, op.y, u01(ue.y), u01(ue.z));
     V3 rd = normalize(fma3(-2.f * dot(n, d), n, d));
     if (op.z > 0.f) {
-        const U4 us = philox4x32_10(U4{rlo, rhi, 2u * (uint32_t)nd, J.job_id}, J.seed_lo, J.seed_hi);
+        const U4 us = philox4x32_10<true>(U4{rlo, rhi, 2u * (uint32_t)nd, J.job_id}, J.seed_lo, J.seed_hi);
         rd = perturb(rd, op.z, u01(ue.w), u01(us.x));
     }
     d = rd;
@@ -944,10 +944,10 @@ __device__ __forceinline__ uint32_t queue_tag(uint32_t offset, int depth, int ke
 // LDS left to each wave (parked rays, log, counters) is small enough for 8
 // waves per SIMD: the form for scenes whose rays rarely walk the BVH.
 template <int BLOCK, int STACK, int MINW, int QUEUED, int LHITS>
-__global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict__ jobs,
+__global__ __launch_bounds__(BLOCK, MINW) void k_trace(JobDev single, const JobDev* __restrict__ jobs,
                                                        const uint64_t* __restrict__ chunk_prefix, int32_t n_jobs,
                                                        uint64_t total_chunks, uint32_t* __restrict__ work_counter,
-                                                       int32_t chunk_rays, int32_t static_chunks, JobDev single) {
+                                                       int32_t chunk_rays, int32_t static_chunks) {
     constexpr int SS = STACK > 0 ? BLOCK : 1;  // stride of the stack's entries
     __shared__ int32_t lds_stack[STACK > 0 ? STACK * BLOCK : 1];
     __shared__ uint32_t lds_counters[BLOCK / 64][kNumCounters];
@@ -1197,7 +1197,15 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(const JobDev* __restrict_
             lo = __builtin_amdgcn_readfirstlane(lo);
             run_chunk(*(const JobDev*)(const ST_CONST JobDev*)(jobs + lo), c - chunk_prefix[lo]);
         } else {
-            run_chunk(single, c);
+            // `single` is the first argument: offset 0 of the kernel argument
+            // segment. Read there through an opaque pointer, its fields are
+            // loaded at their uses as the job records' are; named directly, the
+            // compiler hoists every field into scalar registers, more than fit,
+            // and reloads the spilled ones with vector instructions (its
+            // address would be that of a private copy).
+            const ST_CONST JobDev* sp = (const ST_CONST JobDev*)__builtin_amdgcn_kernarg_segment_ptr();
+            asm volatile("" : "+s"(sp));
+            run_chunk(*(const JobDev*)sp, c);
         }
     }
     if (!static_chunks && lane == 0) {
@@ -1989,8 +1997,8 @@ template <int BLOCK, int STACK, int MINW, int QUEUED, int LHITS>
 static void launch_variant(hipStream_t s, int grid, const JobDev* jobs, const uint64_t* chunk_prefix, int32_t n_jobs,
                            uint64_t total_chunks, uint32_t* work_counter, int chunk, int static_chunks,
                            const JobDev& single) {
-    hipLaunchKernelGGL((k_trace<BLOCK, STACK, MINW, QUEUED, LHITS>), dim3(grid), dim3(BLOCK), 0, s, jobs, chunk_prefix,
-                       n_jobs, total_chunks, work_counter, chunk, static_chunks, single);
+    hipLaunchKernelGGL((k_trace<BLOCK, STACK, MINW, QUEUED, LHITS>), dim3(grid), dim3(BLOCK), 0, s, single, jobs,
+                       chunk_prefix, n_jobs, total_chunks, work_counter, chunk, static_chunks);
 }
 
 template <int BLOCK, int STACK, int MINW, int QUEUED, int LHITS>
