@@ -5,7 +5,7 @@ field with a 150 m tower and a cylindrical receiver (flux map 256 x 128, int64
 bins, at most 4 interactions per ray). Scene, BVH and outputs are resident in
 HBM before timing starts. Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--rays 1e8]
+  python bench.py [--gpus 1] [--steps 20] [--warmup 10] [--rays 1e8]
                   [--no-cpu-baseline] [--cpu-sample N] [--no-serve]
 """
 from __future__ import annotations
@@ -126,8 +126,8 @@ def flux_accuracy(gpu, ora, rays: int) -> dict:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--rays", type=float, default=1e8)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
