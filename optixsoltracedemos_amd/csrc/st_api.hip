@@ -92,13 +92,15 @@ LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kerne
     } else {
         sh.queued = kernel == ST_KERNEL_AUTO ? kDefaultQueued : want_queued;
         sh.block = block > 0 ? block : 256;
-        sh.grid = grid > 0 ? grid : cus * 8;
+        sh.grid = grid > 0 ? grid : 0;  // 0: set below from the variant's residency
         sh.stack = bvh_depth <= 16 && sh.block >= 128 ? 16 : 32;
         sh.waves = sh.queued ? (sh.stack == 16 ? 5 : 3) : (sh.stack == 16 ? 6 : 4);
+        // listed launches (their BVH walks are rare) keep the stack in private
+        // memory, which leaves LDS for 7 waves per SIMD
         if (listed && sh.queued && waves <= 0 && bvh_depth <= kStackDepth &&
-            trace_variant_exists(sh.block, 0, 6, true)) {
+            trace_variant_exists(sh.block, 0, 7, true)) {
             sh.stack = 0;
-            sh.waves = 6;
+            sh.waves = 7;
         }
     }
     if (waves > 0 && !use_tuned) {
@@ -118,12 +120,19 @@ LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kerne
     }
     if (sh.queued && !want_grouped && min_elements <= kLdsHitElements && block <= 0 && waves <= 0 &&
         (bvh_depth <= 16 || (listed && bvh_depth <= kStackDepth))) {
-        // listed launches (their BVH walks are rare) keep the stack in private
-        // memory, which leaves LDS for 6 waves per SIMD
+        // listed launches keep the stack in private memory (7 waves per SIMD)
         sh.block = 256;
         sh.stack = listed && bvh_depth <= kStackDepth ? 0 : bvh_depth <= 8 ? 8 : 16;
-        sh.waves = sh.stack == 0 ? 6 : 5;
+        sh.waves = sh.stack == 0 ? 7 : 5;
         sh.lds_hits = true;
+        if (grid <= 0) sh.grid = 0;
+    }
+    // persistent grid: as many blocks as the device holds at once (a block
+    // beyond that starts only when a resident one ends, near the launch's end,
+    // and takes chunks there)
+    if (sh.grid <= 0) {
+        const int rw = trace_resident_waves(sh);
+        sh.grid = rw > 0 ? rw / (sh.block / 64) : cus * 8;
     }
     // Chunks of 64 to 2048 rays. Every fetch from the shared chunk counter (a
     // single address, ~20 ns each in series) costs launch time, so a launch
@@ -1185,7 +1194,9 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
     if (log_trials)
         fprintf(stderr, "st_scene_autotune: default block %d stack %d waves %d queued %d grid %d chunk %d: %.3f ms\n",
                 def.block, def.stack, def.waves, (int)def.queued, def.grid, def.chunk, def_ms);
-    const int blocks[] = {64, 128, 256}, stacks[] = {0, 16, 32}, wavesv[] = {3, 4, 5, 6, 8}, per_cu[] = {4, 8, 16};
+    // grids: 0 = the blocks the device holds at once, else 4 / 8 / 16 x 256 threads per CU
+    const int blocks[] = {64, 128, 256}, stacks[] = {0, 16, 32}, wavesv[] = {3, 4, 5, 6, 7, 8},
+              per_cu[] = {0, 4, 8, 16};
     for (int rf = 0; rf < 2; ++rf)
     for (int b : blocks)
         for (int stk : stacks)
@@ -1198,7 +1209,13 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
                     sh.stack = stk;
                     sh.waves = w;
                     sh.queued = rf != 0;
-                    sh.grid = cus * pc * 256 / b;
+                    if (pc > 0) {
+                        sh.grid = cus * pc * 256 / b;
+                    } else {
+                        const int rw = trace_resident_waves(sh);
+                        if (rw <= 0) continue;
+                        sh.grid = rw / (b / 64);
+                    }
                     const uint64_t wv = (uint64_t)sh.grid * (uint64_t)(b / 64);
                     uint64_t chunk = (sample_rays / (4 * wv) + 63) / 64 * 64;
                     sh.chunk = (int)std::min<uint64_t>(std::max<uint64_t>(chunk, 64), 2048);

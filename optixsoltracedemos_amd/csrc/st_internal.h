@@ -104,13 +104,16 @@ struct LaunchShape {
     bool queued = false; // continuing rays parked in LDS, traced in full waves
     bool lds_hits = false;  // hit counts of elements below kLdsHitElements kept per wave in
                             // LDS (launches with a scene that small; queued, block 256, stack 8 or
-                            // 16 at 5 waves/SIMD, or stack 0 at 6 for listed launches)
+                            // 16 at 5 waves/SIMD, or stack 0 at 6 or 7 for listed launches)
     bool group_bvh = false; // queued form tracing list and BVH segments in separate steps
-                            // (use_grouped_bvh; compiled for stack 16 at 5 waves/SIMD and stack 0 at 6 or 8)
+                            // (use_grouped_bvh; compiled for stack 16 at 5 waves/SIMD and stack 0 at 6, 7 or 8)
 };
 // Scenes this small count element hits in LDS: their few counters would
 // otherwise take a device-scope atomic from nearly every wave of the launch.
 constexpr int kLdsHitElements = 128;
+// a compiled trace variant: form 0 whole rays, 1 queued, 2 queued with list and
+// BVH segments in separate steps
+bool trace_variant_compiled(int block, int stack, int waves, int form, bool lds_hits);
 bool trace_variant_exists(int block, int stack, int waves, bool queued);
 // traversal stack entries of a shape's stack setting
 inline int stack_entries(int stack) { return stack > 0 ? stack : kStackDepth; }
@@ -120,10 +123,7 @@ bool grouped_variant_exists(int block, int stack, int waves);
 // second mirror, and those without a second-reflection list, walk it) traces
 // the two kinds of segment in separate steps when its shape allows it.
 inline void use_grouped_bvh(LaunchShape& sh, bool mixed) {
-    // with LDS hit counts: compiled for block 256, stack in private memory, 6 waves/SIMD
-    sh.group_bvh = mixed && sh.queued &&
-                   (sh.lds_hits ? sh.block == 256 && sh.stack == 0 && sh.waves == 6
-                                : grouped_variant_exists(sh.block, sh.stack, sh.waves));
+    sh.group_bvh = mixed && sh.queued && trace_variant_compiled(sh.block, sh.stack, sh.waves, 2, sh.lds_hits);
 }
 // waves of this trace variant the device holds at once (0 if unknown)
 int trace_resident_waves(const LaunchShape& shape);

@@ -2016,26 +2016,6 @@ static int resident_blocks_per_cu() {
     return nb;
 }
 
-// Compiled shapes. The queued form keeps 2 KB of parked rays per wave in
-// LDS on top of the stack, which caps it at 5 waves per SIMD (stack 16) or 3
-// (stack 32), or 8 with the stack in private memory (stack 0); its register
-// budgets are chosen to match.
-bool grouped_variant_exists(int block, int stack, int waves) {
-    if (block != 128 && block != 256) return false;
-    return (stack == 16 && waves == 5) || (stack == 0 && (waves == 6 || waves == 8));
-}
-bool trace_variant_exists(int block, int stack, int waves, bool queued) {
-    if (queued) {
-        if (stack == 0) return (block == 128 || block == 256) && (waves == 6 || waves == 8);
-        if (stack == 16) return (block == 128 || block == 256) && (waves == 4 || waves == 5);
-        if (stack == 32) return (block == 64 || block == 128) && waves == 3;
-        return false;
-    }
-    if (stack == 32) return ((block == 64 || block == 128) && (waves == 4 || waves == 5)) || (block == 256 && waves == 4);
-    if (stack == 16) return (block == 128 || block == 256) && (waves == 4 || waves == 6 || waves == 8);
-    return false;
-}
-
 // Q: 0 whole rays, 1 queued, 2 queued with BVH segments grouped
 #define ST_VARIANTS                    \
     ST_VARIANT(64, 32, 4, 0, 0)        \
@@ -2056,19 +2036,46 @@ bool trace_variant_exists(int block, int stack, int waves, bool queued) {
     ST_VARIANT(64, 32, 3, 1, 0)        \
     ST_VARIANT(128, 32, 3, 1, 0)       \
     ST_VARIANT(128, 0, 6, 1, 0)        \
+    ST_VARIANT(128, 0, 7, 1, 0)        \
     ST_VARIANT(128, 0, 8, 1, 0)        \
     ST_VARIANT(256, 0, 6, 1, 0)        \
+    ST_VARIANT(256, 0, 7, 1, 0)        \
     ST_VARIANT(256, 0, 8, 1, 0)        \
     ST_VARIANT(128, 16, 5, 2, 0)       \
     ST_VARIANT(256, 16, 5, 2, 0)       \
     ST_VARIANT(128, 0, 6, 2, 0)        \
+    ST_VARIANT(128, 0, 7, 2, 0)        \
     ST_VARIANT(128, 0, 8, 2, 0)        \
     ST_VARIANT(256, 0, 6, 2, 0)        \
+    ST_VARIANT(256, 0, 7, 2, 0)        \
     ST_VARIANT(256, 0, 8, 2, 0)        \
     ST_VARIANT(256, 8, 5, 1, kLdsHitElements) \
     ST_VARIANT(256, 16, 5, 1, kLdsHitElements) \
     ST_VARIANT(256, 0, 6, 1, kLdsHitElements) \
-    ST_VARIANT(256, 0, 6, 2, kLdsHitElements)
+    ST_VARIANT(256, 0, 6, 2, kLdsHitElements) \
+    ST_VARIANT(256, 0, 7, 1, kLdsHitElements) \
+    ST_VARIANT(256, 0, 7, 2, kLdsHitElements)
+
+// Compiled shapes (the list above). The queued form keeps 2 KB of parked rays
+// per wave in LDS on top of the stack, which caps it at 5 waves per SIMD (stack
+// 16) or 3 (stack 32); with the stack in private memory (stack 0) LDS allows 8,
+// and the register budgets of 6, 7 and 8 waves are compiled (72 registers at 7
+// waves hold the queued form's state with two more spilled values than 80 at
+// 6; 64 at 8 spill inside the loops).
+bool trace_variant_compiled(int block, int stack, int waves, int form, bool lds_hits) {
+    const int h = lds_hits ? kLdsHitElements : 0;
+#define ST_VARIANT(B, S, W, Q, H) \
+    if (block == B && stack == S && waves == W && form == Q && h == H) return true;
+    ST_VARIANTS
+#undef ST_VARIANT
+    return false;
+}
+bool grouped_variant_exists(int block, int stack, int waves) {
+    return trace_variant_compiled(block, stack, waves, 2, false);
+}
+bool trace_variant_exists(int block, int stack, int waves, bool queued) {
+    return trace_variant_compiled(block, stack, waves, queued ? 1 : 0, false);
+}
 
 int trace_resident_waves(const LaunchShape& shape) {
     const int st = shape.stack, w = shape.waves, b = shape.block, h = shape.lds_hits ? kLdsHitElements : 0;

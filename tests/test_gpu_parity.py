@@ -646,7 +646,7 @@ def test_invalid_trace_requests_are_rejected(gpu_ctx):
         sc.trace(1000, kernel=_abi.KERNEL_QUEUED_GROUPED, block_size=64)
     # so is a register budget without a compiled variant
     with pytest.raises(_abi.STError, match="waves/SIMD"):
-        sc.trace(1000, kernel=_abi.KERNEL_QUEUED, block_size=256, waves_per_simd=7)
+        sc.trace(1000, kernel=_abi.KERNEL_QUEUED, block_size=256, waves_per_simd=2)
     with pytest.raises(_abi.STError, match="waves/SIMD"):
         sc.trace(1000, kernel=_abi.KERNEL_QUEUED, block_size=64, waves_per_simd=8)
 
