@@ -1232,8 +1232,8 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
                 }
     // the queued winner with list and BVH segments traced in separate steps
     // (pays where many segments have no list, e.g. rays after two mirrors at
-    // low sun): kept if faster at all, without the 2 % margin of the other
-    // trials, as the form that holds up when the sun moves lower
+    // low sun): kept unless it is more than 1 % slower, as the form that holds
+    // up when the sun moves lower (at high sun the two run within noise)
     if (rc == ST_OK && best.queued && !best.lds_hits && grouped_variant_exists(best.block, best.stack, best.waves)) {
         LaunchShape sh = best;
         sh.group_bvh = true;
@@ -1243,7 +1243,7 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
         if (log_trials)
             fprintf(stderr, "st_scene_autotune: block %d stack %d waves %d queued 2 grid %d chunk %d: %.3f ms\n",
                     sh.block, sh.stack, sh.waves, sh.grid, sh.chunk, ms);
-        if (rc == ST_OK && ms < best_ms) {
+        if (rc == ST_OK && ms < 1.01 * best_ms) {
             best_ms = ms;
             best = sh;
         }
