@@ -171,40 +171,44 @@ def quick_latency(ctx, n_jobs=400, seed=0, serving_scenes=None):
     return run_stream(ctx, n_jobs=n_jobs, seed=seed, serving_scenes=serving_scenes)
 
 
-def compare_plain(ctx, n_jobs=200, seed=0, flux=(64, 32), max_active_jobs=64, serving_scenes=None, plain=True,
-                  weights=(0.4, 0.3, 0.2, 0.1), sampling=_abi.POS_STRATIFIED, rays_hi=1e8, **server_kw):
-    """Aggregate throughput of the served form against the plain per-scene loop
-    on the same jobs of the serving mix (all submitted at once: throughput, not
-    latency). Plain loop: per job, on a device-resident scene of its kind, set
-    the sun and re-aim the field in one set-up (st_scene_set_sun_and_aim),
-    trace synchronously and copy the flux map to the host (what a caller of
-    lib/soltrace_system.cpp's update()/run() does per scene state); the served
-    timer includes fetching every map to the host too. Both give every job's
-    flux map bitwise the same."""
+def served_mix(ctx, n_jobs=200, seed=0, flux=(64, 32), max_active_jobs=64, serving_scenes=None,
+               weights=(0.4, 0.3, 0.2, 0.1), sampling=_abi.POS_STRATIFIED, rays_hi=1e8, **server_kw):
+    """The served half of compare_plain: n_jobs jobs of the serving mix, all
+    submitted at once to one server, then waited for and their maps fetched.
+    Returns (maps, seconds, server stats)."""
     import torch
     sc_list = serving_scenes or build_serving_scenes(ctx)
     _, kind, suns, rays = poisson_jobs(n_jobs, seed=seed, weights=weights, rays_hi=rays_hi)
-    # served: one server, every job submitted, then waited for and fetched
     srv = Server(ctx, max_active_jobs=max_active_jobs, **server_kw)
     slots = [srv.add_scene(s) for _, _, s in sc_list]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     hs = [srv.submit(slots[kind[i]], suns[i], int(rays[i]), seed=seed, job_id=i, flux_nx=flux[0], flux_ny=flux[1],
                      position_sampling=sampling) for i in range(n_jobs)]
-    served = []
+    maps = []
     for i, h in enumerate(hs):
         r = srv.wait(h)
         if r["status"] != 0:
             raise RuntimeError(f"job {i} failed: {r['status']}")
         n_el = len(sc_list[kind[i]][1].elements)
-        served.append(srv.fetch(h, n_el, (flux[1], flux[0]))[0])
+        maps.append(srv.fetch(h, n_el, (flux[1], flux[0]))[0])
         srv.release(h)
-    t_served = time.perf_counter() - t0
+    seconds = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
-    if not plain:
-        return {"jobs": n_jobs, "served_s": t_served, "launches": int(st["launches"]), "gpu_busy_s": st["busy_s"]}
-    # plain loop on scenes of its own
+    return maps, seconds, st
+
+
+def plain_loop(ctx, n_jobs=200, seed=0, flux=(64, 32), serving_scenes=None, weights=(0.4, 0.3, 0.2, 0.1),
+               sampling=_abi.POS_STRATIFIED, rays_hi=1e8):
+    """The plain half of compare_plain: per job, on a device-resident scene of
+    its kind, set the sun and re-aim the field in one set-up
+    (st_scene_set_sun_and_aim), trace synchronously and copy the flux map to
+    the host (what a caller of lib/soltrace_system.cpp's update()/run() does per
+    scene state). Returns (maps, seconds)."""
+    import torch
+    sc_list = serving_scenes or build_serving_scenes(ctx)
+    _, kind, suns, rays = poisson_jobs(n_jobs, seed=seed, weights=weights, rays_hi=rays_hi)
     plain_scenes = []
     for _, w, _ in sc_list:
         ps = tracer.Scene(ctx, w.elements)
@@ -213,16 +217,31 @@ def compare_plain(ctx, n_jobs=200, seed=0, flux=(64, 32), max_active_jobs=64, se
         plain_scenes.append((w, ps, res))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    plain = []
+    maps = []
     for i in range(n_jobs):
         w, ps, res = plain_scenes[kind[i]]
         ps.set_sun_and_aim(suns[i], 4.65e-3, w.elements[w.receiver]["origin"], position_sampling=sampling)
         ps.trace_into(ps.make_desc(int(rays[i]), seed=seed, job_id=i, max_depth=5, flux_nx=flux[0], flux_ny=flux[1],
                                    zero=True), res)
-        plain.append(res.flux.cpu().numpy())
-    t_plain = time.perf_counter() - t0
-    same = all(np.array_equal(a, b) for a, b in zip(served, plain))
-    total = int(rays.sum())
+        maps.append(res.flux.cpu().numpy())
+    return maps, time.perf_counter() - t0
+
+
+def compare_plain(ctx, n_jobs=200, seed=0, flux=(64, 32), max_active_jobs=64, serving_scenes=None, plain=True,
+                  weights=(0.4, 0.3, 0.2, 0.1), sampling=_abi.POS_STRATIFIED, rays_hi=1e8, **server_kw):
+    """Aggregate throughput of the served form against the plain per-scene loop
+    on the same jobs of the serving mix (all submitted at once: throughput, not
+    latency): served_mix, then plain_loop (skipped when plain is False). The
+    served timer includes fetching every map to the host too. Both give every
+    job's flux map bitwise the same."""
+    sc_list = serving_scenes or build_serving_scenes(ctx)
+    served, t_served, st = served_mix(ctx, n_jobs, seed, flux, max_active_jobs, sc_list, weights, sampling, rays_hi,
+                                      **server_kw)
+    if not plain:
+        return {"jobs": n_jobs, "served_s": t_served, "launches": int(st["launches"]), "gpu_busy_s": st["busy_s"]}
+    plain_maps, t_plain = plain_loop(ctx, n_jobs, seed, flux, sc_list, weights, sampling, rays_hi)
+    same = all(np.array_equal(a, b) for a, b in zip(served, plain_maps))
+    total = int(poisson_jobs(n_jobs, seed=seed, weights=weights, rays_hi=rays_hi)[3].sum())
     return {"jobs": n_jobs, "total_rays": total, "served_s": t_served, "plain_loop_s": t_plain,
             "max_active_jobs": max_active_jobs,
             "served_rays_per_s": total / t_served, "plain_rays_per_s": total / t_plain,

@@ -402,6 +402,33 @@ def test_reaim_candidate_lists_equal_the_bvh(gpu_ctx, elevation):
             sc.set_sun(scenes.sun_vector(elevation + 4.0, 210.0), w.sun_param, position_sampling=_abi.POS_STRATIFIED)
 
 
+def test_register_budgets_trace_identically(gpu_ctx):
+    """The queued forms compiled for 6, 7 and 8 waves per SIMD (stack in private
+    memory; 80, 72 and 64 registers) and traced on grids at, below and above
+    the device's residency give the same results bitwise, with candidate lists
+    (low and high sun) and without (BVH walks after a mirror)."""
+    w = scenes.heliostat_field(2_000)
+    sc = tracer.Scene(gpu_ctx, w.elements)
+    kw = dict(max_depth=5, flux_nx=w.flux_nx, flux_ny=w.flux_ny, seed=5, records=True)
+    for elevation, lists in ((20.0, True), (65.0, True), (45.0, False)):
+        sc.set_sun_and_aim(scenes.sun_vector(elevation, 160.0), w.sun_param, w.extra["target"],
+                           position_sampling=_abi.POS_STRATIFIED)
+        if lists:
+            sc.build_lists()
+        ref = None
+        for kernel in (_abi.KERNEL_QUEUED, _abi.KERNEL_QUEUED_GROUPED):
+            for waves, grid in ((6, 0), (7, 0), (8, 0), (7, 1000), (7, 4096)):
+                got = sc.trace(700_000, kernel=kernel, block_size=128, grid_blocks=grid, waves_per_simd=waves,
+                               **kw).numpy()
+                li = sc.last_launch()
+                assert li["waves_per_simd"] == waves and li["stack"] == 0, li
+                if ref is None:
+                    ref = got
+                    continue
+                for k in ("flux", "element_hits", "stats", "records", "record_elements"):
+                    assert np.array_equal(got[k], ref[k]), (elevation, kernel, waves, grid, k)
+
+
 def test_lists_build_lazily_and_follow_the_sun(gpu_ctx):
     """Reflection lists appear only after 2^26 traced rays on a scene state (or
     an explicit build) and are dropped by a sun change; results never depend
