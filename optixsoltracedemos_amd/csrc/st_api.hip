@@ -77,7 +77,7 @@ int DevBuf::ensure(size_t n, const char* what) {
 }
 
 LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel, int32_t bvh_depth,
-                       const LaunchShape* tuned, int32_t min_elements, int32_t waves, bool listed) {
+                       const LaunchShape* tuned, int32_t min_elements, int32_t waves) {
     LaunchShape sh;
     const int cus = device_cu_count();
     // ST_KERNEL_MEGA: whole ray per lane; ST_KERNEL_QUEUED: continuing rays
@@ -95,10 +95,11 @@ LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kerne
         sh.grid = grid > 0 ? grid : 0;  // 0: set below from the variant's residency
         sh.stack = bvh_depth <= 16 && sh.block >= 128 ? 16 : 32;
         sh.waves = sh.queued ? (sh.stack == 16 ? 5 : 3) : (sh.stack == 16 ? 6 : 4);
-        // listed launches (their BVH walks are rare) keep the stack in private
-        // memory, which leaves LDS for 7 waves per SIMD
-        if (listed && sh.queued && waves <= 0 && bvh_depth <= kStackDepth &&
-            trace_variant_exists(sh.block, 0, 7, true)) {
+        // the queued form keeps the traversal stack in private memory where the
+        // BVH fits it, which leaves LDS for 7 waves per SIMD: with candidate
+        // lists BVH walks are rare; without them the walks go through the
+        // scratch path, and the extra waves still win
+        if (sh.queued && waves <= 0 && bvh_depth <= kStackDepth && trace_variant_exists(sh.block, 0, 7, true)) {
             sh.stack = 0;
             sh.waves = 7;
         }
@@ -119,11 +120,11 @@ LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kerne
         }
     }
     if (sh.queued && !want_grouped && min_elements <= kLdsHitElements && block <= 0 && waves <= 0 &&
-        (bvh_depth <= 16 || (listed && bvh_depth <= kStackDepth))) {
-        // listed launches keep the stack in private memory (7 waves per SIMD)
+        bvh_depth <= kStackDepth) {
+        // the stack in private memory (7 waves per SIMD)
         sh.block = 256;
-        sh.stack = listed && bvh_depth <= kStackDepth ? 0 : bvh_depth <= 8 ? 8 : 16;
-        sh.waves = sh.stack == 0 ? 7 : 5;
+        sh.stack = 0;
+        sh.waves = 7;
         sh.lds_hits = true;
         if (grid <= 0) sh.grid = 0;
     }
@@ -1010,7 +1011,7 @@ namespace st {
 // second-reflection lists).
 int trace_shape(st_scene s, const st_trace_desc* d, const JobDev& J, LaunchShape* out, uint64_t* chunks) {
     LaunchShape sh = pick_shape(J.ray_count, d->block_size, d->grid_blocks, d->kernel, s->topo.depth,
-                                s->tuned_valid ? &s->tuned : nullptr, s->n, d->waves_per_simd, J.bounce_range != nullptr);
+                                s->tuned_valid ? &s->tuned : nullptr, s->n, d->waves_per_simd);
     if (d->waves_per_simd > 0 && !trace_variant_exists(sh.block, sh.stack, sh.waves, sh.queued))
         return fail(ST_ERR_INVALID, "no %s trace variant for block %d at %d waves/SIMD (BVH depth %d)",
                     sh.queued ? "queued" : "whole-ray", sh.block, d->waves_per_simd, s->topo.depth);
@@ -1184,7 +1185,7 @@ extern "C" int st_scene_autotune(st_scene s, uint64_t sample_rays, int32_t max_d
         return ST_OK;
     };
     const int cus = device_cu_count();
-    const LaunchShape def = pick_shape(sample_rays, 0, 0, ST_KERNEL_AUTO, s->topo.depth, nullptr, s->n, 0, s->bounce_valid);
+    const LaunchShape def = pick_shape(sample_rays, 0, 0, ST_KERNEL_AUTO, s->topo.depth, nullptr, s->n, 0);
     double def_ms = 0;
     int rc = time_shape(def, &def_ms);
     LaunchShape best = def;
@@ -1513,7 +1514,7 @@ extern "C" int st_trace_batch(st_scene s, const st_batch_desc* d, int64_t* flux,
         uint64_t total_rays = 0;
         for (int k = 0; k < K; ++k) total_rays += d->budgets[k];
         b.shape = pick_shape(total_rays, d->block_size, d->grid_blocks, ST_KERNEL_AUTO, s->topo.depth,
-                             s->tuned_valid ? &s->tuned : nullptr, s->n, 0, b.lists);
+                             s->tuned_valid ? &s->tuned : nullptr, s->n, 0);
         use_grouped_bvh(b.shape, b.lists);  // per-step first-reflection lists, no pair lists
         std::vector<JobDev> jobs(K);
         std::vector<uint64_t> prefix(K + 1, 0);

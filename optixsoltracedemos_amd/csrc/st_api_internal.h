@@ -31,19 +31,18 @@ struct DevBuf {
 };
 
 // launch shape for `rays` rays; block/grid/waves > 0 override, else the scene's
-// tuned shape when there is one, else defaults (16-entry stack when the BVH
-// allows); bvh_depth: the deepest scene in the launch; min_elements: the
-// smallest scene in the launch (when it has at most kLdsHitElements elements,
-// the launch counts element hits below that index per wave in LDS: a small
-// scene's few counters would otherwise take every wave's device atomics).
-// An explicit waves-per-SIMD budget takes the smallest compiled stack that holds
-// the BVH; the caller checks that the variant exists (trace_variant_exists).
-// listed: the launch's reflected rays take candidate lists, so its BVH walks
-// are rare and the default queued shape keeps the stack in private memory (6
-// waves per SIMD with 80 registers instead of 5; the 8-wave variants' 64
-// registers spill more than the extra waves repay).
+// tuned shape when there is one, else defaults: the queued form with the
+// traversal stack in private memory when the BVH fits it, at 7 waves per SIMD
+// (72 registers; the LDS-stack variants hold 5, the 8-wave variants' 64
+// registers spill inside the loops), on as many blocks as the device holds at
+// once. bvh_depth: the deepest scene in the launch; min_elements: the smallest
+// scene in the launch (when it has at most kLdsHitElements elements, the
+// launch counts element hits below that index per wave in LDS: a small scene's
+// few counters would otherwise take every wave's device atomics). An explicit
+// waves-per-SIMD budget takes the smallest compiled stack that holds the BVH;
+// the caller checks that the variant exists (trace_variant_exists).
 LaunchShape pick_shape(uint64_t rays, int32_t block, int32_t grid, int32_t kernel, int32_t bvh_depth,
-                       const LaunchShape* tuned, int32_t min_elements, int32_t waves = 0, bool listed = false);
+                       const LaunchShape* tuned, int32_t min_elements, int32_t waves = 0);
 
 // First-reflection candidates of a scene's mirrors over every bisector re-aim
 // at a target (k_reaim_lists): while every mirror is aimed by the bisector rule

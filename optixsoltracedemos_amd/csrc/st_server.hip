@@ -539,7 +539,7 @@ void worker_main(st_server srv) {
         bool lists = false;  // long jobs have first-reflection lists and no pair lists
         for (ServerJob* j : round) lists = lists || j->lists;
         LaunchShape sh = pick_shape(round_rays, srv->cfg.block_size, srv->cfg.grid_blocks, ST_KERNEL_AUTO, depth,
-                                    nullptr, min_n, 0, lists);
+                                    nullptr, min_n, 0);
         use_grouped_bvh(sh, lists);
         // slot `slot` was last used two rounds ago, and that round has retired
         const int slot = next_slot;
