@@ -172,12 +172,14 @@ typedef struct st_trace_desc {
     int32_t flux_element;   /* element binned into the flux map; -1 = last receiver */
     int32_t flux_nx, flux_ny; /* flux grid; 0 -> no flux map                      */
     int32_t block_size;     /* 0 -> tuned default                                 */
-    int32_t grid_blocks;    /* 0 -> tuned default (persistent kernel)              */
-    int32_t waves_per_simd; /* 0 -> tuned default; else the register budget of the
-                               kernel variant to run (queued forms: 4 or 5 with the
-                               traversal stack in LDS, 6 or 8 with it in private
-                               memory; whole-ray form: 4, 5, 6 or 8); a block size
-                               and budget without a compiled variant is an error */
+    int32_t grid_blocks;    /* 0 -> tuned default (persistent kernel; untuned: the
+                               blocks the device holds at once)                   */
+    int32_t waves_per_simd; /* 0 -> tuned default (untuned queued forms: 7 with the
+                               traversal stack in private memory); else the register
+                               budget of the kernel variant to run (queued forms: 3,
+                               4 or 5 with the stack in LDS, 6, 7 or 8 with it in
+                               private memory; whole-ray form: 4, 5, 6 or 8); a block
+                               size and budget without a compiled variant is an error */
     void* stream;           /* hipStream_t; NULL = the scene's own stream          */
 } st_trace_desc;
 
