@@ -2,7 +2,8 @@
 //
 //   k_frames / k_frames_batch  FP64 element frames, reference AABBs, FP32 packing
 //                              (lib/element.cpp:77-275, lib/math_util.h:21-59)
-//   k_refit                    bottom-up BVH refit, one level per launch
+//   k_refit_levels / k_refit   bottom-up BVH refit: all levels in one launch (one block per
+//                              step), or one launch per level for large trees
 //                              (replaces optixAccelBuild UPDATE, lib/geometry_manager.cpp:249-262)
 //   k_sun_plane                FP64 sun plane (lib/sun_utils.cu:53-179,
 //                              lib/geometry_manager.cpp:97-148)
@@ -1344,6 +1345,37 @@ __global__ void k_refit(const int32_t* __restrict__ child, const int32_t* __rest
     N[4 * node + 3] = make_float4(__int_as_float(c0), __int_as_float(c1), 0.f, 0.f);
 }
 
+// The whole refit in one launch: one block per step walks the levels bottom-up,
+// a barrier between levels (a level reads the boxes the level below wrote, in
+// this block). For BVHs of at most kRefitOneLaunchNodes nodes and
+// kRefitMaxLevels levels; larger ones take k_refit, one launch per level.
+constexpr int32_t kRefitMaxLevels = 32, kRefitOneLaunchNodes = 1 << 16;
+struct RefitLevels {
+    int32_t offset[kRefitMaxLevels + 1];  // level l's nodes: level_nodes[offset[l], offset[l + 1])
+};
+__global__ __launch_bounds__(1024) void k_refit_levels(const int32_t* __restrict__ child,
+                                                       const int32_t* __restrict__ level_nodes, RefitLevels lv,
+                                                       int32_t levels, int32_t n_nodes, int32_t n_prims,
+                                                       const float* __restrict__ bounds_slot, float4* __restrict__ nodes) {
+    const int k = blockIdx.x;
+    float4* N = nodes + (size_t)k * n_nodes * 4;
+    const float* B = bounds_slot + (size_t)k * n_prims * 6;
+    for (int l = levels - 1; l >= 0; --l) {
+        for (int32_t i = lv.offset[l] + (int32_t)threadIdx.x; i < lv.offset[l + 1]; i += (int32_t)blockDim.x) {
+            const int32_t node = level_nodes[i];
+            const int32_t c0 = child[2 * node], c1 = child[2 * node + 1];
+            float l0[3], h0[3], l1[3], h1[3];
+            child_box(c0, N, B, l0, h0);
+            child_box(c1, N, B, l1, h1);
+            N[4 * node + 0] = make_float4(l0[0], h0[0], l0[1], h0[1]);
+            N[4 * node + 1] = make_float4(l1[0], h1[0], l1[1], h1[1]);
+            N[4 * node + 2] = make_float4(l0[2], h0[2], l1[2], h1[2]);
+            N[4 * node + 3] = make_float4(__int_as_float(c0), __int_as_float(c1), 0.f, 0.f);
+        }
+        __syncthreads();
+    }
+}
+
 // FP64 sun plane, one block per step (lib/sun_utils.cu:53-179 + geometry_manager.cpp:97-148)
 __global__ __launch_bounds__(256) void k_sun_plane(const double* __restrict__ ref_aabb, int32_t n,
                                                    const double* __restrict__ sun_units, st_sun_desc sun,
@@ -1975,6 +2007,13 @@ hipError_t launch_frames_batch(hipStream_t s, const st_element_f64* elems, int32
 hipError_t launch_refit(hipStream_t s, const int32_t* child, const int32_t* level_nodes,
                         const int32_t* level_offsets_host, int32_t levels, int32_t n_nodes, int32_t n_prims,
                         int32_t steps, const float* bounds_slot, float4* nodes) {
+    if (levels <= kRefitMaxLevels && n_nodes <= kRefitOneLaunchNodes) {
+        RefitLevels lv{};
+        for (int l = 0; l <= levels; ++l) lv.offset[l] = level_offsets_host[l];
+        hipLaunchKernelGGL(k_refit_levels, dim3((unsigned)steps), dim3(1024), 0, s, child, level_nodes, lv, levels,
+                           n_nodes, n_prims, bounds_slot, nodes);
+        return hipGetLastError();
+    }
     for (int lv = levels - 1; lv >= 0; --lv) {
         const int32_t b = level_offsets_host[lv], c = level_offsets_host[lv + 1] - b;
         if (c <= 0) continue;
