@@ -1500,35 +1500,54 @@ __global__ __launch_bounds__(1024) void k_grid_scan(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ start_all, uint32_t* __restrict__ stats_all,
                                                     JobDev* __restrict__ jobs, const int32_t* __restrict__ items_all,
                                                     uint32_t item_cap) {
-    constexpr int kPer = kGridCells / 1024;
-    __shared__ uint32_t part[1024], top[1024];
-    const int t = threadIdx.x, k = blockIdx.x;
-    const uint32_t* counts = counts_all + (size_t)k * kGridCells;
+    // each of the 16 waves owns 4,096 consecutive cells: their sum and longest
+    // list (coalesced 16-byte loads), then, offset by the waves before it, their
+    // prefix sums (a wave-wide scan of each lane's four counts per 256 cells)
+    constexpr int kWaves = 16, kSeg = kGridCells / kWaves, kIters = kSeg / 256;
+    static_assert(kGridCells % (kWaves * 256) == 0, "grid cells per wave");
+    __shared__ uint32_t wsum[kWaves], wmax[kWaves];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, k = blockIdx.x;
+    const uint4* counts = (const uint4*)(counts_all + (size_t)k * kGridCells + (size_t)wv * kSeg);
     uint32_t* start = start_all + (size_t)k * (kGridCells + 1);
     uint32_t sum = 0, mx = 0;
-    for (int i = 0; i < kPer; ++i) {
-        const uint32_t c = counts[t * kPer + i];
-        sum += c;
-        mx = max(mx, c);
+    for (int i = 0; i < kIters; ++i) {
+        const uint4 c = counts[i * 64 + lane];
+        sum += c.x + c.y + c.z + c.w;
+        mx = max(mx, max(max(c.x, c.y), max(c.z, c.w)));
     }
-    part[t] = sum;
-    top[t] = mx;
+    for (int d = 32; d > 0; d >>= 1) {
+        sum += __shfl_xor(sum, d);
+        mx = max(mx, (uint32_t)__shfl_xor(mx, d));
+    }
+    if (lane == 0) {
+        wsum[wv] = sum;
+        wmax[wv] = mx;
+    }
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = t >= off ? part[t - off] : 0u;
-        const uint32_t w = t >= off ? top[t - off] : 0u;
-        __syncthreads();
-        part[t] += v;
-        top[t] = max(top[t], w);
-        __syncthreads();
+    uint32_t carry = 0, total = 0, longest = 0;
+    for (int j = 0; j < kWaves; ++j) {
+        if (j < wv) carry += wsum[j];
+        total += wsum[j];
+        longest = max(longest, wmax[j]);
     }
-    uint32_t run = t ? part[t - 1] : 0u;
-    for (int i = 0; i < kPer; ++i) {
-        start[t * kPer + i] = run;
-        run += counts[t * kPer + i];
+    uint32_t* out = start + (size_t)wv * kSeg;
+    for (int i = 0; i < kIters; ++i) {
+        const uint4 c = counts[i * 64 + lane];
+        const uint32_t own = c.x + c.y + c.z + c.w;
+        uint32_t inc = own;  // inclusive scan over the wave's lanes
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t v = __shfl_up(inc, d);
+            if (lane >= d) inc += v;
+        }
+        const uint32_t ex = carry + inc - own;
+        uint32_t* o = out + (size_t)(i * 64 + lane) * 4;
+        o[0] = ex;
+        o[1] = ex + c.x;
+        o[2] = ex + c.x + c.y;
+        o[3] = ex + c.x + c.y + c.z;
+        carry += __shfl(inc, 63);
     }
-    if (t == 1023) {
-        const uint32_t total = part[1023], longest = top[1023];
+    if (t == 0) {
         start[kGridCells] = total;
         stats_all[2 * k] = total;
         stats_all[2 * k + 1] = longest;
