@@ -137,6 +137,15 @@ hipError_t launch_frames(hipStream_t s, const st_element_f64* elems, int32_t n, 
                          float4* optics);
 // FP64 bisector aim of all mirrors (in place on the device element array)
 hipError_t launch_aim(hipStream_t s, st_element_f64* elems, int32_t n, const double* sun_unit, const double* target);
+// statistics (ST_STAT_COUNT u64) and sun frame of each of `count` jobs, gathered
+// into stats_out[count][ST_STAT_COUNT] and frames_out[count] (one launch per 32
+// jobs instead of two copies per job)
+hipError_t launch_round_results(hipStream_t s, const uint64_t* const* stats, const SunFrame* const* suns, int32_t count,
+                                uint64_t* stats_out, SunFrame* frames_out);
+// the same into a copy: dst = src with every mirror re-aimed (one pass instead
+// of a device copy and an in-place aim)
+hipError_t launch_aim_copy(hipStream_t s, const st_element_f64* src, st_element_f64* dst, int32_t n,
+                           const double* sun_unit, const double* target);
 // batched: per step k re-aim mirrors along the bisector of sun[k] and target
 hipError_t launch_frames_batch(hipStream_t s, const st_element_f64* elems, int32_t n, const int32_t* elem_slot,
                                int32_t steps, const double* sun_units /* steps*3 */, const double* target,

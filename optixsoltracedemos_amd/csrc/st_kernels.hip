@@ -1302,6 +1302,18 @@ __global__ void k_aim(st_element_f64* __restrict__ elems, int32_t n, const doubl
     for (int k = 0; k < 3; ++k) elems[e].aim_point[k] = aim[k];
 }
 
+__global__ void k_aim_copy(const st_element_f64* __restrict__ src, st_element_f64* __restrict__ dst, int32_t n,
+                           const double* __restrict__ sun_unit, const double* __restrict__ target) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    st_element_f64 E = src[e];
+    if (E.role == ST_ROLE_MIRROR) {
+        const double s[3] = {sun_unit[0], sun_unit[1], sun_unit[2]};
+        bisector_aim_point(E.origin, s, target, E.aim_point);
+    }
+    dst[e] = E;
+}
+
 __device__ __forceinline__ void child_box(int32_t code, const float4* __restrict__ nodes,
                                           const float* __restrict__ bounds, float lo[3], float hi[3]) {
     if (code >= 0) {
@@ -2011,6 +2023,45 @@ hipError_t launch_frames(hipStream_t s, const st_element_f64* elems, int32_t n, 
 
 hipError_t launch_aim(hipStream_t s, st_element_f64* elems, int32_t n, const double* sun_unit, const double* target) {
     hipLaunchKernelGGL(k_aim, dim3((n + 127) / 128), dim3(128), 0, s, elems, n, sun_unit, target);
+    return hipGetLastError();
+}
+
+constexpr int kRoundGather = 32;
+struct RoundResults {
+    const uint64_t* stats[kRoundGather];
+    const SunFrame* sun[kRoundGather];
+};
+// one block per job: its statistics and sun frame, word by word
+__global__ void k_round_results(RoundResults r, uint64_t* __restrict__ stats_out, SunFrame* __restrict__ frames_out) {
+    const int j = blockIdx.x, t = threadIdx.x;
+    if (t < ST_STAT_COUNT) stats_out[(size_t)j * ST_STAT_COUNT + t] = r.stats[j][t];
+    constexpr int kWords = (int)(sizeof(SunFrame) / sizeof(uint32_t));
+    static_assert(sizeof(SunFrame) % sizeof(uint32_t) == 0, "sun frame words");
+    const uint32_t* src = (const uint32_t*)r.sun[j];
+    uint32_t* dst = (uint32_t*)(frames_out + j);
+    for (int i = t; i < kWords; i += (int)blockDim.x) dst[i] = src[i];
+}
+
+hipError_t launch_round_results(hipStream_t s, const uint64_t* const* stats, const SunFrame* const* suns, int32_t count,
+                                uint64_t* stats_out, SunFrame* frames_out) {
+    for (int32_t b = 0; b < count; b += kRoundGather) {
+        const int32_t c = std::min<int32_t>(kRoundGather, count - b);
+        RoundResults r{};
+        for (int32_t i = 0; i < c; ++i) {
+            r.stats[i] = stats[b + i];
+            r.sun[i] = suns[b + i];
+        }
+        hipLaunchKernelGGL(k_round_results, dim3((unsigned)c), dim3(64), 0, s, r, stats_out + (size_t)b * ST_STAT_COUNT,
+                           frames_out + b);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_aim_copy(hipStream_t s, const st_element_f64* src, st_element_f64* dst, int32_t n,
+                           const double* sun_unit, const double* target) {
+    hipLaunchKernelGGL(k_aim_copy, dim3((n + 127) / 128), dim3(128), 0, s, src, dst, n, sun_unit, target);
     return hipGetLastError();
 }
 

@@ -101,6 +101,7 @@ struct st_server_s {
     std::thread worker;
     hipStream_t stream = nullptr, fetch_stream = nullptr;
     DevBuf d_jobs, d_prefix, d_counter;
+    DevBuf d_round_stats[2], d_round_frames[2];  // per round slot: results of the jobs it finishes
     // pinned staging of the two round slots
     JobDev* h_jobs[2] = {nullptr, nullptr};
     uint64_t* h_prefix[2] = {nullptr, nullptr};
@@ -204,13 +205,13 @@ int setup_job(st_server srv, ServerJob* j, int slot) {
     hipError_t e;
     if ((e = hipMemcpyAsync(w.aim_in.ptr, in, sizeof(j->aim_host), hipMemcpyHostToDevice, st)) != hipSuccess)
         return hip_fail(e, "job sun upload");
-    if ((e = hipMemcpyAsync(w.elems.ptr, sc->d_elems.ptr, sizeof(st_element_f64) * n, hipMemcpyDeviceToDevice, st)) !=
-        hipSuccess)
-        return hip_fail(e, "job elements copy");
-    if (!(d.flags & ST_JOB_KEEP_AIM) &&
-        (e = launch_aim(st, (st_element_f64*)w.elems.ptr, sc->n, (const double*)w.aim_in.ptr,
-                        (const double*)w.aim_in.ptr + 3)) != hipSuccess)
-        return hip_fail(e, "job aim");
+    // the job's copy of the elements, every mirror re-aimed at the job's sun
+    if (d.flags & ST_JOB_KEEP_AIM)
+        e = hipMemcpyAsync(w.elems.ptr, sc->d_elems.ptr, sizeof(st_element_f64) * n, hipMemcpyDeviceToDevice, st);
+    else
+        e = launch_aim_copy(st, (const st_element_f64*)sc->d_elems.ptr, (st_element_f64*)w.elems.ptr, sc->n,
+                            (const double*)w.aim_in.ptr, (const double*)w.aim_in.ptr + 3);
+    if (e != hipSuccess) return hip_fail(e, "job elements");
     if ((e = launch_frames(st, (const st_element_f64*)w.elems.ptr, sc->n, (const int32_t*)sc->d_elem_slot.ptr, nullptr,
                            nullptr, j->lists ? (double*)w.frames.ptr : nullptr, (double*)w.ref_aabb.ptr,
                            (float*)w.bounds.ptr, (float4*)w.prims.ptr, nullptr, nullptr)) != hipSuccess)
@@ -571,16 +572,26 @@ void worker_main(st_server srv) {
             round[i]->dispatched += slice[i];
             if (round[i]->dispatched >= round[i]->desc.num_rays) finished.push_back(round[i]);
         }
+        std::vector<const uint64_t*> fin_stats;
+        std::vector<const SunFrame*> fin_suns;
         for (size_t i = 0; i < finished.size() && e == hipSuccess; ++i) {
             ServerJob* j = finished[i];
             if (j->work->copies > 1)
                 e = launch_flux_gather(st, j->dev.flux, j->work->copies, j->dev.flux_bins, 1, (int64_t*)j->out->flux.ptr);
+            fin_stats.push_back((const uint64_t*)j->out->stats.ptr);
+            fin_suns.push_back((const SunFrame*)j->work->sun.ptr);
+        }
+        // their statistics and sun frames: gathered on the device, two copies
+        if (e == hipSuccess && !finished.empty()) {
+            const int32_t nf = (int32_t)finished.size();
+            e = launch_round_results(st, fin_stats.data(), fin_suns.data(), nf, (uint64_t*)srv->d_round_stats[slot].ptr,
+                                     (SunFrame*)srv->d_round_frames[slot].ptr);
             if (e == hipSuccess)
-                e = hipMemcpyAsync(srv->h_stats[slot] + i * ST_STAT_COUNT, j->out->stats.ptr,
-                                   sizeof(uint64_t) * ST_STAT_COUNT, hipMemcpyDeviceToHost, st);
+                e = hipMemcpyAsync(srv->h_stats[slot], srv->d_round_stats[slot].ptr, sizeof(uint64_t) * ST_STAT_COUNT * nf,
+                                   hipMemcpyDeviceToHost, st);
             if (e == hipSuccess)
-                e = hipMemcpyAsync(srv->h_frames[slot] + i, j->work->sun.ptr, sizeof(SunFrame), hipMemcpyDeviceToHost,
-                                   st);
+                e = hipMemcpyAsync(srv->h_frames[slot], srv->d_round_frames[slot].ptr, sizeof(SunFrame) * nf,
+                                   hipMemcpyDeviceToHost, st);
         }
         if (e == hipSuccess) e = hipEventRecord(srv->round_done[slot], st);
         if (e != hipSuccess) {
@@ -656,6 +667,9 @@ extern "C" int st_server_create(st_context ctx, const st_server_config* cfg, st_
         HIP_TRY(hipHostMalloc((void**)&srv->h_frames[k], sizeof(SunFrame) * J, hipHostMallocDefault),
                 "pinned sun frames");
         HIP_TRY(hipEventCreateWithFlags(&srv->round_done[k], hipEventDisableTiming), "round event");
+        if ((rc = srv->d_round_stats[k].ensure(sizeof(uint64_t) * ST_STAT_COUNT * J, "round statistics")) != ST_OK)
+            return rc;
+        if ((rc = srv->d_round_frames[k].ensure(sizeof(SunFrame) * J, "round sun frames")) != ST_OK) return rc;
     }
     HIP_TRY(hipHostMalloc((void**)&srv->h_grid_stats, sizeof(uint32_t) * 2 * J, hipHostMallocDefault), "pinned grid stats");
     if ((rc = srv->d_grid_counts.ensure(sizeof(uint32_t) * kGridCells, "server grid counts")) != ST_OK) return rc;
