@@ -980,6 +980,42 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(JobDev single, const JobD
     // each copy reads J from one address space (constant: scalar loads, never
     // reloaded after the kernel's own stores); one copy over a pointer that
     // may be either would read every field through generic loads.
+    // A job's counts (event counters, LDS hit counts) to its outputs. Rays,
+    // segments and escapes are not counted as they happen: every ray traced
+    // since the last flush has ended, each in exactly one of escaped, absorbed,
+    // back face, truncated or receiver hit, and each segment after the first
+    // follows a mirror hit that was not absorbed.
+    auto flush_counts = [&](const JobDev& J, uint32_t rays) __attribute__((always_inline)) {
+        if (LHITS > 0)
+            for (int i = lane; i < LHITS; i += 64) {
+                const uint32_t h = wh[i];
+                if (h) {
+                    wh[i] = 0;
+                    if (J.hits) gadd(J.hits + i, (unsigned long long)h);
+                }
+            }
+        uint32_t v = lane < kNumCounters ? wc[lane] : 0u;
+        if (lane == ST_STAT_RAYS) v = rays;
+        if (lane == ST_STAT_SEGMENTS) v = rays + wc[ST_STAT_MIRROR_HITS] - wc[ST_STAT_ABSORBED];
+        if (lane == ST_STAT_ESCAPED && !(J.flags & ST_TRACE_COUNT_WORK))
+            v = rays - wc[ST_STAT_ABSORBED] - wc[ST_STAT_BACK_FACE] - wc[ST_STAT_TRUNCATED] - wc[ST_STAT_RECEIVER_HITS];
+        if (lane < kNumCounters) wc[lane] = 0;
+        if (v) {
+            if (lane < kAggCounter) {
+                if (J.stats) gadd(J.stats + lane, (unsigned long long)v);
+            } else if (lane == kAggCounter && J.hits && J.agg_slot >= 0) {
+                gadd(J.hits + meta_elem(ld1(J.meta + J.agg_slot)), (unsigned long long)v);
+            }
+        }
+    };
+    // A launch of one job keeps its counts in the wave's LDS across the wave's
+    // chunks and flushes them every kFlushRays rays (a ray adds at most a few
+    // events: the u32 counters stay far from overflow) and when the wave ends,
+    // instead of at every chunk: each flush is ~10 device atomics on the same
+    // 16 addresses from every wave. Launches of several jobs, and jobs that
+    // count work (tens of node visits and tests per segment), flush per chunk.
+    constexpr uint32_t kFlushRays = 1u << 22;
+    uint32_t wave_rays = 0;  // rays of the single job since the wave's last flush (wave-uniform)
     auto run_chunk = [&](const JobDev& J, uint64_t cj) __attribute__((always_inline)) {
         const ST_CONST SunFrame& S = *(const ST_CONST SunFrame*)J.sun;
         const uint64_t first = cj * (uint64_t)chunk_rays;
@@ -1146,34 +1182,18 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(JobDev single, const JobD
                 finish_ray(J, ray, last);
             }
         }
-        // flush this chunk's counters (same wave wrote them: LDS order suffices)
+        // the chunk's log goes out (same wave wrote it: LDS order suffices)
         if (QUEUED) log_send(wl, J, len);
         else log_flush(wl, J);
-        if (LHITS > 0)
-            for (int i = lane; i < LHITS; i += 64) {
-                const uint32_t h = wh[i];
-                if (h) {
-                    wh[i] = 0;
-                    if (J.hits) gadd(J.hits + i, (unsigned long long)h);
-                }
-            }
-        // rays, segments and escapes are not counted as they happen: every ray
-        // of the chunk has ended, each in exactly one of escaped, absorbed, back
-        // face, truncated or receiver hit, and each segment after the first
-        // follows a mirror hit that was not absorbed
         const uint32_t rays = (uint32_t)(end - first);
-        uint32_t v = lane < kNumCounters ? wc[lane] : 0u;
-        if (lane == ST_STAT_RAYS) v = rays;
-        if (lane == ST_STAT_SEGMENTS) v = rays + wc[ST_STAT_MIRROR_HITS] - wc[ST_STAT_ABSORBED];
-        if (lane == ST_STAT_ESCAPED && !(J.flags & ST_TRACE_COUNT_WORK))
-            v = rays - wc[ST_STAT_ABSORBED] - wc[ST_STAT_BACK_FACE] - wc[ST_STAT_TRUNCATED] - wc[ST_STAT_RECEIVER_HITS];
-        if (lane < kNumCounters) wc[lane] = 0;
-        if (v) {
-            if (lane < kAggCounter) {
-                if (J.stats) gadd(J.stats + lane, (unsigned long long)v);
-            } else if (lane == kAggCounter && J.hits && J.agg_slot >= 0) {
-                gadd(J.hits + meta_elem(ld1(J.meta + J.agg_slot)), (unsigned long long)v);
+        if (!jobs && !(J.flags & ST_TRACE_COUNT_WORK)) {
+            wave_rays += rays;
+            if (wave_rays >= kFlushRays) {
+                flush_counts(J, wave_rays);
+                wave_rays = 0;
             }
+        } else {
+            flush_counts(J, rays);
         }
     };
     for (bool first_pass = true;; first_pass = false) {
@@ -1208,6 +1228,11 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_trace(JobDev single, const JobD
             asm volatile("" : "+s"(sp));
             run_chunk(*(const JobDev*)sp, c);
         }
+    }
+    if (!jobs && wave_rays > 0) {
+        const ST_CONST JobDev* sp = (const ST_CONST JobDev*)__builtin_amdgcn_kernarg_segment_ptr();
+        asm volatile("" : "+s"(sp));
+        flush_counts(*(const JobDev*)sp, wave_rays);
     }
     if (!static_chunks && lane == 0) {
         __threadfence();
