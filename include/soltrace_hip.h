@@ -284,7 +284,10 @@ typedef struct st_tune_result {
     int32_t queued;               /* 1: ST_KERNEL_QUEUED form chosen; 2: the same
                                      with list and BVH segments traced in separate
                                      steps                                        */
-    double best_ms, default_ms;   /* kernel time of the sample, tuned and untuned */
+    double best_ms, default_ms;   /* kernel time of the sample, tuned and untuned;
+                                     best_ms can exceed default_ms by up to 1 %:
+                                     queued = 2 is kept when within 1 % of the
+                                     fastest shape (it holds up at lower sun)     */
 } st_tune_result;
 int st_scene_autotune(st_scene scene, uint64_t sample_rays, int32_t max_depth, st_tune_result* out);
 

@@ -702,7 +702,8 @@ def test_autotune_keeps_results_bitwise(gpu_ctx):
     kw = dict(max_depth=5, flux_nx=w.flux_nx, flux_ny=w.flux_ny)
     before = sc.trace(3_000_000, **kw).numpy()
     t = sc.autotune(sample_rays=4_000_000)
-    assert t["trials"] > 1 and 0 < t["best_ms"] <= t["default_ms"]
+    # the grouped form (queued 2) is kept when within 1 % of the fastest shape
+    assert t["trials"] > 1 and 0 < t["best_ms"] <= 1.01 * t["default_ms"]
     # stack 0: the traversal stack in private memory, 32 entries
     assert t["block"] in (64, 128, 256) and (t["stack"] or 32) >= sc.bvh_info()["depth"]
     after = sc.trace(3_000_000, **kw).numpy()
