@@ -187,6 +187,8 @@ struct SunFrame {
     float vers;                     // pillbox: 1 - cos(alpha) = 2 sin^2(alpha/2)
     float sigma;                    // gaussian sigma
     int32_t shape, position_sampling;
+    int32_t grid_rows;              // rows of the sun-plane grid along e2 (frame_rows)
+    int32_t reserved;
 };
 
 // ---------------------------------------------------------------------------
@@ -332,8 +334,13 @@ ST_INLINE float atan2_poly(float y, float x) {
 
 constexpr int kGridRes = 256;  // sun-plane candidate grid, the stratified-sampling cells
 constexpr int kGridCells = kGridRes * kGridRes;
+// Rows of the grid along e2 under Halton positions: 3^5, so that the rays whose
+// indices agree modulo 2^8 * 3^5 (one Halton cell: a in [i/256, (i+1)/256),
+// b in [j/243, (j+1)/243)) start in one grid cell. Rows 243..255 stay empty.
+constexpr int kHaltonRows = 243;
+ST_INLINE int frame_rows(const SunFrame& F) { return F.grid_rows > 0 ? F.grid_rows : kGridRes; }
 
-// Sun-plane candidate grid: kGridRes x kGridRes cells over the sun plane's
+// Sun-plane candidate grid: kGridRes x frame_rows cells over the sun plane's
 // (a, b) parameters. A primitive is listed in every cell from which a sun ray
 // can reach its traversal box: the box corners projected along the sun onto
 // the plane, widened by the largest lateral drift of a ray in the sun cone over
@@ -416,15 +423,16 @@ ST_INLINE bool grid_range(const float* __restrict__ bounds, const float4* __rest
             fp.n[1][1] = ey[0] / ly;
             fp.h[1] = fabs(fp.n[1][0] * ex[0] + fp.n[1][1] * ex[1]) + M;
             fp.cell[0] = L1 / kGridRes;
-            fp.cell[1] = L2 / kGridRes;
+            fp.cell[1] = L2 / frame_rows(F);
             fp.normals = 2;
         }
     }
     if (amax < 0.0 || amin > 1.0 || bmax < 0.0 || bmin > 1.0) return false;
     r[0] = (int)fmax(0.0, floor(amin * kGridRes));
     r[1] = (int)fmin(kGridRes - 1.0, floor(amax * kGridRes));
-    r[2] = (int)fmax(0.0, floor(bmin * kGridRes));
-    r[3] = (int)fmin(kGridRes - 1.0, floor(bmax * kGridRes));
+    const int rows = frame_rows(F);
+    r[2] = (int)fmax(0.0, floor(bmin * rows));
+    r[3] = (int)fmin(rows - 1.0, floor(bmax * rows));
     return r[0] <= r[1] && r[2] <= r[3];
 }
 

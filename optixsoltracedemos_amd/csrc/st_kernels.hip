@@ -667,8 +667,11 @@ __device__ __forceinline__ void sun_ray(const JobDev& J, const ST_CONST SunFrame
             a = u01(u.x);
             b = u01(u.y);
         }
-        // a * 256 is exact in FP32: the cell of the sampled plane coordinates
-        cell = (min((uint32_t)(b * 256.f), 255u) << 8) | min((uint32_t)(a * 256.f), 255u);
+        // the grid cell of the sampled plane coordinates (a * 256 is exact in
+        // FP32; b * rows may round up across a row boundary: the footprints'
+        // margins cover that)
+        const uint32_t rows = (uint32_t)S.grid_rows;
+        cell = (min((uint32_t)(b * (float)rows), rows - 1u) << 8) | min((uint32_t)(a * 256.f), 255u);
     }
     o = v3(fmaf(a, S.e1[0], fmaf(b, S.e2[0], S.o[0])), fmaf(a, S.e1[1], fmaf(b, S.e2[1], S.o[1])),
            fmaf(a, S.e1[2], fmaf(b, S.e2[2], S.o[2])));
@@ -858,10 +861,29 @@ __device__ __forceinline__ bool shade_mirror(const JobDev& J, uint64_t ray, int 
     return true;
 }
 
+// Residue modulo kHaltonCells = 2^8 * 3^5 of the indices whose Halton
+// positions lie in cell k of a row-by-row walk (serpentine) over the 256 x 243
+// cells: the cell's column reversed in 8 bits is the index modulo 2^8 (the
+// base-2 radical inverse reverses the low bits), its row reversed in 5 ternary
+// digits the index modulo 3^5; the Chinese remainder theorem joins the two
+// (59 = 243^-1 mod 256, 187 = 256^-1 mod 243).
+__device__ __forceinline__ uint32_t halton_residue(uint32_t k) {
+    const uint32_t cy = k >> 8, cx = (cy & 1u) ? 255u - (k & 255u) : (k & 255u);
+    const uint32_t x = __brev(cx) >> 24;
+    uint32_t y = 0, t = cy;
+    for (int d = 0; d < 5; ++d) {
+        const uint32_t q = t / 3u;
+        y = 3u * y + (t - 3u * q);
+        t = q;
+    }
+    return (x * (243u * 59u) + y * (256u * 187u)) % kHaltonCells;
+}
+
 // Ray index of slot s of a job's range; slots are traced in order. Inside the
 // range's whole blocks of kHaltonBlock aligned indices (Halton positions,
-// kOrderHalton) a block is traced cell by cell: its 64 slots 64k..64k+63 hold
-// the indices base + k + j * kHaltonCells, j < 64, which share a Halton cell.
+// kOrderHalton) a block is traced cell by cell, neighbouring cells after one
+// another: its 64 slots 64k..64k+63 hold the indices base + r + j * kHaltonCells,
+// j < 64, r = halton_residue(k), which share a Halton cell.
 // Inside whole stratified sweeps (kOrderStratified) slots 64k..64k+63 hold the
 // group whose cell is the k-th of the sweep's Hilbert curve. Elsewhere slot s
 // is index ray_begin + s. Either way the 64 slots from any multiple of 64 hold
@@ -881,8 +903,11 @@ __device__ __forceinline__ uint64_t ray_of_slot(const JobDev& J, uint64_t s) {
         if (t < (uint64_t)J.order_blocks * kHaltonBlock) {
             const uint32_t u = (uint32_t)t;  // the range lies below 2^32
             const uint32_t q = u / kHaltonBlock, r = u - q * kHaltonBlock;
-            return J.ray_begin + J.order_head + (uint64_t)q * kHaltonBlock + (uint64_t)(r & 63u) * kHaltonCells +
-                   (r >> 6);
+            // the 64 slots of a group share k: its residue on the scalar unit
+            const uint32_t k = r >> 6, k0 = __builtin_amdgcn_readfirstlane(k);
+            uint32_t res = halton_residue(k0);
+            if (k != k0) res = halton_residue(k);
+            return J.ray_begin + J.order_head + (uint64_t)q * kHaltonBlock + (uint64_t)(r & 63u) * kHaltonCells + res;
         }
     }
     return J.ray_begin + s;
@@ -1502,6 +1527,8 @@ __global__ __launch_bounds__(256) void k_sun_plane(const double* __restrict__ re
     F.sigma = (float)sun.param;
     F.shape = sun.shape;
     F.position_sampling = sun.position_sampling;
+    F.grid_rows = sun.position_sampling == ST_POS_HALTON ? kHaltonRows : kGridRes;
+    F.reserved = 0;
     out[k] = F;
 }
 

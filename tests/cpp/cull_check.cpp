@@ -340,6 +340,7 @@ int check_grid(double tan_scale, bool flat) {
             F.v2[k] = F.v1[k] + L2 * e2[k];
         }
         F.distance = D;
+        F.grid_rows = (cfg & 1) ? kHaltonRows : kGridRes;
         const double alpha = std::exp(uni(std::log(1e-3), std::log(2e-2)));
         // a flat rectangle whose centre projects inside the plane, below it
         double f[9];
@@ -392,7 +393,7 @@ int check_grid(double tan_scale, bool flat) {
             }
             const double a = dot(r, e1) / L1, b = dot(r, e2) / L2;
             if (!(a >= 0.0 && a < 1.0 && b >= 0.0 && b < 1.0)) continue;
-            const int ix = (int)(a * kGridRes), iy = (int)(b * kGridRes);
+            const int ix = (int)(a * kGridRes), iy = (int)(b * F.grid_rows);
             ++hits;
             const bool ok = any && ix >= fp.r[0] && ix <= fp.r[1] && iy >= fp.r[2] && iy <= fp.r[3] && cell_meets(fp, ix, iy);
             if (!ok) ++unlisted;
