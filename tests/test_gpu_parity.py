@@ -323,7 +323,7 @@ def test_flux_copies_cleared_between_calls(gpu_ctx):
 
 
 @pytest.mark.parametrize("case", ["field_stratified", "field_philox", "field_low_sun", "field_curved_rough",
-                                  "field_flat_rough", "dish_gaussian", "three_halton"])
+                                  "field_flat_rough", "dish_gaussian", "three_halton", "field_halton"])
 def test_candidate_lists_equal_bvh(gpu_ctx, case):
     """Hits found through the candidate lists (sun-plane cells for sun rays,
     per-mirror lists for first reflections) are exactly what a full BVH
@@ -346,6 +346,12 @@ def test_candidate_lists_equal_bvh(gpu_ctx, case):
         w.elements["zrot_deg"][m] = np.linspace(-40.0, 40.0, int(m.sum()))
         w.elements["slope_error"][m], w.elements["spec_error"][m] = 5e-3, 2e-3
         n, pos = 1_000_000, _abi.POS_PHILOX
+    elif case == "field_halton":
+        # two whole blocks of 64 * 2^8 * 3^5 indices traced Halton cell by cell
+        # (cells walked row by row, the sun-plane grid of 243 rows), then the
+        # rest in index order
+        w = scenes.heliostat_field(10_000, elevation=25.0)
+        n, pos = 8_000_000, _abi.POS_HALTON
     elif case.startswith("field"):
         w = scenes.heliostat_field(10_000, elevation=12.0 if case == "field_low_sun" else 60.0)
         n = 1_000_000
